@@ -1,0 +1,279 @@
+"""KB projectors, padded k-batch and the batched H|psi> apply (CPU oracle).
+
+Restates core/hamiltonian.py:35-191 (ProjectorData, build_projector_data,
+projectors, becp, HamiltonianK) and core/batch.py:126-235, 238-505, 590-595
+(BatchedK, build_batched, projectors_b, BatchedHamiltonian.apply / apply_cols,
+becp_b). Only the FFT local-term path is restated for the apply; the Toeplitz
+identity (batch.py:333-403) is provided as ``toeplitz_local`` for tests.
+
+Pinned by: brute-force dense H (oracle.dense) eigenvalues, the Toeplitz identity
+(test_toeplitz_local.py:61-78), Hermiticity <y,Hx> = <Hy,x>, the empty-lattice
+and Mathieu closed forms (test_exact_limits.py:39-100), and golden projector
+tables written from the reference's build_projector_data.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from oracle.grids import GSphere
+from oracle.pseudo import AnalyticSpecies, kb_form_factor
+from oracle.units import HBAR2_2M, MINUS_I_POW
+from oracle.ylm import ylm_all
+
+CDTYPE = torch.complex128
+RDTYPE = torch.float64
+
+
+def real_of(cdtype: torch.dtype) -> torch.dtype:
+    """dtypes.py:40-42."""
+    return torch.float32 if cdtype == torch.complex64 else torch.float64
+
+
+@dataclass
+class ProjectorData:
+    """core/hamiltonian.py:35-43."""
+
+    atom_index: torch.Tensor  # (nproj,)
+    f_ylm_phase_free: torch.Tensor  # (nproj, npw) complex
+    kpg: torch.Tensor  # (npw, 3)
+    dij_full: torch.Tensor  # (nproj, nproj) [eV]
+
+
+def build_projector_data(sphere: GSphere, species_of_atom: list[AnalyticSpecies],
+                         volume: float) -> ProjectorData:
+    """core/hamiltonian.py:46-105 with the analytic F_l tables (pseudo.kb_form_factor)
+    in place of the UPF spline tables (scf/loop.py:283-315)."""
+    lmax = max(max(sp.beta_ls) for sp in species_of_atom)
+    y = ylm_all(lmax, sphere.kpg)
+    q = np.sqrt(sphere.kpg2.numpy())
+    cols, atom_idx, blocks = [], [], []
+    for a, sp in enumerate(species_of_atom):
+        ls = sp.beta_ls
+        tables = torch.as_tensor(np.stack([kb_form_factor(sp, l, q) for l in ls]),
+                                 dtype=RDTYPE)
+        nchan = len(ls)
+        cols_a = []
+        for i in range(nchan):
+            l = ls[i]
+            for m_col in range(2 * l + 1):
+                f = tables[i]
+                yl = y[:, l * l + m_col]
+                pref = (4.0 * math.pi / math.sqrt(volume)) * MINUS_I_POW[l]
+                cols_a.append(pref * (f * yl).to(CDTYPE))
+                atom_idx.append(a)
+        cols += cols_a
+        nb_a = len(cols_a)
+        block = torch.zeros((nb_a, nb_a), dtype=RDTYPE)
+        offs, row = [], 0
+        for i in range(nchan):
+            offs.append(row)
+            row += 2 * ls[i] + 1
+        dsp = sp.dij
+        for i in range(nchan):
+            for j in range(nchan):
+                if ls[i] != ls[j]:
+                    continue
+                for m_col in range(2 * ls[i] + 1):
+                    block[offs[i] + m_col, offs[j] + m_col] = dsp[i, j]
+        blocks.append(block)
+    return ProjectorData(atom_index=torch.tensor(atom_idx, dtype=torch.int64),
+                         f_ylm_phase_free=torch.stack(cols, dim=0),
+                         kpg=sphere.kpg, dij_full=torch.block_diag(*blocks))
+
+
+def projectors(pd: ProjectorData, positions: torch.Tensor) -> torch.Tensor:
+    """core/hamiltonian.py:108-112 -- p (nproj, npw), differentiable in positions."""
+    phase_arg = pd.kpg @ positions.T
+    phases = torch.exp(torch.complex(torch.zeros_like(phase_arg), -phase_arg))
+    return pd.f_ylm_phase_free * phases[:, pd.atom_index].T
+
+
+def becp(p: torch.Tensor, c: torch.Tensor) -> torch.Tensor:
+    """core/hamiltonian.py:115-117."""
+    return c @ p.conj().T
+
+
+def g_to_r(coeffs: torch.Tensor, flat_idx: torch.Tensor, shape) -> torch.Tensor:
+    """core/fftbox.py:29-37 + 46-70: sphere -> box -> N ifftn."""
+    batch = coeffs.shape[:-1]
+    n = shape[0] * shape[1] * shape[2]
+    flat = torch.zeros(*batch, n, dtype=coeffs.dtype)
+    flat = flat.index_add(-1, flat_idx, coeffs)
+    return torch.fft.ifftn(flat.reshape(*batch, *shape), dim=(-3, -2, -1)) * n
+
+
+def r_to_g(f: torch.Tensor) -> torch.Tensor:
+    """core/fftbox.py:73-77."""
+    n = f.shape[-3] * f.shape[-2] * f.shape[-1]
+    return torch.fft.fftn(f, dim=(-3, -2, -1)) / n
+
+
+def box_to_sphere(box: torch.Tensor, flat_idx: torch.Tensor) -> torch.Tensor:
+    """core/fftbox.py:40-43."""
+    return box.reshape(*box.shape[:-3], -1).index_select(-1, flat_idx)
+
+
+class HamiltonianK:
+    """core/hamiltonian.py:120-191 (FFT path) -- the single-k differentiable apply
+    whose autograd gradients the hand-written adjoint must reproduce."""
+
+    def __init__(self, sphere: GSphere, shape, v_eff_r: torch.Tensor, pd: ProjectorData,
+                 p: torch.Tensor) -> None:
+        self.sphere, self.shape, self.v_eff_r, self.pd, self.p = sphere, shape, v_eff_r, pd, p
+        self.t = HBAR2_2M * sphere.kpg2
+
+    def apply(self, c: torch.Tensor) -> torch.Tensor:
+        out = self.t * c
+        psi = g_to_r(c, self.sphere.flat_idx, self.shape)
+        out = out + box_to_sphere(r_to_g(psi * self.v_eff_r), self.sphere.flat_idx)
+        if self.p.shape[0]:
+            b = becp(self.p, c)
+            out = out + (b.to(self.p.dtype) @ self.pd.dij_full.to(self.p.dtype)) @ self.p
+        return out
+
+
+@dataclass
+class BatchedK:
+    """core/batch.py:126-171."""
+
+    npw: torch.Tensor
+    mask: torch.Tensor
+    flat_idx: torch.Tensor
+    kpg: torch.Tensor
+    t: torch.Tensor
+    proj_phase_free: torch.Tensor
+    proj_atom_index: torch.Tensor
+    dij_full: torch.Tensor
+
+    @property
+    def nk(self) -> int:
+        return int(self.npw.shape[0])
+
+    @property
+    def npw_max(self) -> int:
+        return int(self.mask.shape[1])
+
+
+def build_batched(spheres: list[GSphere], proj_data: list[ProjectorData]) -> BatchedK:
+    """core/batch.py:174-205."""
+    nk = len(spheres)
+    npw = torch.tensor([s.npw for s in spheres], dtype=torch.int64)
+    m = int(npw.max())
+    mask = torch.zeros(nk, m, dtype=torch.bool)
+    flat_idx = torch.zeros(nk, m, dtype=torch.int64)
+    kpg = torch.zeros(nk, m, 3, dtype=RDTYPE)
+    t = torch.zeros(nk, m, dtype=RDTYPE)
+    nproj = proj_data[0].f_ylm_phase_free.shape[0] if proj_data else 0
+    pf = torch.zeros(nk, nproj, m, dtype=CDTYPE)
+    for ik, (s, pd) in enumerate(zip(spheres, proj_data, strict=True)):
+        n = s.npw
+        mask[ik, :n] = True
+        flat_idx[ik, :n] = s.flat_idx
+        kpg[ik, :n] = s.kpg
+        t[ik, :n] = HBAR2_2M * s.kpg2
+        if nproj:
+            pf[ik, :, :n] = pd.f_ylm_phase_free
+    return BatchedK(npw=npw, mask=mask, flat_idx=flat_idx, kpg=kpg, t=t, proj_phase_free=pf,
+                    proj_atom_index=proj_data[0].atom_index if nproj
+                    else torch.zeros(0, dtype=torch.int64),
+                    dij_full=proj_data[0].dij_full if nproj
+                    else torch.zeros((0, 0), dtype=RDTYPE))
+
+
+def projectors_b(bk: BatchedK, positions: torch.Tensor) -> torch.Tensor:
+    """core/batch.py:229-235 -- (nk, nproj, npw_max), differentiable in positions."""
+    if bk.proj_phase_free.shape[1] == 0:
+        return bk.proj_phase_free
+    phase_arg = torch.einsum("kgi,ai->kga", bk.kpg, positions)
+    phases = torch.exp(torch.complex(torch.zeros_like(phase_arg), -phase_arg))
+    return bk.proj_phase_free * phases[:, :, bk.proj_atom_index].permute(0, 2, 1)
+
+
+def becp_b(p: torch.Tensor, c: torch.Tensor, p_conj: torch.Tensor | None = None) -> torch.Tensor:
+    """core/batch.py:590-595."""
+    pc = p.conj() if p_conj is None else p_conj
+    return torch.einsum("kpg,kbg->kbp", pc, c)
+
+
+class BatchedHamiltonian:
+    """core/batch.py:238-505 (FFT local path, unchunked as on CPU, batch.py:43-44).
+
+    Padded plane-wave slots scatter into a trash slot one past the box
+    (batch.py:277-279); fftn(ifftn(.)) is norm-neutral (batch.py:422-424)."""
+
+    def __init__(self, bk: BatchedK, shape, v_eff_r: torch.Tensor, p: torch.Tensor) -> None:
+        self.bk = bk
+        self.shape = tuple(shape)
+        self.n = shape[0] * shape[1] * shape[2]
+        self.v_eff_r = v_eff_r
+        self.gather_idx = bk.flat_idx
+        self.p = p
+        self.idx_scatter = torch.where(bk.mask, bk.flat_idx, torch.full_like(bk.flat_idx, self.n))
+        self._tab_cache: dict = {}
+
+    def _tables(self, cdtype):
+        """batch.py:306-331 -- tables cast to the coefficients' precision."""
+        cached = self._tab_cache.get(cdtype)
+        if cached is None:
+            rdtype = real_of(cdtype)
+            p = self.p.to(cdtype)
+            cached = (self.bk.t.to(rdtype), self.v_eff_r.to(rdtype), p,
+                      p.conj().resolve_conj(), self.bk.dij_full.to(cdtype))
+            self._tab_cache[cdtype] = cached
+        return cached
+
+    def local_fft(self, c: torch.Tensor, v_eff: torch.Tensor) -> torch.Tensor:
+        """batch.py:405-428 -- scatter -> ifftn -> x V -> fftn -> gather."""
+        nk, nb, m = c.shape
+        box = torch.zeros(nk, nb, self.n + 1, dtype=c.dtype)
+        idx = self.idx_scatter[:, None, :].expand(nk, nb, m)
+        box.scatter_(2, idx, c)
+        psi = torch.fft.ifftn(box[..., : self.n].reshape(nk, nb, *self.shape), dim=(-3, -2, -1))
+        vg = torch.fft.fftn(psi * v_eff, dim=(-3, -2, -1)).reshape(nk, nb, self.n)
+        gath = self.gather_idx[:, None, :].expand(nk, nb, m)
+        return vg.gather(2, gath)
+
+    def apply(self, c: torch.Tensor) -> torch.Tensor:
+        """batch.py:480-505."""
+        t_r, v_eff, p, p_conj, dij = self._tables(c.dtype)
+        out = t_r[:, None, :] * c
+        out += self.local_fft(c, v_eff)
+        if p.shape[1]:
+            b = becp_b(p, c, p_conj)
+            out = out + torch.einsum("kbp,pq,kqg->kbg", b, dij, p)
+        return out * self.bk.mask[:, None, :]
+
+    def apply_cols(self, c: torch.Tensor, kcol: torch.Tensor) -> torch.Tensor:
+        """batch.py:507-547 -- H on a compacted column batch (ncol, npw_max)."""
+        t_r, v_eff, p, p_conj, dij = self._tables(c.dtype)
+        out = t_r[kcol] * c
+        ncol, m = c.shape
+        box = torch.zeros(ncol, self.n + 1, dtype=c.dtype)
+        box.scatter_(1, self.idx_scatter[kcol], c)
+        psi = torch.fft.ifftn(box[:, : self.n].reshape(ncol, *self.shape), dim=(-3, -2, -1))
+        vg = torch.fft.fftn(psi * v_eff, dim=(-3, -2, -1)).reshape(ncol, self.n)
+        out = out + vg.gather(1, self.gather_idx[kcol])
+        if p.shape[1]:
+            ps, ps_conj = p[kcol], p_conj[kcol]
+            b = torch.einsum("cpg,cg->cp", ps_conj, c)
+            out = out + torch.einsum("cp,pq,cqg->cg", b, dij, ps)
+        return out * self.bk.mask[kcol]
+
+
+def toeplitz_local(bk: BatchedK, shape, v_eff_r: torch.Tensor, c: torch.Tensor) -> torch.Tensor:
+    """batch.py:333-403 -- local V.psi as M @ c with M[i,j] = Vhat(G_i - G_j)."""
+    s1s2, s2 = shape[1] * shape[2], shape[2]
+    flat = bk.flat_idx
+    g = torch.stack([flat // s1s2, (flat % s1s2) // s2, flat % s2], dim=-1)
+    n = torch.tensor(shape)
+    diff = (g[:, :, None, :] - g[:, None, :, :]) % n
+    idx = diff[..., 0] * s1s2 + diff[..., 1] * s2 + diff[..., 2]
+    vhat = r_to_g(v_eff_r.to(c.dtype)).reshape(-1)
+    M = vhat[idx]
+    cm = c * bk.mask[:, None, :]
+    return torch.einsum("kij,kbj->kbi", M, cm) * bk.mask[:, None, :]
