@@ -1,0 +1,101 @@
+"""ctypes binding of libgradwave_hip.so (declarations in include/gradwave_hip.h).
+
+The library is the only compute path: if it is missing or cannot be loaded,
+every entry point raises. There is no CPU or PyTorch fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgradwave_hip.so")
+
+GW_FP64, GW_FP32 = 0, 1
+GW_TERM_KINETIC, GW_TERM_LOCAL, GW_TERM_NONLOCAL, GW_TERM_ALL = 1, 2, 4, 7
+GW_PLAN_FORCE_UNFUSED, GW_PLAN_TIMING, GW_PLAN_LEGACY = 1, 2, 4
+STAGES = ("plane_inv", "xpass", "plane_fwd", "becp", "dmul", "backproj")
+
+_P = C.c_void_p
+_I = C.c_int
+_L = C.c_long
+_Z = C.c_size_t
+_D = C.c_double
+
+# name -> (restype, argtypes); mirrors include/gradwave_hip.h
+SIGNATURES = {
+    "gw_last_error": (C.c_char_p, []),
+    "gw_version": (_I, []),
+    "gw_plan_create": (_I, [C.POINTER(_P), _I, _I, _I, _I, _I, _P, _P]),
+    "gw_plan_destroy": (_I, [_P]),
+    "gw_plan_set_flags": (_I, [_P, _I]),
+    "gw_plan_stage_times": (_I, [_P, _P, _P, _I, _I]),
+    "gw_plan_info": (_I, [_P, _I, _P, _I]),
+    "gw_plan_set_kinetic": (_I, [_P, _P, _P]),
+    "gw_plan_set_potential": (_I, [_P, _P, _P]),
+    "gw_plan_set_projectors": (_I, [_P, _I, _P, _P, _P]),
+    "gw_hpsi_workspace_bytes": (_Z, [_P, _I, _I]),
+    "gw_hpsi_apply": (_I, [_P, _I, _I, _P, _P, _I, _P, _Z, _P]),
+    "gw_hpsi_apply_strided": (_I, [_P, _I, _I, _P, _L, _L, _P, _L, _L, _I, _P, _Z, _P]),
+    "gw_wfc_to_real_workspace_bytes": (_Z, [_P, _I, _I]),
+    "gw_wfc_to_real": (_I, [_P, _I, _P, _P, _I, _P, _Z, _P]),
+    "gw_sphere_to_box": (_I, [_P, _I, _P, _P, _I, _P]),
+    "gw_box_to_sphere": (_I, [_P, _I, _P, _P, _I, _P]),
+    "gw_zgemm_inner_workspace_bytes": (_Z, [_I, _I, _I, _I, _I]),
+    "gw_zgemm_inner": (_I, [_I, _I, _I, _I, _I, _P, _L, _L, _P, _L, _L, _P, _L, _L, _P, _Z, _P]),
+    "gw_zgemm_combine": (_I, [_I, _I, _I, _I, _I, _P, _L, _L, _I, _P, _L, _L, _P, _L, _L, _D, _P]),
+    "gw_residual": (_I, [_I, _I, _I, _I, _P, _P, _L, _L, _P, _P, _P, _P]),
+    "gw_teter_expand": (_I, [_I, _I, _I, _I, _I, _P, _P, _L, _L, _P, _L, _P, _P, _L, _L, _P]),
+    "gw_row_norms": (_I, [_I, _I, _I, _I, _P, _L, _L, _P, _P]),
+}
+
+_lib = None
+
+
+class GWError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises GWError if it is not built or fails to load."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GWError(f"{LIB_PATH} not built: run python -m gradwave_amd.build")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().gw_last_error()
+        raise GWError(f"{what} failed (code {rc}): {msg.decode() if msg else ''}")
+
+
+def stream_ptr(device: torch.device | None = None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def ptr(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+def precision_of(dtype: torch.dtype) -> int:
+    if dtype in (torch.complex128, torch.float64):
+        return GW_FP64
+    if dtype in (torch.complex64, torch.float32):
+        return GW_FP32
+    raise TypeError(f"unsupported dtype {dtype}")
+
+
+def require_gpu(t: torch.Tensor, name: str) -> None:
+    if not t.is_cuda:
+        raise GWError(f"{name} must be a GPU tensor (gradwave_amd has no CPU path)")

@@ -1,0 +1,342 @@
+// Complex contractions of H|psi> and Davidson on MFMA (fp64 16x16x4, fp32 16x16x4).
+//
+// No complex MFMA exists, so both kernels run a real GEMM in which one operand
+// is the natural interleaved (re, im) storage and the other is expanded on the
+// fly, making the complex result come out interleaved:
+//
+//  inner   C[i][j] = sum_g conj(X[i][g]) Y[j][g]            (long K = npw)
+//          D (M x 2N) = Xr (M x 2K) . B (2K x 2N),
+//          B[kk][2j] = Yr_j[kk],  B[kk][2j+1] = (Yi, -Yr)_j[kk]
+//          split-K over the grid, fp64 partial sums reduced by a second kernel.
+//  combine Z[a][g] = beta Z + sum_j U'[j][a] Y[j][g]        (long N = npw)
+//          Zr (A x 2G) = U2 (A x 2J) . E (2J x 2G),
+//          U2[a][2j] = Re U'[j][a], U2[a][2j+1] = Im U'[j][a],
+//          E[2j][2g+c] = (Yr, Yi)[c], E[2j+1][2g+c] = (-Yi, Yr)[c]
+//
+// The reduction index of an MFMA step is free to permute as long as A and B
+// agree, so each lane loads V consecutive reals of its row (a 64-byte burst
+// for fp64) and feeds them to V consecutive MFMA steps. Operands stream from
+// global memory straight into VGPRs (no LDS staging yet). MFMA 16x16x4 lane
+// layout: A[i=l%16][k=l/16], B[k=l/16][j=l%16], D col l%16 and row Mfma<T>::row.
+#include <algorithm>
+
+#include "gw_internal.h"
+
+namespace gw {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+// Output row of accumulator register r in lane group lk = lane/16 (probed on
+// gfx950): f64 16x16x4 interleaves rows (lk + 4r), f32 16x16x4 blocks them (4lk + r).
+template <> struct Mfma<double> {
+  using acc = v4d;
+  static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+  static __device__ __forceinline__ acc op(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  using acc = v4f;
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+  static __device__ __forceinline__ acc op(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// Linear workgroup id -> work id so that consecutive work ids share an XCD
+// (dispatch places workgroup L on XCD L % 8); -1 for padding ids.
+__device__ __forceinline__ long gemm_work_id(long total) {
+  const long L = blockIdx.x;
+  const long per = (total + 7) / 8;
+  const long w = (L % 8) * per + L / 8;
+  return w < total ? w : -1;
+}
+
+constexpr int kV = 8;        // reals per lane per MFMA group (4 complex)
+constexpr int kTile = 64;    // workgroup tile (rows x real cols), 4 waves of 32x32
+constexpr int kStepC = 2 * kV;  // complex elements of K consumed per group (4 lanes-groups x V reals / 2)
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_inner(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long sx,
+             const cpx<T>* __restrict__ Y, long ldy, long sy, int splits, int kchunk,
+             double* __restrict__ part, int ntx, int nty, long total) {
+  using A4 = typename Mfma<T>::acc;
+  // work id order: (k, split) slowest, then row tile, then column tile, so the
+  // tiles of one K-chunk run together on one XCD and share its L2
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const int bx = (int)(wid % ntx), by = (int)((wid / ntx) % nty), bz = (int)(wid / ((long)ntx * nty));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int i0 = by * kTile + (w >> 1) * 32;
+  const int jj0 = bx * kTile + (w & 1) * 32;  // real column of D
+  const int k = bz / splits, sp = bz - k * splits;
+  const int gb = sp * kchunk, ge = min(K, gb + kchunk);
+  const cpx<T>* Xk = X + (long)k * sx;
+  const cpx<T>* Yk = Y + (long)k * sy;
+  A4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = A4{0, 0, 0, 0};
+  const cpx<T>* xrow[2];
+  bool xok[2];
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm) {
+    const int row = i0 + bm * 16 + li;
+    xok[bm] = row < M;
+    xrow[bm] = Xk + (long)(xok[bm] ? row : 0) * ldx;
+  }
+  const cpx<T>* yrow[2];
+  bool yok[2], yodd[2];
+#pragma unroll
+  for (int bn = 0; bn < 2; ++bn) {
+    const int jj = jj0 + bn * 16 + li;
+    const int j = jj >> 1;
+    yok[bn] = j < N;
+    yodd[bn] = jj & 1;
+    yrow[bn] = Yk + (long)(yok[bn] ? j : 0) * ldy;
+  }
+  for (int g0 = gb; g0 < ge; g0 += kStepC) {
+    const int gl = g0 + lk * (kV / 2);  // this lane's first complex element
+    T av[2][kV], bv[2][kV];
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm) {
+#pragma unroll
+      for (int v = 0; v < kV / 2; ++v) {
+        const int g = gl + v;
+        cpx<T> x = (xok[bm] && g < ge) ? xrow[bm][g] : cpx<T>{0, 0};
+        av[bm][2 * v] = x.x;
+        av[bm][2 * v + 1] = x.y;
+      }
+    }
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn) {
+#pragma unroll
+      for (int v = 0; v < kV / 2; ++v) {
+        const int g = gl + v;
+        cpx<T> y = (yok[bn] && g < ge) ? yrow[bn][g] : cpx<T>{0, 0};
+        if (yodd[bn]) { bv[bn][2 * v] = y.y; bv[bn][2 * v + 1] = -y.x; }
+        else { bv[bn][2 * v] = y.x; bv[bn][2 * v + 1] = y.y; }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < kV; ++v)
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn) acc[bm][bn] = Mfma<T>::op(av[bm][v], bv[bn][v], acc[bm][bn]);
+  }
+  const int ncol = 2 * N;
+  double* pk = part + ((long)k * splits + sp) * M * ncol;
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = i0 + bm * 16 + Mfma<T>::row(lk, r);
+        const int col = jj0 + bn * 16 + li;
+        if (row < M && col < ncol) pk[(long)row * ncol + col] = (double)acc[bm][bn][r];
+      }
+}
+
+template <typename T>
+__global__ void k_gemm_inner_reduce(int M, int N, int splits, const double* __restrict__ part,
+                                    cpx<T>* __restrict__ C, long ldc, long sc) {
+  const int k = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;  // complex index
+  const long tot = (long)M * N;
+  if (e >= tot) return;
+  const int i = (int)(e / N), j = (int)(e - (long)i * N);
+  const double* pk = part + (long)k * splits * M * 2 * N;
+  double re = 0, im = 0;
+  for (int s = 0; s < splits; ++s) {
+    const double* q = pk + (long)s * M * 2 * N + (long)i * 2 * N + 2 * j;
+    re += q[0];
+    im += q[1];
+  }
+  cpx<T> out;
+  out.x = (T)re;
+  out.y = (T)im;
+  C[(long)k * sc + (long)i * ldc + j] = out;
+}
+
+static int inner_splits(int nk, int M, int N, int K) {
+  const long tiles = (long)((2 * N + kTile - 1) / kTile) * ((M + kTile - 1) / kTile) * nk;
+  long s = (2048 + tiles - 1) / tiles;
+  const long smax = std::max<long>(1, (K + 4 * kStepC - 1) / (4 * kStepC));  // >= 4 groups per split
+  s = std::max<long>(1, std::min<long>(s, smax));
+  return (int)std::min<long>(s, 4096);
+}
+
+template <typename T>
+size_t gemm_inner_partials_bytes(int nk, int M, int N, int K) {
+  if (M <= 0 || N <= 0 || K <= 0) return 0;
+  const int s = inner_splits(nk, M, N, K);
+  return (size_t)nk * s * M * 2 * N * sizeof(double);
+}
+template size_t gemm_inner_partials_bytes<double>(int, int, int, int);
+template size_t gemm_inner_partials_bytes<float>(int, int, int, int);
+
+template <typename T>
+int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, const cpx<T>* Y, long ldy,
+               long sy, cpx<T>* C, long ldc, long sc, void* partials, size_t part_bytes, hipStream_t st) {
+  if (M <= 0 || N <= 0 || nk <= 0) return GW_OK;
+  if (K <= 0) {
+    for (int k = 0; k < nk; ++k)
+      for (int i = 0; i < M; ++i) hipMemsetAsync(C + (long)k * sc + (long)i * ldc, 0, N * sizeof(cpx<T>), st);
+    return GW_OK;
+  }
+  const int splits = inner_splits(nk, M, N, K);
+  const size_t need = (size_t)nk * splits * M * 2 * N * sizeof(double);
+  GW_REQUIRE(partials && part_bytes >= need, GW_ERR_WORKSPACE, "gemm_inner workspace %zu < %zu", part_bytes, need);
+  int kchunk = (K + splits - 1) / splits;
+  kchunk = ((kchunk + kStepC - 1) / kStepC) * kStepC;
+  const int ntx = (2 * N + kTile - 1) / kTile, nty = (M + kTile - 1) / kTile;
+  const long total = (long)ntx * nty * nk * splits;
+  hipLaunchKernelGGL(k_gemm_inner<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X, ldx,
+                     sx, Y, ldy, sy, splits, kchunk, (double*)partials, ntx, nty, total);
+  const long tot = (long)M * N;
+  hipLaunchKernelGGL(k_gemm_inner_reduce<T>, dim3((tot + 255) / 256, nk), dim3(256), 0, st, M, N, splits,
+                     (const double*)partials, C, ldc, sc);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+template int gemm_inner<double>(int, int, int, int, const double2*, long, long, const double2*, long, long,
+                                double2*, long, long, void*, size_t, hipStream_t);
+template int gemm_inner<float>(int, int, int, int, const float2*, long, long, const float2*, long, long,
+                               float2*, long, long, void*, size_t, hipStream_t);
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_combine(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu, long su, int conj_u,
+               const cpx<T>* __restrict__ Y, long ldy, long sy, cpx<T>* __restrict__ Z, long ldz, long sz,
+               T beta, int ntx, int nty, long total) {
+  using A4 = typename Mfma<T>::acc;
+  // work id order: k slowest, then column tile, then row tile, so the row tiles
+  // that re-read one block of Y run together on one XCD
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const int by = (int)(wid % nty), bx = (int)((wid / nty) % ntx), k = (int)(wid / ((long)ntx * nty));
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int a0 = by * kTile + (w >> 1) * 32;
+  const int c0 = bx * kTile + (w & 1) * 32;  // real column of Zr
+  const cpx<T>* Uk = U + (long)k * su;
+  const cpx<T>* Yk = Y + (long)k * sy;
+  A4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = A4{0, 0, 0, 0};
+  int arow[2];
+  bool aok[2];
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm) { arow[bm] = a0 + bm * 16 + li; aok[bm] = arow[bm] < A; }
+  int gcol[2];
+  bool gok[2], gim[2];
+#pragma unroll
+  for (int bn = 0; bn < 2; ++bn) {
+    const int col = c0 + bn * 16 + li;
+    gcol[bn] = col >> 1;
+    gim[bn] = col & 1;
+    gok[bn] = gcol[bn] < G;
+  }
+  const T usign = conj_u ? T(-1) : T(1);
+  for (int j0 = 0; j0 < J; j0 += kStepC) {
+    const int jl = j0 + lk * (kV / 2);
+    T av[2][kV], bv[2][kV];
+#pragma unroll
+    for (int v = 0; v < kV / 2; ++v) {
+      const int j = jl + v;
+      const bool jok = j < J;
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm) {
+        cpx<T> u = (jok && aok[bm]) ? Uk[(long)j * ldu + arow[bm]] : cpx<T>{0, 0};
+        av[bm][2 * v] = u.x;
+        av[bm][2 * v + 1] = usign * u.y;
+      }
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn) {
+        cpx<T> y = (jok && gok[bn]) ? Yk[(long)j * ldy + gcol[bn]] : cpx<T>{0, 0};
+        if (gim[bn]) { bv[bn][2 * v] = y.y; bv[bn][2 * v + 1] = y.x; }
+        else { bv[bn][2 * v] = y.x; bv[bn][2 * v + 1] = -y.y; }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < kV; ++v)
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn) acc[bm][bn] = Mfma<T>::op(av[bm][v], bv[bn][v], acc[bm][bn]);
+  }
+  const int ncol = 2 * G;
+  T* Zk = reinterpret_cast<T*>(Z + (long)k * sz);
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = a0 + bm * 16 + Mfma<T>::row(lk, r);
+        const int col = c0 + bn * 16 + li;
+        if (row < A && col < ncol) {
+          T* zp = Zk + (long)row * 2 * ldz + col;
+          const T v = acc[bm][bn][r];
+          *zp = (beta != T(0)) ? v + beta * (*zp) : v;
+        }
+      }
+}
+
+template <typename T>
+int gemm_combine(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su, int conj_u, const cpx<T>* Y,
+                 long ldy, long sy, cpx<T>* Z, long ldz, long sz, T beta, hipStream_t st) {
+  if (A <= 0 || G <= 0 || nk <= 0) return GW_OK;
+  const int ntx = (2 * G + kTile - 1) / kTile, nty = (A + kTile - 1) / kTile;
+  const long total = (long)ntx * nty * nk;
+  hipLaunchKernelGGL(k_gemm_combine<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, A, G, J, U, ldu,
+                     su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, ntx, nty, total);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+template int gemm_combine<double>(int, int, int, int, const double2*, long, long, int, const double2*, long, long,
+                                  double2*, long, long, double, hipStream_t);
+template int gemm_combine<float>(int, int, int, int, const float2*, long, long, int, const float2*, long, long,
+                                 float2*, long, long, float, hipStream_t);
+
+}  // namespace gw
+
+using namespace gw;
+
+extern "C" size_t gw_zgemm_inner_workspace_bytes(int precision, int nk, int M, int N, int K) {
+  return precision == GW_FP64 ? gemm_inner_partials_bytes<double>(nk, M, N, K)
+                              : gemm_inner_partials_bytes<float>(nk, M, N, K);
+}
+
+extern "C" int gw_zgemm_inner(int precision, int nk, int M, int N, int K, const void* X, long ldx, long sx,
+                              const void* Y, long ldy, long sy, void* C, long ldc, long sc, void* ws,
+                              size_t ws_bytes, void* stream) {
+  GW_REQUIRE(X && Y && C && nk >= 0 && M >= 0 && N >= 0 && K >= 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(ldx >= K && ldy >= K && ldc >= N, GW_ERR_ARG, "leading dimension too small");
+  if (precision == GW_FP64)
+    return gemm_inner<double>(nk, M, N, K, (const double2*)X, ldx, sx, (const double2*)Y, ldy, sy, (double2*)C,
+                              ldc, sc, ws, ws_bytes, (hipStream_t)stream);
+  return gemm_inner<float>(nk, M, N, K, (const float2*)X, ldx, sx, (const float2*)Y, ldy, sy, (float2*)C, ldc,
+                           sc, ws, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int gw_zgemm_combine(int precision, int nk, int A, int G, int J, const void* U, long ldu, long su,
+                                int conj_u, const void* Y, long ldy, long sy, void* Z, long ldz, long sz,
+                                double beta, void* stream) {
+  GW_REQUIRE(U && Y && Z && nk >= 0 && A >= 0 && G >= 0 && J >= 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(ldu >= A && ldy >= G && ldz >= G, GW_ERR_ARG, "leading dimension too small");
+  if (precision == GW_FP64)
+    return gemm_combine<double>(nk, A, G, J, (const double2*)U, ldu, su, conj_u, (const double2*)Y, ldy, sy,
+                                (double2*)Z, ldz, sz, beta, (hipStream_t)stream);
+  return gemm_combine<float>(nk, A, G, J, (const float2*)U, ldu, su, conj_u, (const float2*)Y, ldy, sy,
+                             (float2*)Z, ldz, sz, (float)beta, (hipStream_t)stream);
+}

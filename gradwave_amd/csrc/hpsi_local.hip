@@ -1,0 +1,1342 @@
+// Local + kinetic terms of H|psi> on the FFT box, and the operator plan.
+//
+// Reference contract: core/batch.py:405-428 (_local_fft_into) and 480-505
+// (apply): out = t*c + gather(fftn(V * ifftn(scatter(c)))) per band, mask kept.
+//
+// MI355X design. The sphere occupies ~45% of the x-planes and ~17% of the
+// (x, y) sticks of the box, so the 3-D transforms are pruned:
+//   P1  per (x-plane, band): scatter the plane's sphere points into an LDS
+//       plane tile, inverse z-FFT on the non-empty sticks only, inverse y-FFT
+//       on all columns, store the plane (planes outside the sphere's x-extent
+//       are never materialised).
+//   P2  per (yz-line chunk, band group): inverse x-FFT, multiply by V(r) held
+//       in registers across the band group, forward x-FFT; only the sphere's
+//       x-planes are read back and written.
+//   P3  per (x-plane, band): forward y-FFT on all columns, forward z-FFT on
+//       the sticks, gather the sphere points, add t*c, write H c.
+// Each 1-D transform leaves digit-permuted order and the matching forward
+// transform consumes it (gw_fft.h), so V(r) is stored pre-permuted and no
+// reordering pass exists. When a plane tile does not fit in LDS (fp64 boxes
+// above ~95x95 per plane) P1/P3 split into a stick kernel (z) and a
+// column-chunk plane kernel (y) that exchange z-transformed sticks through HBM.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <vector>
+
+#include "gw_internal.h"
+#include "gw_regfft.h"
+
+namespace gw {
+
+static thread_local char g_err[1024] = "";
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+bool split_length(int n, int* a, int* b) {
+  // most balanced split with both factors in the codelet set
+  int best_a = -1, best_b = -1;
+  for (int x = 1; x <= 16; ++x) {
+    if (n % x) continue;
+    int y = n / x;
+    if (!codelet_ok(x) || !codelet_ok(y)) continue;
+    if (best_a < 0 || std::abs(x - y) < std::abs(best_a - best_b) ||
+        (std::abs(x - y) == std::abs(best_a - best_b) && x > best_a)) {
+      best_a = x;
+      best_b = y;
+    }
+  }
+  if (best_a < 0) return false;
+  *a = best_a;
+  *b = best_b;
+  return true;
+}
+
+constexpr int kMaxLds = 160 * 1024;
+constexpr int kPlaneThreads = 512;
+constexpr int kXThreads = 256;
+constexpr int kXVpt = 16;      // V values cached per thread in the x pass
+constexpr int kXBandGroup = 4; // bands per x-pass workgroup (V reuse)
+constexpr int kStickChunk = 8; // sticks per workgroup in the unfused z pass
+constexpr int kColChunk = 32;  // columns per workgroup in the unfused y pass
+
+static size_t csize(int precision) { return precision == GW_FP64 ? 16 : 8; }
+
+bool plane_fits_lds(const Plan* p, int precision) {
+  if (p->flags & GW_PLAN_FORCE_UNFUSED) return false;
+  return (size_t)p->n[1] * p->n[2] * csize(precision) <= (size_t)std::min(p->lds_max, kMaxLds);
+}
+
+static int x_lines(const Plan* p) {
+  int n23 = p->n[1] * p->n[2];
+  int L = (kXVpt * kXThreads) / p->n[0];
+  return std::max(1, std::min(L, n23));
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+ApplyLayout apply_layout(const Plan* p, int nb, int precision) {
+  ApplyLayout L{};
+  const size_t e = csize(precision);
+  const size_t nbt = (size_t)p->nk * nb;
+  L.fused = plane_fits_lds(p, precision);
+  size_t off = 0;
+  L.box_off = off;
+  L.box_bytes = nbt * p->na * p->n[1] * p->n[2] * e;
+  off = align256(off + L.box_bytes);
+  L.stick_off = off;
+  L.stick_bytes = L.fused ? 0 : nbt * p->nst_max * p->n[2] * e;
+  off = align256(off + L.stick_bytes);
+  L.becp_off = off;
+  L.becp_bytes = (size_t)p->nk * p->nproj * nb * e;
+  off = align256(off + L.becp_bytes);
+  L.u_off = off;
+  L.u_bytes = L.becp_bytes;
+  off = align256(off + L.u_bytes);
+  L.part_off = off;
+  L.part_bytes = 0;
+  if (p->nproj) {
+    L.part_bytes = precision == GW_FP64
+                       ? gemm_inner_partials_bytes<double>(p->nk, p->nproj, nb, p->npw_max)
+                       : gemm_inner_partials_bytes<float>(p->nk, p->nproj, nb, p->npw_max);
+  }
+  off = align256(off + L.part_bytes);
+  L.total = off;
+  return L;
+}
+
+// ------------------------------------------------------------------ kernels
+
+struct Geom {
+  int n1, n2, n3, na, nst_max, npw_max, nb;
+  int xlo, xhi;  // active x-planes = [0, xlo) U [xhi, n1), or xlo = -1
+  int st_max_plane;
+  int fx0, fx1, fy0, fy1, fz0, fz1;
+  const int* planes;
+  const int* st_plane_ptr;
+  const int* st_i1;
+  const int* st_pt_ptr;
+  const int* pt_src;
+  const int* pt_yz;
+  const int* npw;
+};
+
+static Geom geom_of(const Plan* p, int nb) {
+  Geom g;
+  g.n1 = p->n[0]; g.n2 = p->n[1]; g.n3 = p->n[2];
+  g.na = p->na; g.nst_max = p->nst_max; g.npw_max = p->npw_max; g.nb = nb;
+  g.xlo = p->xlo; g.xhi = p->xhi; g.st_max_plane = std::max(1, p->st_max_plane);
+  g.fx0 = p->f[0][0]; g.fx1 = p->f[0][1];
+  g.fy0 = p->f[1][0]; g.fy1 = p->f[1][1];
+  g.fz0 = p->f[2][0]; g.fz1 = p->f[2][1];
+  g.planes = p->planes; g.st_plane_ptr = p->st_plane_ptr; g.st_i1 = p->st_i1;
+  g.st_pt_ptr = p->st_pt_ptr; g.pt_src = p->pt_src; g.pt_yz = p->pt_yz; g.npw = p->npw;
+  return g;
+}
+
+template <typename T> __device__ __forceinline__ cpx<T> czero() { cpx<T> z; z.x = 0; z.y = 0; return z; }
+
+// P1 fused: scatter -> inverse z (sticks) -> inverse y (all columns) -> store plane.
+template <typename T>
+__global__ void __launch_bounds__(kPlaneThreads)
+k_plane_inv(Geom g, const cpx<T>* __restrict__ c, long ldc, long sc, cpx<T>* __restrict__ box,
+            const cpx<T>* __restrict__ twy, const cpx<T>* __restrict__ twz) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
+  const int a = blockIdx.x, bi = blockIdx.y;
+  const int k = bi / g.nb, b = bi - k * g.nb;
+  const int n23 = g.n2 * g.n3;
+  for (int e = threadIdx.x; e < n23; e += blockDim.x) tile[e] = czero<T>();
+  const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
+  const int s0 = sp[a], s1 = sp[a + 1];
+  const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
+  const int p0 = pp[s0], p1 = pp[s1];
+  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  const int* src = g.pt_src + (long)k * g.npw_max;
+  const int* yz = g.pt_yz + (long)k * g.npw_max;
+  __syncthreads();
+  for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) tile[yz[q]] = cb[src[q]];
+  __syncthreads();
+  if (s1 > s0) {
+    line_fft_inverse<T, false>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1,
+                               g.st_i1 + (long)k * g.nst_max + s0);
+    __syncthreads();
+    line_fft_inverse<T, true>(tile, twy, g.fy0, g.fy1, g.n3, 1, g.n3, nullptr);
+    __syncthreads();
+  }
+  cpx<T>* dst = box + ((long)bi * g.na + a) * n23;
+  for (int e = threadIdx.x; e < n23; e += blockDim.x) dst[e] = tile[e];
+}
+
+// P2: inverse x-FFT, x V(r), forward x-FFT over a chunk of L yz-lines.
+template <typename T>
+__global__ void __launch_bounds__(kXThreads)
+k_xpass(Geom g, int L, int nbt, cpx<T>* __restrict__ box, const T* __restrict__ vp,
+        const cpx<T>* __restrict__ twx) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);  // [n1][L]
+  const int n23 = g.n2 * g.n3;
+  const int j0 = blockIdx.x * L;
+  const int nl = min(L, n23 - j0);
+  const int ne = g.n1 * L;
+  T vr[kXVpt];
+#pragma unroll
+  for (int i = 0; i < kXVpt; ++i) {
+    const int e = threadIdx.x + i * blockDim.x;
+    T v = 0;
+    if (e < ne) {
+      const int px = e / L, l = e - px * L;
+      if (l < nl) v = vp[(long)px * n23 + j0 + l];
+    }
+    vr[i] = v;
+  }
+  for (int bg = 0; bg < kXBandGroup; ++bg) {
+    const int bi = blockIdx.y * kXBandGroup + bg;
+    if (bi >= nbt) break;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) tile[e] = czero<T>();
+    __syncthreads();
+    cpx<T>* bb = box + (long)bi * g.na * n23 + j0;
+    for (int e = threadIdx.x; e < g.na * L; e += blockDim.x) {
+      const int a = e / L, l = e - a * L;
+      if (l < nl) tile[g.planes[a] * L + l] = bb[(long)a * n23 + l];
+    }
+    __syncthreads();
+    line_fft_inverse<T, true>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kXVpt; ++i) {
+      const int e = threadIdx.x + i * blockDim.x;
+      if (e < ne) { tile[e].x *= vr[i]; tile[e].y *= vr[i]; }
+    }
+    __syncthreads();
+    line_fft_forward<T, true>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
+    __syncthreads();
+    for (int e = threadIdx.x; e < g.na * L; e += blockDim.x) {
+      const int a = e / L, l = e - a * L;
+      if (l < nl) bb[(long)a * n23 + l] = tile[g.planes[a] * L + l];
+    }
+    __syncthreads();
+  }
+}
+
+// P2 with wave-cooperative lines (gw_regfft.h): each x-line is spread over S
+// lanes, the transpose between the two DFT passes stays inside the wave's own
+// LDS area, V(r) (natural x order) is held in registers across the band group,
+// and only the sphere's x-planes are loaded and stored.
+template <typename T, int N, int M, int S>
+__global__ void __launch_bounds__(256)
+k_xpass_wave(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __restrict__ vx,
+             const cpx<T>* __restrict__ twx) {
+  using W = WaveLine<T, N, M, S>;
+  constexpr int LW = W::LW;
+  __shared__ cpx<T> E[4 * W::E_PER_WAVE];
+  __shared__ cpx<T> tw[N];
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twx[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = lane / LW, l = lane - t * LW;
+  const bool part = t < S;
+  const int n23 = g.n2 * g.n3;
+  const int j = (blockIdx.x * 4 + wave) * LW + l;
+  const bool jok = j < n23;
+  cpx<T>* Ew = E + wave * W::E_PER_WAVE;
+  // the sphere's x-planes are [0, xlo) U [xhi, N)
+  auto slot = [&](int i0) { return i0 < g.xlo ? i0 : (i0 >= g.xhi ? g.xlo + i0 - g.xhi : -1); };
+  T vr[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) vr[s] = (t < M && jok) ? vx[(long)(t + M * s) * n23 + j] : T(0);
+  for (int bg = 0; bg < bgroup; ++bg) {
+    const int bi = blockIdx.y * bgroup + bg;
+    if (bi >= nbt) break;
+    cpx<T>* bb = box + (long)bi * g.na * n23 + j;
+    cpx<T> x[M], y[S];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int sl = part ? slot(S * q + t) : -1;
+      x[q] = (jok && sl >= 0) ? bb[(long)sl * n23] : czero<T>();
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) y[s] = czero<T>();
+    W::inverse(x, y, t, l, part, Ew, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) { y[s].x *= vr[s]; y[s].y *= vr[s]; }
+    W::forward(y, x, t, l, part, Ew, tw);
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int sl = part ? slot(S * q + t) : -1;
+      if (jok && sl >= 0) bb[(long)sl * n23] = x[q];
+    }
+  }
+}
+
+// P3 fused: load plane -> forward y (all columns) -> forward z (sticks) -> gather (+ t c).
+template <typename T, bool KIN>
+__global__ void __launch_bounds__(kPlaneThreads)
+k_plane_fwd(Geom g, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ c, long ldc,
+            long sc, const T* __restrict__ t, cpx<T>* __restrict__ out, long ldo, long so,
+            T scale, const cpx<T>* __restrict__ twy, const cpx<T>* __restrict__ twz) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
+  const int a = blockIdx.x, bi = blockIdx.y;
+  const int k = bi / g.nb, b = bi - k * g.nb;
+  const int n23 = g.n2 * g.n3;
+  const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
+  const int s0 = sp[a], s1 = sp[a + 1];
+  const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
+  const int p0 = pp[s0], p1 = pp[s1];
+  cpx<T>* ob = out + (long)k * so + (long)b * ldo;
+  if (a == 0) {  // padded slots of this band are zero (reference: out * mask)
+    for (int q = g.npw[k] + threadIdx.x; q < g.npw_max; q += blockDim.x) ob[q] = czero<T>();
+  }
+  if (s1 == s0) return;
+  const cpx<T>* srcb = box + ((long)bi * g.na + a) * n23;
+  for (int e = threadIdx.x; e < n23; e += blockDim.x) tile[e] = srcb[e];
+  __syncthreads();
+  line_fft_forward<T, true>(tile, twy, g.fy0, g.fy1, g.n3, 1, g.n3, nullptr);
+  __syncthreads();
+  line_fft_forward<T, false>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1,
+                             g.st_i1 + (long)k * g.nst_max + s0);
+  __syncthreads();
+  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  const int* src = g.pt_src + (long)k * g.npw_max;
+  const int* yz = g.pt_yz + (long)k * g.npw_max;
+  const T* tk = t + (long)k * g.npw_max;
+  for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
+    const int s = src[q];
+    cpx<T> v = tile[yz[q]];
+    v.x *= scale; v.y *= scale;
+    if constexpr (KIN) {
+      const cpx<T> cv = cb[s];
+      const T ts = tk[s];
+      v.x += ts * cv.x; v.y += ts * cv.y;
+    }
+    ob[s] = v;
+  }
+}
+
+// ---- unfused path (plane tile larger than LDS) ----
+
+// z inverse on a chunk of sticks: scatter points into stick rows, transform, store.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_sticks_inv(Geom g, const cpx<T>* __restrict__ c, long ldc, long sc, cpx<T>* __restrict__ sticks,
+             const cpx<T>* __restrict__ twz) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);  // [kStickChunk][n3]
+  const int bi = blockIdx.y;
+  const int k = bi / g.nb, b = bi - k * g.nb;
+  const int nst = g.st_plane_ptr[(long)k * (g.na + 1) + g.na];
+  const int s0 = blockIdx.x * kStickChunk;
+  if (s0 >= nst) return;
+  const int s1 = min(s0 + kStickChunk, nst);
+  for (int e = threadIdx.x; e < kStickChunk * g.n3; e += blockDim.x) tile[e] = czero<T>();
+  __syncthreads();
+  const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
+  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  const int* src = g.pt_src + (long)k * g.npw_max;
+  const int* yz = g.pt_yz + (long)k * g.npw_max;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  for (int s = s0 + wave; s < s1; s += nw)
+    for (int q = pp[s] + lane; q < pp[s + 1]; q += 64)
+      tile[(s - s0) * g.n3 + yz[q] % g.n3] = cb[src[q]];
+  __syncthreads();
+  line_fft_inverse<T, false>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1, nullptr);
+  __syncthreads();
+  cpx<T>* dst = sticks + ((long)bi * g.nst_max + s0) * g.n3;
+  for (int e = threadIdx.x; e < (s1 - s0) * g.n3; e += blockDim.x) dst[e] = tile[e];
+}
+
+// y inverse on a column chunk of one plane, reading z-transformed sticks.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_plane_inv_y(Geom g, const cpx<T>* __restrict__ sticks, cpx<T>* __restrict__ box,
+              const cpx<T>* __restrict__ twy) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);  // [n2][kColChunk]
+  const int a = blockIdx.x, bi = blockIdx.y, c0 = blockIdx.z * kColChunk;
+  const int k = bi / g.nb;
+  const int cw = min(kColChunk, g.n3 - c0);
+  for (int e = threadIdx.x; e < g.n2 * kColChunk; e += blockDim.x) tile[e] = czero<T>();
+  __syncthreads();
+  const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
+  const int s0 = sp[a], s1 = sp[a + 1];
+  const int* i1s = g.st_i1 + (long)k * g.nst_max;
+  for (int e = threadIdx.x; e < (s1 - s0) * kColChunk; e += blockDim.x) {
+    const int s = s0 + e / kColChunk, cc = e % kColChunk;
+    if (cc < cw) tile[i1s[s] * kColChunk + cc] = sticks[((long)bi * g.nst_max + s) * g.n3 + c0 + cc];
+  }
+  __syncthreads();
+  if (s1 > s0) {
+    line_fft_inverse<T, true>(tile, twy, g.fy0, g.fy1, kColChunk, 1, kColChunk, nullptr);
+    __syncthreads();
+  }
+  cpx<T>* dst = box + ((long)bi * g.na + a) * g.n2 * g.n3 + c0;
+  for (int e = threadIdx.x; e < g.n2 * kColChunk; e += blockDim.x) {
+    const int i1 = e / kColChunk, cc = e % kColChunk;
+    if (cc < cw) dst[(long)i1 * g.n3 + cc] = tile[e];
+  }
+}
+
+// y forward on a column chunk of one plane, writing the sticks' columns.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_plane_fwd_y(Geom g, const cpx<T>* __restrict__ box, cpx<T>* __restrict__ sticks,
+              const cpx<T>* __restrict__ twy) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
+  const int a = blockIdx.x, bi = blockIdx.y, c0 = blockIdx.z * kColChunk;
+  const int k = bi / g.nb;
+  const int cw = min(kColChunk, g.n3 - c0);
+  const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
+  const int s0 = sp[a], s1 = sp[a + 1];
+  if (s1 == s0) return;
+  const cpx<T>* srcb = box + ((long)bi * g.na + a) * g.n2 * g.n3 + c0;
+  for (int e = threadIdx.x; e < g.n2 * kColChunk; e += blockDim.x) {
+    const int i1 = e / kColChunk, cc = e % kColChunk;
+    tile[e] = cc < cw ? srcb[(long)i1 * g.n3 + cc] : czero<T>();
+  }
+  __syncthreads();
+  line_fft_forward<T, true>(tile, twy, g.fy0, g.fy1, kColChunk, 1, kColChunk, nullptr);
+  __syncthreads();
+  const int* i1s = g.st_i1 + (long)k * g.nst_max;
+  for (int e = threadIdx.x; e < (s1 - s0) * kColChunk; e += blockDim.x) {
+    const int s = s0 + e / kColChunk, cc = e % kColChunk;
+    if (cc < cw) sticks[((long)bi * g.nst_max + s) * g.n3 + c0 + cc] = tile[i1s[s] * kColChunk + cc];
+  }
+}
+
+// z forward on a chunk of sticks and gather of the sphere points (+ t c).
+template <typename T, bool KIN>
+__global__ void __launch_bounds__(256)
+k_sticks_fwd(Geom g, const cpx<T>* __restrict__ sticks, const cpx<T>* __restrict__ c, long ldc,
+             long sc, const T* __restrict__ t, cpx<T>* __restrict__ out, long ldo, long so,
+             T scale, const cpx<T>* __restrict__ twz) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
+  const int bi = blockIdx.y;
+  const int k = bi / g.nb, b = bi - k * g.nb;
+  cpx<T>* ob = out + (long)k * so + (long)b * ldo;
+  if (blockIdx.x == 0) {
+    for (int q = g.npw[k] + threadIdx.x; q < g.npw_max; q += blockDim.x) ob[q] = czero<T>();
+  }
+  const int nst = g.st_plane_ptr[(long)k * (g.na + 1) + g.na];
+  const int s0 = blockIdx.x * kStickChunk;
+  if (s0 >= nst) return;
+  const int s1 = min(s0 + kStickChunk, nst);
+  const cpx<T>* srcb = sticks + ((long)bi * g.nst_max + s0) * g.n3;
+  for (int e = threadIdx.x; e < (s1 - s0) * g.n3; e += blockDim.x) tile[e] = srcb[e];
+  __syncthreads();
+  line_fft_forward<T, false>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1, nullptr);
+  __syncthreads();
+  const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
+  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  const int* src = g.pt_src + (long)k * g.npw_max;
+  const int* yz = g.pt_yz + (long)k * g.npw_max;
+  const T* tk = t + (long)k * g.npw_max;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  for (int s = s0 + wave; s < s1; s += nw)
+    for (int q = pp[s] + lane; q < pp[s + 1]; q += 64) {
+      const int si = src[q];
+      cpx<T> v = tile[(s - s0) * g.n3 + yz[q] % g.n3];
+      v.x *= scale; v.y *= scale;
+      if constexpr (KIN) {
+        const cpx<T> cv = cb[si];
+        v.x += tk[si] * cv.x; v.y += tk[si] * cv.y;
+      }
+      ob[si] = v;
+    }
+}
+
+// kinetic only: out = t*c (0 in padding)
+template <typename T>
+__global__ void k_kinetic(int nb, int npw_max, const int* __restrict__ npw, const cpx<T>* __restrict__ c,
+                          long ldc, long sc, const T* __restrict__ t, cpx<T>* __restrict__ out,
+                          long ldo, long so) {
+  const int bi = blockIdx.y, k = bi / nb, b = bi - k * nb;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npw_max) return;
+  cpx<T> v = czero<T>();
+  if (q < npw[k]) {
+    const cpx<T> cv = c[(long)k * sc + (long)b * ldc + q];
+    const T tv = t[(long)k * npw_max + q];
+    v.x = tv * cv.x; v.y = tv * cv.y;
+  }
+  out[(long)k * so + (long)b * ldo + q] = v;
+}
+
+// U[k][q][b] = sum_p D[p][q] * becp[k][p][b]   (D sparse, CSR by column q)
+template <typename T>
+__global__ void k_dmul(int nproj, int nb, const int* __restrict__ colptr, const int* __restrict__ row,
+                       const double* __restrict__ val, const cpx<T>* __restrict__ becp,
+                       cpx<T>* __restrict__ u) {
+  const int k = blockIdx.y;
+  const long tot = (long)nproj * nb;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= tot) return;
+  const int q = (int)(e / nb), b = (int)(e - (long)q * nb);
+  const cpx<T>* bk = becp + (long)k * tot;
+  cpx<T> acc = czero<T>();
+  for (int i = colptr[q]; i < colptr[q + 1]; ++i) {
+    const T d = (T)val[i];
+    const cpx<T> v = bk[(long)row[i] * nb + b];
+    acc.x += d * v.x; acc.y += d * v.y;
+  }
+  u[(long)k * tot + e] = acc;
+}
+
+// ---- wave-line path: natural order on every axis, cubic y/z ----
+
+// Linear workgroup id -> work id so that consecutive work ids share an XCD
+// (dispatch places workgroup L on XCD L % 8). Returns -1 for the padding ids.
+__device__ __forceinline__ long xcd_work_id(long total) {
+  const long L = blockIdx.x;
+  const long per = (total + 7) / 8;
+  const long w = (L % 8) * per + L / 8;
+  return w < total ? w : -1;
+}
+
+// P1: scatter -> z inverse on compact stick rows (LDS) -> y inverse on columns,
+// natural-order rows stored straight to the box.
+template <typename T, int N, int M, int S>
+__global__ void __launch_bounds__(256)
+k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long sc,
+              cpx<T>* __restrict__ box, const cpx<T>* __restrict__ twg) {
+  using W = WaveLine<T, N, M, S>;
+  constexpr int LW = W::LW;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* E = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* tw = E + 4 * W::E_PER_WAVE;
+  cpx<T>* rows = tw + N;
+  int* map = reinterpret_cast<int*>(rows + (long)g.st_max_plane * N);
+  const long wid = xcd_work_id(total);
+  if (wid < 0) return;
+  const int a = (int)(wid % g.na), bi = (int)(wid / g.na);
+  const int k = bi / g.nb, b = bi - k * g.nb;
+  const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
+  const int s0 = sp[a], s1 = sp[a + 1], ns = s1 - s0;
+  const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
+  const int* i1s = g.st_i1 + (long)k * g.nst_max;
+  const int* src = g.pt_src + (long)k * g.npw_max;
+  const int* yz = g.pt_yz + (long)k * g.npw_max;
+  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) { tw[i] = twg[i]; map[i] = -1; }
+  for (int e = threadIdx.x; e < ns * N; e += blockDim.x) rows[e] = czero<T>();
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) map[i1s[s]] = s - s0;
+  for (int s = s0 + wave; s < s1; s += 4)
+    for (int q = pp[s] + lane; q < pp[s + 1]; q += 64) rows[(s - s0) * N + yz[q] % N] = cb[src[q]];
+  __syncthreads();
+  const int t = lane / LW, l = lane - t * LW;
+  const bool part = t < S;
+  cpx<T>* Ew = E + wave * W::E_PER_WAVE;
+  cpx<T> x[M], y[S];
+  // z inverse on the plane's sticks (rows of `rows`)
+  for (int r0 = wave * LW; r0 < ns; r0 += 4 * LW) {
+    const int row = r0 + l;
+    const bool ok = part && row < ns;
+#pragma unroll
+    for (int q = 0; q < M; ++q) x[q] = ok ? rows[row * N + S * q + t] : czero<T>();
+#pragma unroll
+    for (int u = 0; u < S; ++u) y[u] = czero<T>();
+    W::inverse(x, y, t, l, part, Ew, tw);
+    if (t < M && row < ns) {
+#pragma unroll
+      for (int u = 0; u < S; ++u) rows[row * N + t + M * u] = y[u];
+    }
+  }
+  __syncthreads();
+  // y inverse on all columns; column input is non-zero only at stick rows
+  cpx<T>* dst = box + ((long)bi * g.na + a) * N * N;
+  for (int c0 = wave * LW; c0 < N; c0 += 4 * LW) {
+    const int col = c0 + l;
+    const bool ok = part && col < N;
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int st = ok ? map[S * q + t] : -1;
+      x[q] = st >= 0 ? rows[st * N + col] : czero<T>();
+    }
+#pragma unroll
+    for (int u = 0; u < S; ++u) y[u] = czero<T>();
+    W::inverse(x, y, t, l, part, Ew, tw);
+    if (t < M && col < N) {
+#pragma unroll
+      for (int u = 0; u < S; ++u) dst[(long)(t + M * u) * N + col] = y[u];
+    }
+  }
+}
+
+// P3: y forward on columns from the box -> stick rows (LDS) -> z forward -> gather (+ t c).
+template <typename T, int N, int M, int S, bool KIN>
+__global__ void __launch_bounds__(256)
+k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ c, long ldc,
+              long sc, const T* __restrict__ tk_all, cpx<T>* __restrict__ out, long ldo, long so, T scale,
+              const cpx<T>* __restrict__ twg) {
+  using W = WaveLine<T, N, M, S>;
+  constexpr int LW = W::LW;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* E = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* tw = E + 4 * W::E_PER_WAVE;
+  cpx<T>* rows = tw + N;
+  int* map = reinterpret_cast<int*>(rows + (long)g.st_max_plane * N);
+  const long wid = xcd_work_id(total);
+  if (wid < 0) return;
+  const int a = (int)(wid % g.na), bi = (int)(wid / g.na);
+  const int k = bi / g.nb, b = bi - k * g.nb;
+  cpx<T>* ob = out + (long)k * so + (long)b * ldo;
+  if (a == 0) {  // padded slots of this band are zero (reference: out * mask)
+    for (int q = g.npw[k] + threadIdx.x; q < g.npw_max; q += blockDim.x) ob[q] = czero<T>();
+  }
+  const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
+  const int s0 = sp[a], s1 = sp[a + 1], ns = s1 - s0;
+  if (ns == 0) return;
+  const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
+  const int* i1s = g.st_i1 + (long)k * g.nst_max;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) { tw[i] = twg[i]; map[i] = -1; }
+  __syncthreads();
+  for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) map[i1s[s]] = s - s0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = lane / LW, l = lane - t * LW;
+  const bool part = t < S;
+  cpx<T>* Ew = E + wave * W::E_PER_WAVE;
+  cpx<T> x[M], y[S];
+  const cpx<T>* srcp = box + ((long)bi * g.na + a) * N * N;
+  for (int c0 = wave * LW; c0 < N; c0 += 4 * LW) {
+    const int col = c0 + l;
+    const bool ok = col < N;
+#pragma unroll
+    for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? srcp[(long)(t + M * u) * N + col] : czero<T>();
+    W::forward(y, x, t, l, part, Ew, tw);
+    if (ok && part) {
+#pragma unroll
+      for (int q = 0; q < M; ++q) {
+        const int st = map[S * q + t];
+        if (st >= 0) rows[st * N + col] = x[q];
+      }
+    }
+  }
+  __syncthreads();
+  for (int r0 = wave * LW; r0 < ns; r0 += 4 * LW) {
+    const int row = r0 + l;
+    const bool ok = row < ns;
+#pragma unroll
+    for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? rows[row * N + t + M * u] : czero<T>();
+    W::forward(y, x, t, l, part, Ew, tw);
+    if (ok && part) {
+#pragma unroll
+      for (int q = 0; q < M; ++q) rows[row * N + S * q + t] = x[q];
+    }
+  }
+  __syncthreads();
+  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  const int* src = g.pt_src + (long)k * g.npw_max;
+  const int* yz = g.pt_yz + (long)k * g.npw_max;
+  const T* tk = tk_all + (long)k * g.npw_max;
+  for (int s = s0 + wave; s < s1; s += 4)
+    for (int q = pp[s] + lane; q < pp[s + 1]; q += 64) {
+      const int si = src[q];
+      cpx<T> v = rows[(s - s0) * N + yz[q] % N];
+      v.x *= scale;
+      v.y *= scale;
+      if constexpr (KIN) {
+        const cpx<T> cv = cb[si];
+        v.x += tk[si] * cv.x;
+        v.y += tk[si] * cv.y;
+      }
+      ob[si] = v;
+    }
+}
+
+// wave-line configurations: N = M*S with M <= S <= 16
+template <int N> struct WaveCfg { static constexpr int M = 0, S = 0; };
+template <> struct WaveCfg<24> { static constexpr int M = 4, S = 6; };
+template <> struct WaveCfg<36> { static constexpr int M = 6, S = 6; };
+template <> struct WaveCfg<90> { static constexpr int M = 9, S = 10; };
+template <> struct WaveCfg<120> { static constexpr int M = 10, S = 12; };
+
+static bool wave_length(int n) { return n == 24 || n == 36 || n == 90 || n == 120; }
+
+template <typename T, int N>
+static size_t plane_w_lds(const Plan& P) {
+  using C = WaveCfg<N>;
+  using W = WaveLine<T, N, C::M, C::S>;
+  return (size_t)(4 * W::E_PER_WAVE + N + (size_t)P.st_max_plane * N) * sizeof(cpx<T>) + N * sizeof(int);
+}
+
+// Launches P1/P2/P3 of the wave-line path; false if the box is not handled.
+template <typename T, int N>
+static int launch_wave_path_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, long ldc, long sc, cpx<T>* box,
+                              const T* vn, const T* t, cpx<T>* out, long ldo, long so, T scale, bool kin,
+                              StageTimer& tm, hipStream_t st) {
+  using C = WaveCfg<N>;
+  using W = WaveLine<T, N, C::M, C::S>;
+  const long total = (long)P.na * nbt;
+  const int nblk = (int)(8 * ((total + 7) / 8));
+  const size_t lds = plane_w_lds<T, N>(P);
+  const cpx<T>* tw = tw_of<T>(&P, 1);
+  hipEvent_t ev = tm.begin();
+  hipLaunchKernelGGL((k_plane_inv_w<T, N, C::M, C::S>), dim3(nblk), dim3(256), lds, st, g, total, c, ldc, sc, box,
+                     tw);
+  tm.end(ST_PLANE_INV, ev);
+  ev = tm.begin();
+  const int n23 = N * N;
+  const int bgroup = 8;
+  const int lines_per_wg = 4 * W::LW;
+  hipLaunchKernelGGL((k_xpass_wave<T, N, C::M, C::S>),
+                     dim3((n23 + lines_per_wg - 1) / lines_per_wg, (nbt + bgroup - 1) / bgroup), dim3(256), 0, st,
+                     g, nbt, bgroup, box, vn, tw_of<T>(&P, 0));
+  tm.end(ST_XPASS, ev);
+  ev = tm.begin();
+  if (kin)
+    hipLaunchKernelGGL((k_plane_fwd_w<T, N, C::M, C::S, true>), dim3(nblk), dim3(256), lds, st, g, total, box, c,
+                       ldc, sc, t, out, ldo, so, scale, tw);
+  else
+    hipLaunchKernelGGL((k_plane_fwd_w<T, N, C::M, C::S, false>), dim3(nblk), dim3(256), lds, st, g, total, box, c,
+                       ldc, sc, t, out, ldo, so, scale, tw);
+  tm.end(ST_PLANE_FWD, ev);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
+static bool wave_path_ok(const Plan& P, int precision) {
+  if (P.flags & GW_PLAN_LEGACY) return false;
+  if (P.n[0] != P.n[1] || P.n[1] != P.n[2] || !wave_length(P.n[0]) || P.xlo < 0) return false;
+  const size_t e = precision == GW_FP64 ? 16 : 8;
+  const size_t need = (size_t)(P.st_max_plane + 1) * P.n[0] * e + 4 * 64 * 17 * e + P.n[0] * 4;
+  return need <= (size_t)std::min(P.lds_max, kMaxLds);
+}
+
+template <typename T>
+static int launch_wave_path(Plan& P, const Geom& g, int nbt, const cpx<T>* c, long ldc, long sc, cpx<T>* box,
+                            const T* vn, const T* t, cpx<T>* out, long ldo, long so, T scale, bool kin,
+                            StageTimer& tm, hipStream_t st) {
+  switch (P.n[0]) {
+    case 24: return launch_wave_path_n<T, 24>(P, g, nbt, c, ldc, sc, box, vn, t, out, ldo, so, scale, kin, tm, st);
+    case 36: return launch_wave_path_n<T, 36>(P, g, nbt, c, ldc, sc, box, vn, t, out, ldo, so, scale, kin, tm, st);
+    case 90: return launch_wave_path_n<T, 90>(P, g, nbt, c, ldc, sc, box, vn, t, out, ldo, so, scale, kin, tm, st);
+    case 120: return launch_wave_path_n<T, 120>(P, g, nbt, c, ldc, sc, box, vn, t, out, ldo, so, scale, kin, tm, st);
+    default: set_error("no wave-line configuration for n=%d", P.n[0]); return GW_ERR_UNSUPPORTED;
+  }
+}
+
+template <typename T, int N>
+static void set_wave_attr_n(int want) {
+  using C = WaveCfg<N>;
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_w<T, N, C::M, C::S>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, true>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, false>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+}
+static void set_wave_lds_attrs(int want) {
+  set_wave_attr_n<double, 24>(want); set_wave_attr_n<float, 24>(want);
+  set_wave_attr_n<double, 36>(want); set_wave_attr_n<float, 36>(want);
+  set_wave_attr_n<double, 90>(want); set_wave_attr_n<float, 90>(want);
+  set_wave_attr_n<double, 120>(want); set_wave_attr_n<float, 120>(want);
+}
+
+// ---- set-up kernels ----
+template <typename T>
+__global__ void k_permute_v(int n1, int n2, int n3, const int* __restrict__ px, const int* __restrict__ py,
+                            const int* __restrict__ pz, const double* __restrict__ v, T* __restrict__ vp) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)n1 * n2 * n3;
+  if (e >= n) return;
+  const int i2 = (int)(e % n3);
+  const int i1 = (int)((e / n3) % n2);
+  const int i0 = (int)(e / ((long)n2 * n3));
+  vp[e] = (T)v[((long)px[i0] * n2 + py[i1]) * n3 + pz[i2]];
+}
+
+__global__ void k_cast_r(long n, const double* __restrict__ a, float* __restrict__ b) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) b[e] = (float)a[e];
+}
+
+// copy projectors (nk, nproj, npw_max), zero padding, write both precisions
+__global__ void k_proj_copy(int nproj, int npw_max, const int* __restrict__ npw,
+                            const double2* __restrict__ p, double2* __restrict__ p64,
+                            float2* __restrict__ p32) {
+  const int row = blockIdx.y;  // k*nproj + j
+  const int k = row / nproj;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npw_max) return;
+  const long o = (long)row * npw_max + q;
+  double2 v = p[o];
+  if (q >= npw[k]) { v.x = 0; v.y = 0; }
+  p64[o] = v;
+  p32[o] = make_float2((float)v.x, (float)v.y);
+}
+
+template <typename T>
+__global__ void k_fill_zero(long n, cpx<T>* __restrict__ a) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) a[e] = czero<T>();
+}
+
+template <typename T>
+__global__ void k_pack_box(Geom g, const cpx<T>* __restrict__ c, cpx<T>* __restrict__ box, int unpack) {
+  const int a = blockIdx.x, bi = blockIdx.y;
+  const int k = bi / g.nb;
+  const long n = (long)g.n1 * g.n2 * g.n3;
+  const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
+  const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
+  const int p0 = pp[sp[a]], p1 = pp[sp[a + 1]];
+  const int* src = g.pt_src + (long)k * g.npw_max;
+  const int* yz = g.pt_yz + (long)k * g.npw_max;
+  const long base = (long)g.planes[a] * g.n2 * g.n3;
+  for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
+    if (unpack) {
+      const_cast<cpx<T>*>(c)[(long)bi * g.npw_max + src[q]] = box[(long)bi * n + base + yz[q]];
+    } else {
+      box[(long)bi * n + base + yz[q]] = c[(long)bi * g.npw_max + src[q]];
+    }
+  }
+}
+
+// psi(r) in natural order from the P1 planes: inverse x and unpermute
+template <typename T>
+__global__ void __launch_bounds__(kXThreads)
+k_x_to_real(Geom g, int L, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ twx,
+            const int* __restrict__ px, const int* __restrict__ py, const int* __restrict__ pz,
+            cpx<T>* __restrict__ psi) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
+  const int n23 = g.n2 * g.n3;
+  const int j0 = blockIdx.x * L;
+  const int nl = min(L, n23 - j0);
+  const int ne = g.n1 * L;
+  const int bi = blockIdx.y;
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) tile[e] = czero<T>();
+  __syncthreads();
+  const cpx<T>* bb = box + (long)bi * g.na * n23 + j0;
+  for (int e = threadIdx.x; e < g.na * L; e += blockDim.x) {
+    const int a = e / L, l = e - a * L;
+    if (l < nl) tile[g.planes[a] * L + l] = bb[(long)a * n23 + l];
+  }
+  __syncthreads();
+  line_fft_inverse<T, true>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
+  __syncthreads();
+  const long nbox = (long)g.n1 * n23;
+  for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+    const int x = e / L, l = e - x * L;
+    if (l >= nl) continue;
+    const int j = j0 + l;
+    const int y = j / g.n3, z = j - y * g.n3;
+    psi[(long)bi * nbox + ((long)px[x] * g.n2 + py[y]) * g.n3 + pz[z]] = tile[e];
+  }
+}
+
+}  // namespace gw
+
+// ------------------------------------------------------------------ C ABI
+
+using namespace gw;
+
+struct gw_plan {
+  Plan p;
+  int* perm[3];  // device permutation per axis: position -> index
+  int* ident;    // identity maps of length n1, n2, n3 (natural order)
+  int* ident_y;
+  int* ident_z;
+  StageTimer timer;
+};
+
+static int grid_blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
+
+extern "C" const char* gw_last_error(void) { return gw::g_err; }
+extern "C" int gw_version(void) { return 1; }
+
+extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int npw_max,
+                              const int64_t* flat_idx, const int64_t* npw) {
+  GW_REQUIRE(out && flat_idx && npw, GW_ERR_ARG, "null argument");
+  GW_REQUIRE(n1 > 0 && n2 > 0 && n3 > 0 && nk > 0 && npw_max > 0, GW_ERR_ARG, "bad sizes");
+  gw_plan* h = new gw_plan();
+  std::memset(h, 0, sizeof(*h));
+  Plan& P = h->p;
+  P.n[0] = n1; P.n[1] = n2; P.n[2] = n3;
+  P.nk = nk; P.npw_max = npw_max;
+  for (int ax = 0; ax < 3; ++ax) {
+    if (!split_length(P.n[ax], &P.f[ax][0], &P.f[ax][1])) {
+      delete h;
+      set_error("box length %d cannot be split into two codelets <= 16", P.n[ax]);
+      return GW_ERR_UNSUPPORTED;
+    }
+  }
+  const long N = (long)n1 * n2 * n3;
+  const long n23 = (long)n2 * n3;
+  // decode points, active planes
+  std::vector<char> plane_on(n1, 0);
+  for (int k = 0; k < nk; ++k) {
+    if (npw[k] < 0 || npw[k] > npw_max) { delete h; set_error("npw[%d] out of range", k); return GW_ERR_ARG; }
+    for (long q = 0; q < npw[k]; ++q) {
+      const int64_t f = flat_idx[(long)k * npw_max + q];
+      if (f < 0 || f >= N) { delete h; set_error("flat_idx out of box at k=%d q=%ld", k, q); return GW_ERR_ARG; }
+      plane_on[f / n23] = 1;
+    }
+  }
+  std::vector<int> planes, slot(n1, -1);
+  for (int i = 0; i < n1; ++i)
+    if (plane_on[i]) { slot[i] = (int)planes.size(); planes.push_back(i); }
+  P.na = (int)planes.size();
+  if (P.na == 0) { planes.push_back(0); slot[0] = 0; P.na = 1; }
+  // per-k sticks
+  std::vector<std::vector<int>> kst_i1(nk), kst_ptr(nk), kpl_ptr(nk), kpt_src(nk), kpt_yz(nk);
+  int nst_max = 1, st_max_plane = 0, max_pts_plane = 0;
+  for (int k = 0; k < nk; ++k) {
+    const long nq = npw[k];
+    std::vector<int> order(nq);
+    std::iota(order.begin(), order.end(), 0);
+    auto key = [&](int q) { const int64_t f = flat_idx[(long)k * npw_max + q];
+                            return std::make_pair(slot[f / n23], f % n23); };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return key(x) < key(y); });
+    std::vector<int>& sti = kst_i1[k];
+    std::vector<int>& stp = kst_ptr[k];
+    std::vector<int> st_slot;
+    for (long j = 0; j < nq; ++j) {
+      const int q = order[j];
+      const int64_t f = flat_idx[(long)k * npw_max + q];
+      const int sl = slot[f / n23];
+      const int i1 = (int)((f % n23) / n3);
+      if (sti.empty() || st_slot.back() != sl || sti.back() != i1) {
+        sti.push_back(i1); st_slot.push_back(sl); stp.push_back((int)j);
+      }
+      kpt_src[k].push_back(q);
+      kpt_yz[k].push_back((int)(f % n23));
+    }
+    stp.push_back((int)nq);
+    std::vector<int>& plp = kpl_ptr[k];
+    plp.assign(P.na + 1, 0);
+    int s = 0;
+    for (int a = 0; a <= P.na; ++a) {
+      while (s < (int)st_slot.size() && st_slot[s] < a) ++s;
+      plp[a] = s;
+    }
+    for (int a = 0; a < P.na; ++a) {
+      st_max_plane = std::max(st_max_plane, plp[a + 1] - plp[a]);
+      max_pts_plane = std::max(max_pts_plane, stp[plp[a + 1]] - stp[plp[a]]);
+    }
+    nst_max = std::max(nst_max, (int)sti.size());
+  }
+  P.nst_max = nst_max;
+  P.st_max_plane = st_max_plane;
+  P.max_pts_plane = max_pts_plane;
+  std::vector<int> h_plp((long)nk * (P.na + 1)), h_sti((long)nk * nst_max, 0),
+      h_stp((long)nk * (nst_max + 1), 0), h_src((long)nk * npw_max, 0), h_yz((long)nk * npw_max, 0),
+      h_npw(nk);
+  for (int k = 0; k < nk; ++k) {
+    std::copy(kpl_ptr[k].begin(), kpl_ptr[k].end(), h_plp.begin() + (long)k * (P.na + 1));
+    std::copy(kst_i1[k].begin(), kst_i1[k].end(), h_sti.begin() + (long)k * nst_max);
+    std::copy(kst_ptr[k].begin(), kst_ptr[k].end(), h_stp.begin() + (long)k * (nst_max + 1));
+    for (long s = (long)kst_ptr[k].size(); s <= nst_max; ++s)
+      h_stp[(long)k * (nst_max + 1) + s] = kst_ptr[k].back();
+    std::copy(kpt_src[k].begin(), kpt_src[k].end(), h_src.begin() + (long)k * npw_max);
+    std::copy(kpt_yz[k].begin(), kpt_yz[k].end(), h_yz.begin() + (long)k * npw_max);
+    h_npw[k] = (int)npw[k];
+  }
+  std::vector<int> h_slot(n1, -1);
+  for (int a = 0; a < P.na; ++a) h_slot[planes[a]] = a;
+  {  // wrapped contiguous range [0, lo) U [hi, n1) with slots in that order?
+    int lo = 0;
+    while (lo < n1 && plane_on[lo]) ++lo;
+    int hi = n1;
+    while (hi > lo && plane_on[hi - 1]) --hi;
+    bool ok = true;
+    for (int i = lo; i < hi; ++i) ok = ok && !plane_on[i];
+    P.xlo = ok ? lo : -1;
+    P.xhi = ok ? hi : n1;
+  }
+  auto up = [&](int** d, const std::vector<int>& v) -> int {
+    GW_CHECK_HIP(hipMalloc(d, std::max<size_t>(1, v.size()) * sizeof(int)));
+    GW_CHECK_HIP(hipMemcpy(*d, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+    return GW_OK;
+  };
+  int rc;
+  if ((rc = up(&P.planes, planes)) || (rc = up(&P.st_plane_ptr, h_plp)) || (rc = up(&P.st_i1, h_sti)) ||
+      (rc = up(&P.st_pt_ptr, h_stp)) || (rc = up(&P.pt_src, h_src)) || (rc = up(&P.pt_yz, h_yz)) ||
+      (rc = up(&P.npw, h_npw)) || (rc = up(&P.slot_of, h_slot))) {
+    gw_plan_destroy(h);
+    return rc;
+  }
+  // twiddles e^{+2 pi i t/n} and axis permutations
+  for (int ax = 0; ax < 3; ++ax) {
+    const int n = P.n[ax], a = P.f[ax][0], b = P.f[ax][1];
+    std::vector<double2> t64(n);
+    std::vector<float2> t32(n);
+    for (int t = 0; t < n; ++t) {
+      const long double ang = 2.0L * 3.14159265358979323846264338327950288L * t / n;
+      t64[t] = make_double2((double)cosl(ang), (double)sinl(ang));
+      t32[t] = make_float2((float)cosl(ang), (float)sinl(ang));
+    }
+    std::vector<int> perm(n);
+    for (int k1 = 0; k1 < a; ++k1)
+      for (int k2 = 0; k2 < b; ++k2) perm[b * k1 + k2] = k1 + a * k2;
+    if (hipMalloc(&P.tw64[ax], n * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&P.tw32[ax], n * sizeof(float2)) != hipSuccess ||
+        hipMalloc(&h->perm[ax], n * sizeof(int)) != hipSuccess) {
+      set_error("hipMalloc twiddles failed"); gw_plan_destroy(h); return GW_ERR_HIP;
+    }
+    hipMemcpy(P.tw64[ax], t64.data(), n * sizeof(double2), hipMemcpyHostToDevice);
+    hipMemcpy(P.tw32[ax], t32.data(), n * sizeof(float2), hipMemcpyHostToDevice);
+    hipMemcpy(h->perm[ax], perm.data(), n * sizeof(int), hipMemcpyHostToDevice);
+  }
+  if (hipMalloc(&P.t64, (size_t)nk * npw_max * sizeof(double)) != hipSuccess ||
+      hipMalloc(&P.t32, (size_t)nk * npw_max * sizeof(float)) != hipSuccess ||
+      hipMalloc(&P.vp64, (size_t)N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&P.vp32, (size_t)N * sizeof(float)) != hipSuccess ||
+      hipMalloc(&P.vx64, (size_t)N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&P.vx32, (size_t)N * sizeof(float)) != hipSuccess ||
+      hipMalloc(&h->ident, (size_t)n1 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&h->ident_y, (size_t)n2 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&h->ident_z, (size_t)n3 * sizeof(int)) != hipSuccess) {
+    set_error("hipMalloc tables failed"); gw_plan_destroy(h); return GW_ERR_HIP;
+  }
+  hipMemset(P.t64, 0, (size_t)nk * npw_max * sizeof(double));
+  hipMemset(P.t32, 0, (size_t)nk * npw_max * sizeof(float));
+  hipMemset(P.vp64, 0, (size_t)N * sizeof(double));
+  hipMemset(P.vp32, 0, (size_t)N * sizeof(float));
+  hipMemset(P.vx64, 0, (size_t)N * sizeof(double));
+  hipMemset(P.vx32, 0, (size_t)N * sizeof(float));
+  {
+    std::vector<int> id(std::max(n1, std::max(n2, n3)));
+    std::iota(id.begin(), id.end(), 0);
+    hipMemcpy(h->ident, id.data(), n1 * sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpy(h->ident_y, id.data(), n2 * sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpy(h->ident_z, id.data(), n3 * sizeof(int), hipMemcpyHostToDevice);
+  }
+  GW_CHECK_HIP(hipDeviceSynchronize());
+  int dev = 0, lds = 65536;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+    P.lds_max = lds;
+  else
+    P.lds_max = 65536;
+  // plane kernels may take more than 64 KiB of dynamic LDS on gfx950
+  const int want = std::min(P.lds_max, kMaxLds);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd<double, true>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd<double, false>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd<float, true>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_xpass<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_xpass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_x_to_real<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_x_to_real<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  set_wave_lds_attrs(want);
+  *out = h;
+  return GW_OK;
+}
+
+extern "C" int gw_plan_set_flags(gw_plan* h, int flags) {
+  GW_REQUIRE(h, GW_ERR_ARG, "null plan");
+  h->p.flags = flags;
+  return GW_OK;
+}
+
+extern "C" int gw_plan_stage_times(gw_plan* h, double* ms, long* calls, int n, int reset) {
+  GW_REQUIRE(h && ms && calls && n >= 0, GW_ERR_ARG, "bad argument");
+  h->timer.collect();
+  for (int i = 0; i < n && i < ST_COUNT; ++i) { ms[i] = h->timer.ms[i]; calls[i] = h->timer.calls[i]; }
+  if (reset)
+    for (int i = 0; i < ST_COUNT; ++i) { h->timer.ms[i] = 0; h->timer.calls[i] = 0; }
+  return GW_OK;
+}
+
+extern "C" int gw_plan_info(const gw_plan* h, int precision, int* info, int n) {
+  GW_REQUIRE(h && info, GW_ERR_ARG, "bad argument");
+  const Plan& P = h->p;
+  const int v[] = {P.na, P.nst_max, wave_path_ok(P, precision) ? 2 : (plane_fits_lds(&P, precision) ? 1 : 0), P.lds_max, P.nproj,
+                   P.f[0][0], P.f[0][1], P.f[1][0], P.f[1][1], P.f[2][0], P.f[2][1]};
+  for (int i = 0; i < n && i < (int)(sizeof(v) / sizeof(v[0])); ++i) info[i] = v[i];
+  return GW_OK;
+}
+
+extern "C" int gw_plan_destroy(gw_plan* h) {
+  if (!h) return GW_OK;
+  Plan& P = h->p;
+  void* ptrs[] = {P.planes, P.st_plane_ptr, P.st_i1, P.st_pt_ptr, P.pt_src, P.pt_yz, P.npw,
+                  P.tw64[0], P.tw64[1], P.tw64[2], P.tw32[0], P.tw32[1], P.tw32[2],
+                  P.t64, P.t32, P.vp64, P.vp32, P.vx64, P.vx32, P.slot_of, P.p64, P.p32, P.d_colptr,
+                  P.d_row, P.d_val, h->perm[0], h->perm[1], h->perm[2], h->ident, h->ident_y,
+                  h->ident_z};
+  hipDeviceSynchronize();
+  h->timer.collect();
+  for (hipEvent_t e : h->timer.pool) hipEventDestroy(e);
+  for (void* q : ptrs)
+    if (q) hipFree(q);
+  delete h;
+  return GW_OK;
+}
+
+extern "C" int gw_plan_set_kinetic(gw_plan* h, const double* t, void* stream) {
+  GW_REQUIRE(h && t, GW_ERR_ARG, "null argument");
+  Plan& P = h->p;
+  hipStream_t st = (hipStream_t)stream;
+  const long n = (long)P.nk * P.npw_max;
+  GW_CHECK_HIP(hipMemcpyAsync(P.t64, t, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(k_cast_r, dim3(grid_blocks(n, 256)), dim3(256), 0, st, n, P.t64, P.t32);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
+extern "C" int gw_plan_set_potential(gw_plan* h, const double* v, void* stream) {
+  GW_REQUIRE(h && v, GW_ERR_ARG, "null argument");
+  Plan& P = h->p;
+  hipStream_t st = (hipStream_t)stream;
+  const long n = (long)P.n[0] * P.n[1] * P.n[2];
+  hipLaunchKernelGGL(k_permute_v<double>, dim3(grid_blocks(n, 256)), dim3(256), 0, st, P.n[0], P.n[1],
+                     P.n[2], h->perm[0], h->perm[1], h->perm[2], v, P.vp64);
+  hipLaunchKernelGGL(k_permute_v<float>, dim3(grid_blocks(n, 256)), dim3(256), 0, st, P.n[0], P.n[1],
+                     P.n[2], h->perm[0], h->perm[1], h->perm[2], v, P.vp32);
+  hipLaunchKernelGGL(k_permute_v<double>, dim3(grid_blocks(n, 256)), dim3(256), 0, st, P.n[0], P.n[1],
+                     P.n[2], h->ident, h->ident_y, h->ident_z, v, P.vx64);
+  hipLaunchKernelGGL(k_permute_v<float>, dim3(grid_blocks(n, 256)), dim3(256), 0, st, P.n[0], P.n[1],
+                     P.n[2], h->ident, h->ident_y, h->ident_z, v, P.vx32);
+  GW_CHECK_HIP(hipGetLastError());
+  P.have_v = true;
+  return GW_OK;
+}
+
+extern "C" int gw_plan_set_projectors(gw_plan* h, int nproj, const void* p, const double* dij,
+                                      void* stream) {
+  GW_REQUIRE(h && nproj >= 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(nproj == 0 || (p && dij), GW_ERR_ARG, "null projector table");
+  Plan& P = h->p;
+  hipStream_t st = (hipStream_t)stream;
+  GW_CHECK_HIP(hipStreamSynchronize(st));
+  for (void** q : {(void**)&P.p64, (void**)&P.p32, (void**)&P.d_colptr, (void**)&P.d_row, (void**)&P.d_val})
+    if (*q) { hipFree(*q); *q = nullptr; }
+  P.nproj = nproj;
+  if (!nproj) return GW_OK;
+  const size_t n = (size_t)P.nk * nproj * P.npw_max;
+  GW_CHECK_HIP(hipMalloc(&P.p64, n * sizeof(double2)));
+  GW_CHECK_HIP(hipMalloc(&P.p32, n * sizeof(float2)));
+  hipLaunchKernelGGL(k_proj_copy, dim3(grid_blocks(P.npw_max, 256), P.nk * nproj), dim3(256), 0, st, nproj,
+                     P.npw_max, P.npw, (const double2*)p, P.p64, P.p32);
+  GW_CHECK_HIP(hipGetLastError());
+  std::vector<int> colptr(nproj + 1, 0), row;
+  std::vector<double> val;
+  for (int q = 0; q < nproj; ++q) {
+    for (int r = 0; r < nproj; ++r) {
+      const double d = dij[(long)r * nproj + q];
+      if (d != 0.0) { row.push_back(r); val.push_back(d); }
+    }
+    colptr[q + 1] = (int)row.size();
+  }
+  P.d_nnz = (int)row.size();
+  GW_CHECK_HIP(hipMalloc(&P.d_colptr, colptr.size() * sizeof(int)));
+  GW_CHECK_HIP(hipMalloc(&P.d_row, std::max<size_t>(1, row.size()) * sizeof(int)));
+  GW_CHECK_HIP(hipMalloc(&P.d_val, std::max<size_t>(1, val.size()) * sizeof(double)));
+  GW_CHECK_HIP(hipMemcpy(P.d_colptr, colptr.data(), colptr.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!row.empty()) {
+    GW_CHECK_HIP(hipMemcpy(P.d_row, row.data(), row.size() * sizeof(int), hipMemcpyHostToDevice));
+    GW_CHECK_HIP(hipMemcpy(P.d_val, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return GW_OK;
+}
+
+extern "C" size_t gw_hpsi_workspace_bytes(const gw_plan* h, int nb, int precision) {
+  if (!h || nb <= 0) return 0;
+  return apply_layout(&h->p, nb, precision).total;
+}
+
+template <typename T>
+static int apply_t(gw_plan* h, int terms, const cpx<T>* c, long ldc, long sc, cpx<T>* out, long ldo,
+                   long so, int nb, unsigned char* ws, hipStream_t st) {
+  Plan& P = h->p;
+  const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
+  const ApplyLayout L = apply_layout(&P, nb, prec);
+  const Geom g = geom_of(&P, nb);
+  const int nbt = P.nk * nb;
+  const T* t = prec == GW_FP64 ? (const T*)P.t64 : (const T*)P.t32;
+  const T* vp = prec == GW_FP64 ? (const T*)P.vp64 : (const T*)P.vp32;
+  const cpx<T>* pr = prec == GW_FP64 ? (const cpx<T>*)P.p64 : (const cpx<T>*)P.p32;
+  cpx<T>* box = (cpx<T>*)(ws + L.box_off);
+  const bool kin = terms & GW_TERM_KINETIC;
+  const T scale = T(1.0 / ((double)P.n[0] * P.n[1] * P.n[2]));
+  StageTimer& tm = h->timer;
+  tm.on = (P.flags & GW_PLAN_TIMING) != 0;
+  tm.st = st;
+  hipEvent_t ev;
+  if (terms & GW_TERM_LOCAL) {
+    GW_REQUIRE(P.have_v, GW_ERR_ARG, "local term requested but no potential set");
+    if (wave_path_ok(P, prec)) {
+      const T* vn = prec == GW_FP64 ? (const T*)P.vx64 : (const T*)P.vx32;
+      int rc = launch_wave_path<T>(P, g, nbt, c, ldc, sc, box, vn, t, out, ldo, so, scale, kin, tm, st);
+      if (rc) return rc;
+    } else {
+      const size_t plane_lds = (size_t)P.n[1] * P.n[2] * sizeof(cpx<T>);
+      ev = tm.begin();
+      if (L.fused) {
+        hipLaunchKernelGGL(k_plane_inv<T>, dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, c, ldc, sc,
+                           box, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
+      } else {
+        cpx<T>* sticks = (cpx<T>*)(ws + L.stick_off);
+        hipLaunchKernelGGL(k_sticks_inv<T>, dim3((P.nst_max + kStickChunk - 1) / kStickChunk, nbt), dim3(256),
+                           kStickChunk * P.n[2] * sizeof(cpx<T>), st, g, c, ldc, sc, sticks, tw_of<T>(&P, 2));
+        hipLaunchKernelGGL(k_plane_inv_y<T>, dim3(P.na, nbt, (P.n[2] + kColChunk - 1) / kColChunk), dim3(256),
+                           P.n[1] * kColChunk * sizeof(cpx<T>), st, g, sticks, box, tw_of<T>(&P, 1));
+      }
+      tm.end(ST_PLANE_INV, ev);
+      const int XL = x_lines(&P);
+      ev = tm.begin();
+        hipLaunchKernelGGL(k_xpass<T>, dim3((P.n[1] * P.n[2] + XL - 1) / XL, (nbt + kXBandGroup - 1) / kXBandGroup),
+                           dim3(kXThreads), (size_t)P.n[0] * XL * sizeof(cpx<T>), st, g, XL, nbt, box, vp,
+                           tw_of<T>(&P, 0));
+      tm.end(ST_XPASS, ev);
+      ev = tm.begin();
+      if (L.fused) {
+        if (kin)
+          hipLaunchKernelGGL((k_plane_fwd<T, true>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, box,
+                             c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
+        else
+          hipLaunchKernelGGL((k_plane_fwd<T, false>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, box,
+                             c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
+      } else {
+        cpx<T>* sticks = (cpx<T>*)(ws + L.stick_off);
+        hipLaunchKernelGGL(k_plane_fwd_y<T>, dim3(P.na, nbt, (P.n[2] + kColChunk - 1) / kColChunk), dim3(256),
+                           P.n[1] * kColChunk * sizeof(cpx<T>), st, g, box, sticks, tw_of<T>(&P, 1));
+        const int nsc = std::max(1, (P.nst_max + kStickChunk - 1) / kStickChunk);
+        if (kin)
+          hipLaunchKernelGGL((k_sticks_fwd<T, true>), dim3(nsc, nbt), dim3(256), kStickChunk * P.n[2] * sizeof(cpx<T>),
+                             st, g, sticks, c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 2));
+        else
+          hipLaunchKernelGGL((k_sticks_fwd<T, false>), dim3(nsc, nbt), dim3(256), kStickChunk * P.n[2] * sizeof(cpx<T>),
+                             st, g, sticks, c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 2));
+      }
+      tm.end(ST_PLANE_FWD, ev);
+
+    }
+  } else if (kin) {
+    hipLaunchKernelGGL(k_kinetic<T>, dim3(grid_blocks(P.npw_max, 256), nbt), dim3(256), 0, st, nb, P.npw_max,
+                       P.npw, c, ldc, sc, t, out, ldo, so);
+  }
+  GW_CHECK_HIP(hipGetLastError());
+  const bool base = terms & (GW_TERM_KINETIC | GW_TERM_LOCAL);
+  if ((terms & GW_TERM_NONLOCAL) && P.nproj) {
+    cpx<T>* becp = (cpx<T>*)(ws + L.becp_off);
+    cpx<T>* u = (cpx<T>*)(ws + L.u_off);
+    const long pstride = (long)P.nproj * P.npw_max;
+    ev = tm.begin();
+    int rc = gemm_inner<T>(P.nk, P.nproj, nb, P.npw_max, pr, P.npw_max, pstride, c, ldc, sc, becp, nb,
+                           (long)P.nproj * nb, ws + L.part_off, L.part_bytes, st);
+    if (rc) return rc;
+    tm.end(ST_BECP, ev);
+    ev = tm.begin();
+    hipLaunchKernelGGL(k_dmul<T>, dim3(grid_blocks((long)P.nproj * nb, 256), P.nk), dim3(256), 0, st, P.nproj,
+                       nb, P.d_colptr, P.d_row, P.d_val, becp, u);
+    GW_CHECK_HIP(hipGetLastError());
+    tm.end(ST_DMUL, ev);
+    ev = tm.begin();
+    rc = gemm_combine<T>(P.nk, nb, P.npw_max, P.nproj, u, nb, (long)P.nproj * nb, 0, pr, P.npw_max, pstride,
+                         out, ldo, so, base ? T(1) : T(0), st);
+    if (rc) return rc;
+    tm.end(ST_BACKPROJ, ev);
+  } else if (!base) {
+    for (int k = 0; k < P.nk; ++k)
+      hipLaunchKernelGGL(k_fill_zero<T>, dim3(grid_blocks((long)nb * ldo, 256)), dim3(256), 0, st,
+                         (long)(nb - 1) * ldo + P.npw_max, out + (long)k * so);
+  }
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
+extern "C" int gw_hpsi_apply_strided(gw_plan* h, int precision, int terms, const void* c, long ldc,
+                                     long sc, void* out, long ldo, long so, int nb, void* ws,
+                                     size_t ws_bytes, void* stream) {
+  GW_REQUIRE(h && c && out && nb > 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(precision == GW_FP64 || precision == GW_FP32, GW_ERR_ARG, "bad precision");
+  const ApplyLayout L = apply_layout(&h->p, nb, precision);
+  GW_REQUIRE(ws && ws_bytes >= L.total, GW_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, L.total);
+  GW_REQUIRE(ldc >= h->p.npw_max && ldo >= h->p.npw_max, GW_ERR_ARG, "row stride < npw_max");
+  if (precision == GW_FP64)
+    return apply_t<double>(h, terms, (const double2*)c, ldc, sc, (double2*)out, ldo, so, nb,
+                           (unsigned char*)ws, (hipStream_t)stream);
+  return apply_t<float>(h, terms, (const float2*)c, ldc, sc, (float2*)out, ldo, so, nb, (unsigned char*)ws,
+                        (hipStream_t)stream);
+}
+
+extern "C" int gw_hpsi_apply(gw_plan* h, int precision, int terms, const void* c, void* out, int nb,
+                             void* ws, size_t ws_bytes, void* stream) {
+  if (!h) { set_error("null plan"); return GW_ERR_ARG; }
+  const long ld = h->p.npw_max, s = (long)nb * h->p.npw_max;
+  return gw_hpsi_apply_strided(h, precision, terms, c, ld, s, out, ld, s, nb, ws, ws_bytes, stream);
+}
+
+extern "C" size_t gw_wfc_to_real_workspace_bytes(const gw_plan* h, int nb, int precision) {
+  if (!h || nb <= 0) return 0;
+  return apply_layout(&h->p, nb, precision).total;
+}
+
+template <typename T>
+static int to_real_t(gw_plan* h, const cpx<T>* c, cpx<T>* psi, int nb, unsigned char* ws, hipStream_t st) {
+  Plan& P = h->p;
+  const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
+  const ApplyLayout L = apply_layout(&P, nb, prec);
+  const Geom g = geom_of(&P, nb);
+  const int nbt = P.nk * nb;
+  cpx<T>* box = (cpx<T>*)(ws + L.box_off);
+  const long ld = P.npw_max, sc = (long)nb * P.npw_max;
+  if (L.fused) {
+    hipLaunchKernelGGL(k_plane_inv<T>, dim3(P.na, nbt), dim3(kPlaneThreads), (size_t)P.n[1] * P.n[2] * sizeof(cpx<T>),
+                       st, g, c, ld, sc, box, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
+  } else {
+    cpx<T>* sticks = (cpx<T>*)(ws + L.stick_off);
+    hipLaunchKernelGGL(k_sticks_inv<T>, dim3((P.nst_max + kStickChunk - 1) / kStickChunk, nbt), dim3(256),
+                       kStickChunk * P.n[2] * sizeof(cpx<T>), st, g, c, ld, sc, sticks, tw_of<T>(&P, 2));
+    hipLaunchKernelGGL(k_plane_inv_y<T>, dim3(P.na, nbt, (P.n[2] + kColChunk - 1) / kColChunk), dim3(256),
+                       P.n[1] * kColChunk * sizeof(cpx<T>), st, g, sticks, box, tw_of<T>(&P, 1));
+  }
+  const int XL = x_lines(&P);
+  hipLaunchKernelGGL(k_x_to_real<T>, dim3((P.n[1] * P.n[2] + XL - 1) / XL, nbt), dim3(kXThreads),
+                     (size_t)P.n[0] * XL * sizeof(cpx<T>), st, g, XL, box, tw_of<T>(&P, 0), h->perm[0],
+                     h->perm[1], h->perm[2], psi);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
+extern "C" int gw_wfc_to_real(gw_plan* h, int precision, const void* c, void* psi, int nb, void* ws,
+                              size_t ws_bytes, void* stream) {
+  GW_REQUIRE(h && c && psi && nb > 0, GW_ERR_ARG, "bad argument");
+  const ApplyLayout L = apply_layout(&h->p, nb, precision);
+  GW_REQUIRE(ws && ws_bytes >= L.total, GW_ERR_WORKSPACE, "workspace too small");
+  if (precision == GW_FP64)
+    return to_real_t<double>(h, (const double2*)c, (double2*)psi, nb, (unsigned char*)ws, (hipStream_t)stream);
+  return to_real_t<float>(h, (const float2*)c, (float2*)psi, nb, (unsigned char*)ws, (hipStream_t)stream);
+}
+
+template <typename T>
+static int pack_t(gw_plan* h, const void* c, void* box, int nb, int unpack, hipStream_t st) {
+  Plan& P = h->p;
+  const Geom g = geom_of(&P, nb);
+  const int nbt = P.nk * nb;
+  const long N = (long)P.n[0] * P.n[1] * P.n[2];
+  if (!unpack)
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3(grid_blocks(N * nbt, 256)), dim3(256), 0, st, N * nbt, (cpx<T>*)box);
+  else
+    hipLaunchKernelGGL(k_fill_zero<T>, dim3(grid_blocks((long)nbt * P.npw_max, 256)), dim3(256), 0, st,
+                       (long)nbt * P.npw_max, (cpx<T>*)c);
+  hipLaunchKernelGGL(k_pack_box<T>, dim3(P.na, nbt), dim3(256), 0, st, g, (const cpx<T>*)c, (cpx<T>*)box, unpack);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
+extern "C" int gw_sphere_to_box(gw_plan* h, int precision, const void* c, void* box, int nb, void* stream) {
+  GW_REQUIRE(h && c && box && nb > 0, GW_ERR_ARG, "bad argument");
+  return precision == GW_FP64 ? pack_t<double>(h, c, box, nb, 0, (hipStream_t)stream)
+                              : pack_t<float>(h, c, box, nb, 0, (hipStream_t)stream);
+}
+
+extern "C" int gw_box_to_sphere(gw_plan* h, int precision, const void* box, void* c_out, int nb,
+                                void* stream) {
+  GW_REQUIRE(h && c_out && box && nb > 0, GW_ERR_ARG, "bad argument");
+  return precision == GW_FP64 ? pack_t<double>(h, c_out, const_cast<void*>(box), nb, 1, (hipStream_t)stream)
+                              : pack_t<float>(h, c_out, const_cast<void*>(box), nb, 1, (hipStream_t)stream);
+}

@@ -1,0 +1,203 @@
+"""Drop-in ``BatchedHamiltonian`` for gradwave's batched H|psi> on MI355X.
+
+Mirrors core/batch.py:238-559 (constructor signature, ``apply``, ``apply_cols``,
+attributes ``bk``, ``shape``, ``v_eff_r``, dtype polymorphism over complex64 /
+complex128 per core/batch.py:306-331) and the band*k apply tally of
+core/batch.py:111-123. All arithmetic runs in libgradwave_hip (pruned LDS box
+FFTs with fused kinetic/local terms, MFMA projector contractions); this module
+only owns the plan and the device workspace.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from gradwave_amd import _lib as L
+
+_HAPPLY_TALLY = {"on": False, "count": 0}
+
+
+def reset_happly_tally() -> None:
+    """core/batch.py:111-118."""
+    _HAPPLY_TALLY["on"] = True
+    _HAPPLY_TALLY["count"] = 0
+
+
+def happly_tally() -> int:
+    """core/batch.py:120-123."""
+    return _HAPPLY_TALLY["count"]
+
+
+class Plan:
+    """Owner of a gw_plan handle: geometry from BatchedK maps plus the tables."""
+
+    def __init__(self, shape, flat_idx: torch.Tensor, npw: torch.Tensor) -> None:
+        n1, n2, n3 = (int(s) for s in shape)
+        flat = flat_idx.detach().to("cpu", torch.int64).contiguous()
+        npw_h = npw.detach().to("cpu", torch.int64).contiguous()
+        nk, npw_max = flat.shape
+        self.handle = C.c_void_p()
+        self._lib = L.lib()
+        L.check(self._lib.gw_plan_create(C.byref(self.handle), n1, n2, n3, nk, npw_max,
+                                         flat.data_ptr(), npw_h.data_ptr()), "gw_plan_create")
+        self.shape = (n1, n2, n3)
+        self.nk, self.npw_max = nk, npw_max
+
+    def set_kinetic(self, t: torch.Tensor) -> None:
+        t = t.detach().to(torch.float64).contiguous()
+        L.require_gpu(t, "t")
+        L.check(self._lib.gw_plan_set_kinetic(self.handle, t.data_ptr(), L.stream_ptr(t.device)),
+                "gw_plan_set_kinetic")
+        self._t = t  # keep alive until the async copy has run
+
+    def set_potential(self, v_r: torch.Tensor) -> None:
+        v = v_r.detach().to(torch.float64).contiguous()
+        L.require_gpu(v, "v_eff_r")
+        L.check(self._lib.gw_plan_set_potential(self.handle, v.data_ptr(), L.stream_ptr(v.device)),
+                "gw_plan_set_potential")
+        self._v = v
+
+    def set_projectors(self, p: torch.Tensor, dij: torch.Tensor) -> None:
+        nproj = int(p.shape[1])
+        if nproj:
+            pp = p.detach().to(torch.complex128).contiguous()
+            L.require_gpu(pp, "p")
+            d = dij.detach().to("cpu", torch.float64).contiguous()
+            L.check(self._lib.gw_plan_set_projectors(self.handle, nproj, pp.data_ptr(), d.data_ptr(),
+                                                     L.stream_ptr(pp.device)),
+                    "gw_plan_set_projectors")
+            self._p = pp
+        else:
+            L.check(self._lib.gw_plan_set_projectors(self.handle, 0, None, None, None),
+                    "gw_plan_set_projectors")
+        self.nproj = nproj
+
+    def __del__(self) -> None:
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self._lib.gw_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = C.c_void_p()
+
+
+class BatchedHamiltonian:
+    """H apply for all k at once (core/batch.py:238-505), fixed V_eff(r) and projectors.
+
+    ``bk`` needs the BatchedK fields npw, mask, flat_idx, t and dij_full
+    (core/batch.py:126-171); ``v_eff_r`` (n1,n2,n3) real and ``p``
+    (nk, nproj, npw_max) complex live on the GPU. DFT+U projectors and the
+    USPP dual grid (batch.py:256-266) are outside this operator.
+    """
+
+    def __init__(self, bk, shape, v_eff_r: torch.Tensor, p: torch.Tensor, hub_q=None,
+                 hub_dij=None, smooth=None) -> None:
+        if hub_q is not None or hub_dij is not None or smooth is not None:
+            raise NotImplementedError("DFT+U and the USPP dual grid are not part of this operator")
+        L.require_gpu(bk.mask, "bk.mask")
+        self.bk = bk
+        self.shape = tuple(int(s) for s in shape)
+        self.n = self.shape[0] * self.shape[1] * self.shape[2]
+        self.v_eff_r = v_eff_r
+        self.p = p
+        self.plan = Plan(self.shape, bk.flat_idx, bk.npw)
+        self.plan.set_kinetic(bk.t)
+        self.plan.set_potential(v_eff_r)
+        self.plan.set_projectors(p, bk.dij_full)
+        self._ws: dict[int, torch.Tensor] = {}
+        self.device = bk.mask.device
+
+    def _workspace(self, nb: int, prec: int) -> torch.Tensor:
+        need = int(L.lib().gw_hpsi_workspace_bytes(self.plan.handle, nb, prec))
+        ws = self._ws.get(prec)
+        if ws is None or ws.numel() < need:
+            ws = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
+            self._ws[prec] = ws
+        return ws
+
+    def apply_terms(self, c: torch.Tensor, terms: int = L.GW_TERM_ALL,
+                    out: torch.Tensor | None = None) -> torch.Tensor:
+        if c.dim() != 3 or c.shape[0] != self.plan.nk or c.shape[2] != self.plan.npw_max:
+            raise ValueError(f"c must be (nk={self.plan.nk}, nb, npw_max={self.plan.npw_max}), "
+                             f"got {tuple(c.shape)}")
+        L.require_gpu(c, "c")
+        prec = L.precision_of(c.dtype)
+        if c.stride(2) != 1:
+            c = c.contiguous()
+        nb = int(c.shape[1])
+        if out is None:
+            out = torch.empty(c.shape, dtype=c.dtype, device=c.device)
+        if nb == 0:
+            return out
+        ws = self._workspace(nb, prec)
+        L.check(L.lib().gw_hpsi_apply_strided(
+            self.plan.handle, prec, terms, c.data_ptr(), c.stride(1), c.stride(0),
+            out.data_ptr(), out.stride(1), out.stride(0), nb, ws.data_ptr(), ws.numel(),
+            L.stream_ptr(c.device)), "gw_hpsi_apply")
+        return out
+
+    def apply(self, c: torch.Tensor) -> torch.Tensor:
+        """(nk, nb, npw_max) -> H c, mask preserved (core/batch.py:480-505)."""
+        if _HAPPLY_TALLY["on"]:
+            _HAPPLY_TALLY["count"] += int(c.shape[0] * c.shape[1])
+        return self.apply_terms(c, L.GW_TERM_ALL)
+
+    __call__ = apply
+
+    def apply_cols(self, c: torch.Tensor, kcol: torch.Tensor) -> torch.Tensor:
+        """H on a compacted column batch (ncol, npw_max) with k index kcol (core/batch.py:507-547).
+
+        The columns are grouped by k into a padded (nk, ncol_k_max, npw_max)
+        block, applied, and scattered back."""
+        if _HAPPLY_TALLY["on"]:
+            _HAPPLY_TALLY["count"] += int(c.shape[0])
+        ncol = int(c.shape[0])
+        out = torch.zeros_like(c)
+        if ncol == 0:
+            return out
+        kc = kcol.to(c.device)
+        counts = torch.bincount(kc, minlength=self.plan.nk)
+        width = int(counts.max())
+        order = torch.argsort(kc, stable=True)
+        starts = torch.cumsum(counts, 0) - counts
+        slot = torch.arange(ncol, device=c.device) - starts[kc[order]]
+        blk = torch.zeros(self.plan.nk, width, c.shape[1], dtype=c.dtype, device=c.device)
+        blk[kc[order], slot] = c[order]
+        hb = self.apply_terms(blk, L.GW_TERM_ALL)
+        out[order] = hb[kc[order], slot]
+        return out
+
+    def set_flags(self, flags: int) -> None:
+        L.check(L.lib().gw_plan_set_flags(self.plan.handle, int(flags)), "gw_plan_set_flags")
+
+    def stage_times(self, reset: bool = True) -> dict[str, tuple[float, int]]:
+        """Per-stage (summed ms, calls) from HIP events, when GW_PLAN_TIMING is set."""
+        ms = (C.c_double * 6)()
+        calls = (C.c_long * 6)()
+        L.check(L.lib().gw_plan_stage_times(self.plan.handle, ms, calls, 6, int(reset)),
+                "gw_plan_stage_times")
+        return {n: (ms[i], calls[i]) for i, n in enumerate(L.STAGES)}
+
+    def info(self, dtype: torch.dtype = torch.complex128) -> dict[str, int]:
+        v = (C.c_int * 11)()
+        L.check(L.lib().gw_plan_info(self.plan.handle, L.precision_of(dtype), v, 11), "gw_plan_info")
+        return {"active_planes": v[0], "max_sticks": v[1], "fused_planes": v[2], "lds_bytes": v[3],
+                "nproj": v[4], "split_x": (v[5], v[6]), "split_y": (v[7], v[8]),
+                "split_z": (v[9], v[10])}
+
+    def wfc_to_real(self, c: torch.Tensor) -> torch.Tensor:
+        """psi(r) = N ifftn(scatter c) on the natural box (nk, nb, n1, n2, n3)
+        -- core/batch.py:208-216 (g_to_r_b)."""
+        L.require_gpu(c, "c")
+        prec = L.precision_of(c.dtype)
+        c = c.contiguous()
+        nk, nb, _ = c.shape
+        psi = torch.empty(nk, nb, *self.shape, dtype=c.dtype, device=c.device)
+        ws = self._workspace(nb, prec)
+        L.check(L.lib().gw_wfc_to_real(self.plan.handle, prec, c.data_ptr(), psi.data_ptr(), nb,
+                                       ws.data_ptr(), ws.numel(), L.stream_ptr(c.device)),
+                "gw_wfc_to_real")
+        return psi

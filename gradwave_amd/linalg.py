@@ -1,0 +1,112 @@
+"""Thin tensor wrappers over the MFMA contractions and Davidson vector kernels.
+
+Every function takes GPU tensors whose last axis is contiguous and passes raw
+pointers plus strides (in complex elements) to libgradwave_hip.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from gradwave_amd import _lib as L
+
+_WS: dict[tuple, torch.Tensor] = {}
+
+
+def _scratch(device: torch.device, nbytes: int) -> torch.Tensor:
+    key = (device.type, device.index)
+    ws = _WS.get(key)
+    if ws is None or ws.numel() < nbytes:
+        ws = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, device=device)
+        _WS[key] = ws
+    return ws
+
+
+def _rows(t: torch.Tensor) -> torch.Tensor:
+    if t.stride(-1) != 1:
+        t = t.contiguous()
+    return t
+
+
+def inner(x: torch.Tensor, y: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    """C[k] = X[k]^* Y[k]^T, i.e. C[k][i][j] = sum_g conj(x[k][i][g]) y[k][j][g].
+
+    The reference's subspace matrix torch.matmul(v.conj(), hv.mT)
+    (solvers/davidson.py:648) and projection coefficients (davidson.py:431)."""
+    L.require_gpu(x, "x")
+    x, y = _rows(x), _rows(y)
+    nk, M, K = x.shape
+    N = y.shape[1]
+    assert y.shape[0] == nk and y.shape[2] == K and x.dtype == y.dtype
+    prec = L.precision_of(x.dtype)
+    if out is None:
+        out = torch.empty(nk, M, N, dtype=x.dtype, device=x.device)
+    need = int(L.lib().gw_zgemm_inner_workspace_bytes(prec, nk, M, N, K))
+    ws = _scratch(x.device, need)
+    L.check(L.lib().gw_zgemm_inner(prec, nk, M, N, K, x.data_ptr(), x.stride(1), x.stride(0),
+                                   y.data_ptr(), y.stride(1), y.stride(0), out.data_ptr(),
+                                   out.stride(1), out.stride(0), ws.data_ptr(), ws.numel(),
+                                   L.stream_ptr(x.device)), "gw_zgemm_inner")
+    return out
+
+
+def combine(u: torch.Tensor, y: torch.Tensor, out: torch.Tensor | None = None,
+            beta: float = 0.0, conj_u: bool = False) -> torch.Tensor:
+    """Z[k][a][g] = beta Z + sum_j u[k][j][a] y[k][j][g] -- the Ritz combination
+    einsum("kja,kjg->kag", u, v) of solvers/davidson.py:653-654."""
+    L.require_gpu(y, "y")
+    u, y = _rows(u), _rows(y)
+    nk, J, A = u.shape
+    G = y.shape[2]
+    assert y.shape[0] == nk and y.shape[1] == J and u.dtype == y.dtype
+    prec = L.precision_of(y.dtype)
+    if out is None:
+        out = torch.empty(nk, A, G, dtype=y.dtype, device=y.device)
+        beta = 0.0
+    L.check(L.lib().gw_zgemm_combine(prec, nk, A, G, J, u.data_ptr(), u.stride(1), u.stride(0),
+                                     int(conj_u), y.data_ptr(), y.stride(1), y.stride(0),
+                                     out.data_ptr(), out.stride(1), out.stride(0), float(beta),
+                                     L.stream_ptr(y.device)), "gw_zgemm_combine")
+    return out
+
+
+def residual(x: torch.Tensor, hx: torch.Tensor, eig: torch.Tensor):
+    """r = hx - eig x and rn = ||r|| in fp64 (solvers/davidson.py:656-657)."""
+    assert x.stride() == hx.stride() and x.stride(-1) == 1
+    nk, nb, m = x.shape
+    r = torch.empty_like(x)
+    assert r.stride() == x.stride()
+    rn = torch.empty(nk, nb, dtype=torch.float64, device=x.device)
+    e = eig.to(torch.float64).contiguous()
+    L.check(L.lib().gw_residual(L.precision_of(x.dtype), nk, nb, m, x.data_ptr(), hx.data_ptr(),
+                                x.stride(1), x.stride(0), e.data_ptr(), r.data_ptr(), rn.data_ptr(),
+                                L.stream_ptr(x.device)), "gw_residual")
+    return r, rn
+
+
+def teter_expand(x: torch.Tensor, r: torch.Tensor, t: torch.Tensor, sel: torch.Tensor,
+                 out: torch.Tensor | None = None) -> torch.Tensor:
+    """Teter-preconditioned, unit-normalised residuals of the selected bands
+    (solvers/davidson.py:729-743, solvers/precond.py:20-39)."""
+    assert x.stride() == r.stride() and x.stride(-1) == 1
+    nk, nb, m = x.shape
+    nsel = int(sel.shape[1])
+    if out is None:
+        out = torch.empty(nk, nsel, m, dtype=x.dtype, device=x.device)
+    tt = t.to(torch.float64).contiguous()
+    s = sel.to(torch.int64).contiguous()
+    L.check(L.lib().gw_teter_expand(L.precision_of(x.dtype), nk, nb, nsel, m, x.data_ptr(),
+                                    r.data_ptr(), x.stride(1), x.stride(0), tt.data_ptr(),
+                                    tt.stride(0), s.data_ptr(), out.data_ptr(), out.stride(1),
+                                    out.stride(0), L.stream_ptr(x.device)), "gw_teter_expand")
+    return out
+
+
+def row_norms(v: torch.Tensor) -> torch.Tensor:
+    v = _rows(v)
+    nk, mm, g = v.shape
+    out = torch.empty(nk, mm, dtype=torch.float64, device=v.device)
+    L.check(L.lib().gw_row_norms(L.precision_of(v.dtype), nk, mm, g, v.data_ptr(), v.stride(1),
+                                 v.stride(0), out.data_ptr(), L.stream_ptr(v.device)),
+            "gw_row_norms")
+    return out

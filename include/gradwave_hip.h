@@ -1,0 +1,141 @@
+/*
+ * gradwave_hip -- C ABI of the MI355X (gfx950) kernels behind gradwave's
+ * plane-wave H|psi> apply and batched block Davidson.
+ *
+ * Conventions (all entry points):
+ *   - plain pointers and sizes; device pointers are HIP device memory owned by
+ *     the caller (no ownership transfer); `stream` is a hipStream_t or NULL;
+ *   - complex data is interleaved (re, im), row-major, in the reference's
+ *     padded k-batch layout (nk, nb, npw_max) with zeros in padded slots
+ *     (core/batch.py:1-14);
+ *   - precision GW_FP64 = complex128/float64 data, GW_FP32 = complex64/float32;
+ *   - return 0 on success, a GW_ERR_* code otherwise; gw_last_error() gives
+ *     the message of the last failure on the calling thread.
+ *
+ * The reference (gradwave, pure Python/PyTorch) has no C ABI; each entry point
+ * names the reference function whose tensor contract it implements. A
+ * reference-side binding is the ctypes stub in INTEGRATION.md.
+ */
+#ifndef GRADWAVE_HIP_H
+#define GRADWAVE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  GW_OK = 0,
+  GW_ERR_ARG = 1,         /* bad argument / shape */
+  GW_ERR_HIP = 2,         /* HIP runtime error */
+  GW_ERR_UNSUPPORTED = 3, /* box length or size outside what the kernels handle */
+  GW_ERR_WORKSPACE = 4    /* workspace smaller than gw_*_workspace_bytes */
+};
+enum { GW_FP64 = 0, GW_FP32 = 1 };
+/* terms of H for gw_hpsi_apply (bit mask) */
+enum { GW_TERM_KINETIC = 1, GW_TERM_LOCAL = 2, GW_TERM_NONLOCAL = 4, GW_TERM_ALL = 7 };
+
+typedef struct gw_plan gw_plan;
+
+const char* gw_last_error(void);
+int gw_version(void);
+
+/* ---- operator set-up: BatchedHamiltonian.__init__ / _tables (core/batch.py:249-331) ---- */
+
+/* Geometry from the padded BatchedK maps (core/batch.py:126-205): flat_idx
+ * (nk, npw_max) int64 host array, 0 in padding; npw (nk) int64 host array.
+ * Box lengths must factor as a*b with a, b in {1..10,12,14,15,16}. */
+int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int npw_max,
+                   const int64_t* flat_idx, const int64_t* npw);
+int gw_plan_destroy(gw_plan* plan);
+/* plan options (no reference counterpart): GW_PLAN_LEGACY selects the generic
+ * LDS two-pass kernels (any 7-smooth box) instead of the wave-line kernels of
+ * cubic 24/36/90/120 boxes; GW_PLAN_FORCE_UNFUSED then splits each x-plane pass
+ * into stick and column-chunk kernels (both for testing). */
+enum { GW_PLAN_FORCE_UNFUSED = 1, GW_PLAN_TIMING = 2, GW_PLAN_LEGACY = 4 };
+int gw_plan_set_flags(gw_plan* plan, int flags);
+/* With GW_PLAN_TIMING set, gw_hpsi_apply records HIP events around each of its
+ * stages on the launch stream: 0 x-plane inverse (pack, z, y), 1 x pass
+ * (x inverse, V, x forward), 2 x-plane forward (y, z, gather, kinetic),
+ * 3 projection <p|c>, 4 D multiply, 5 back-projection. This returns the summed
+ * milliseconds and call counts per stage (synchronising on pending events). */
+int gw_plan_stage_times(gw_plan* plan, double* ms, long* calls, int n, int reset);
+/* info[0..10] = active x-planes, max sticks per k, fused plane path (0/1) for
+ * `precision` (2 wave-line, 1 LDS plane tile, 0 split), LDS bytes per
+ * workgroup, nproj, factor pairs of n1, n2, n3. */
+int gw_plan_info(const gw_plan* plan, int precision, int* info, int n);
+/* kinetic diagonal t = HBAR2_2M |k+G|^2 (nk, npw_max) float64, device (BatchedK.t) */
+int gw_plan_set_kinetic(gw_plan* plan, const double* t, void* stream);
+/* local potential v_eff_r (n1, n2, n3) float64, device (BatchedHamiltonian.v_eff_r) */
+int gw_plan_set_potential(gw_plan* plan, const double* v_r, void* stream);
+/* projectors p (nk, nproj, npw_max) complex128 device (projectors_b, core/batch.py:229-235)
+ * and D (nproj, nproj) float64 host (BatchedK.dij_full); nproj may be 0. */
+int gw_plan_set_projectors(gw_plan* plan, int nproj, const void* p, const double* dij,
+                           void* stream);
+
+/* ---- H|psi>: BatchedHamiltonian.apply (core/batch.py:480-505) ----
+ * out = (t*c + FFT^-1[V FFT c] + p D <p|c>) * mask for the terms in `terms`.
+ * c, out: (nk, nb, npw_max) complex device, may not alias. Workspace: device
+ * scratch of at least gw_hpsi_workspace_bytes(plan, nb, precision) bytes. */
+size_t gw_hpsi_workspace_bytes(const gw_plan* plan, int nb, int precision);
+int gw_hpsi_apply(gw_plan* plan, int precision, int terms, const void* c, void* out, int nb,
+                  void* workspace, size_t workspace_bytes, void* stream);
+/* Same on strided blocks: row (band) stride ldc/ldo and k stride sc/so, in
+ * complex elements (e.g. a band slice v[:, m0:m1] of a Davidson basis). */
+int gw_hpsi_apply_strided(gw_plan* plan, int precision, int terms, const void* c, long ldc,
+                          long sc, void* out, long ldo, long so, int nb, void* workspace,
+                          size_t workspace_bytes, void* stream);
+
+/* psi(r) = N ifftn(scatter(c)) on the natural box (nk, nb, n1, n2, n3) complex:
+ * core/fftbox.py:63-70 g_to_r / core/batch.py:208-216 g_to_r_b. */
+size_t gw_wfc_to_real_workspace_bytes(const gw_plan* plan, int nb, int precision);
+int gw_wfc_to_real(gw_plan* plan, int precision, const void* c, void* psi_r, int nb,
+                   void* workspace, size_t workspace_bytes, void* stream);
+
+/* Pack/unpack through the plan's plane/stick maps (bit-exact index parity):
+ * box (nk, nb, n1*n2*n3) complex = scatter of c by flat_idx (core/fftbox.py:29-37);
+ * c_out (nk, nb, npw_max) = gather of box by flat_idx, 0 in padding (fftbox.py:40-43). */
+int gw_sphere_to_box(gw_plan* plan, int precision, const void* c, void* box, int nb, void* stream);
+int gw_box_to_sphere(gw_plan* plan, int precision, const void* box, void* c_out, int nb,
+                     void* stream);
+
+/* ---- dense complex contractions on MFMA (solvers/davidson.py:428-432, 648-654) ----
+ * inner:   C[k][i][j] = sum_g conj(X[k][i][g]) * Y[k][j][g]
+ *          X (nk, M, K) row stride ldx, batch stride sx; Y (nk, N, K); C (nk, M, N).
+ *          e.g. s = v^H hv (davidson.py:648), becp (core/batch.py:590-595).
+ * combine: Z[k][a][g] = beta*Z[k][a][g] + sum_j U'[k][j][a] * Y[k][j][g],
+ *          U' = U or conj(U); U (nk, J, A); Y (nk, J, G); Z (nk, A, G).
+ *          e.g. Ritz vectors x = u^T v (davidson.py:653), back-projection (batch.py:500).
+ * Strides are in complex elements. The inner product needs scratch for its
+ * split-K partial sums: gw_zgemm_inner_workspace_bytes. */
+size_t gw_zgemm_inner_workspace_bytes(int precision, int nk, int M, int N, int K);
+int gw_zgemm_inner(int precision, int nk, int M, int N, int K, const void* X, long ldx, long sx,
+                   const void* Y, long ldy, long sy, void* C, long ldc, long sc,
+                   void* workspace, size_t workspace_bytes, void* stream);
+int gw_zgemm_combine(int precision, int nk, int A, int G, int J, const void* U, long ldu,
+                     long su, int conj_u, const void* Y, long ldy, long sy, void* Z, long ldz,
+                     long sz, double beta, void* stream);
+
+/* ---- Davidson vector kernels ----
+ * residual (davidson.py:656-657): r = hx - eig*x, rn[k][b] = ||r[k][b]|| (float64 out).
+ * x, hx, r: (nk, nb, npw) complex with row stride ld and batch stride s; eig (nk, nb) float64. */
+int gw_residual(int precision, int nk, int nb, int npw, const void* x, const void* hx,
+                long ld, long s, const double* eig, void* r, double* rn, void* stream);
+/* Teter preconditioned, unit-normalised expansion vectors (davidson.py:729-743,
+ * precond.py:20-39): for each selected band sel[k][j] (int64, (nk, nsel)):
+ *   tb = <x|T|x>, d = K(t/tb) * r[sel], d /= ||d|| (if > 1e-300).
+ * d: (nk, nsel, npw) complex out with row stride ldd, batch stride sd. */
+int gw_teter_expand(int precision, int nk, int nb, int nsel, int npw, const void* x,
+                    const void* r, long ld, long s, const double* t, long ldt,
+                    const int64_t* sel, void* d, long ldd, long sd, void* stream);
+/* row norms ||v[k][i]|| for (nk, m, npw) complex, float64 out (nk, m) */
+int gw_row_norms(int precision, int nk, int m, int npw, const void* v, long ld, long s,
+                 double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRADWAVE_HIP_H */

@@ -1,0 +1,98 @@
+"""GPU Davidson parity against the oracle: eigenvalues of the plane-wave H and
+of the reference's synthetic Hermitian operator, fp64 to the 1e-10 Ha tolerance,
+plus the mixed-precision two-stage solve within 1e-6 Ha."""
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_cases import dev, rand_block, ref_h, say, select_k, system
+
+pytestmark = pytest.mark.gpu
+
+from gradwave_amd import linalg as LA  # noqa: E402
+from gradwave_amd.davidson import davidson_batched, davidson_batched_ms, davidson_mixed  # noqa: E402
+from gradwave_amd.hamiltonian import BatchedHamiltonian  # noqa: E402
+from oracle.davidson import davidson_batched as ref_davidson  # noqa: E402
+from oracle.dense import dense_h  # noqa: E402
+from oracle.units import HARTREE_EV  # noqa: E402
+
+TOL = 1e-10 * HARTREE_EV  # 1e-10 Ha residual-norm tolerance, in eV
+
+
+def test_si2_all_k_fp64_matches_oracle_and_dense():
+    say("davidson si2 10k fp64")
+    s = system("si2")
+    cpu, gpu, p = select_k(s)
+    h = BatchedHamiltonian(gpu, s.shape, s.v_r.to(dev()), p.to(dev()))
+    x0 = rand_block(cpu.nk, 8, cpu.npw_max, cpu.mask, seed=0)
+    r = davidson_batched(h.apply, x0.to(dev()), gpu.t, gpu.mask, tol=TOL, max_iter=100)
+    assert float(r.residual_norms.max()) < TOL
+    rr = ref_davidson(ref_h(s).apply, x0, cpu.t, cpu.mask, tol=TOL, max_iter=100)
+    assert float((r.eigenvalues.cpu() - rr.eigenvalues).abs().max()) < 1e-10 * HARTREE_EV
+    for ik in (0, 8):
+        e = torch.linalg.eigvalsh(dense_h(s, ik))[:8]
+        assert float((r.eigenvalues[ik].cpu() - e).abs().max()) < 1e-10 * HARTREE_EV
+    # returned vectors: orthonormal rows, zero padding, residuals re-measured
+    x = r.eigenvectors
+    gram = LA.inner(x, x).cpu()
+    assert float((gram - torch.eye(8, dtype=gram.dtype)).abs().max()) < 1e-10
+    assert float(x.cpu()[~cpu.mask[:, None, :].expand_as(x)].abs().max()) == 0.0
+    hx = h.apply(x)
+    rn = torch.linalg.norm((hx - r.eigenvalues[..., None] * x).cpu(), dim=-1)
+    assert float((rn - r.residual_norms.cpu()).abs().max()) < 1e-12
+
+
+def _synthetic(nk, npw, seed):
+    """The reference's synthetic operator (test_davidson_fp32_expansion.py:24-42)."""
+    torch.manual_seed(seed)
+    diag = torch.linspace(1.0, 100.0, npw, dtype=torch.float64)
+    h = torch.zeros(nk, npw, npw, dtype=torch.complex128)
+    for k in range(nk):
+        a = torch.randn(npw, npw, dtype=torch.complex128)
+        h[k] = 0.5 * (a + a.conj().T) + torch.diag(diag.to(torch.complex128))
+        h[k] = 0.5 * (h[k] + h[k].conj().T)
+    return h, diag.expand(nk, npw).contiguous()
+
+
+@pytest.mark.parametrize("mode", ["off", "on"])
+def test_synthetic_operator_and_fp32_expansion(mode, monkeypatch):
+    say(f"davidson synthetic fp32_expansion={mode}")
+    import os
+
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_golden.npz"))
+    h = torch.as_tensor(gold["syn_h"])
+    x0 = torch.as_tensor(gold["syn_x0"])
+    t = torch.linspace(1.0, 100.0, 80, dtype=torch.float64).expand(4, 80).contiguous()
+    hT = h.transpose(1, 2).contiguous().to(dev())
+
+    def apply(c):  # out[b][i] = sum_j h[i][j] c[b][j], recomputed in c's precision
+        return LA.combine(c.transpose(1, 2).contiguous(), hT.to(c.dtype))
+
+    monkeypatch.setenv("GRADWAVE_FP32_EXPANSION", mode)
+    mask = torch.ones(4, 80, dtype=torch.bool, device=dev())
+    r = davidson_batched(apply, x0.to(dev()), t.to(dev()), mask, tol=1e-9, max_iter=100)
+    assert float(r.residual_norms.max()) < 1e-9
+    ref = torch.linalg.eigvalsh(h)[:, :8]
+    assert float((r.eigenvalues.cpu() - ref).abs().max()) < 1e-9
+    assert float((r.eigenvalues.cpu() - torch.as_tensor(gold[f"syn_{mode}_eig"])).abs().max()) < 1e-9
+    if mode == "on":
+        assert r.n_apply_low > 0 and r.n_apply_full > 0
+
+
+def test_si8_mixed_precision_two_stage():
+    """Precision form on a multi-k cell: complex64 draft + complex128 polish vs the fp64 solve."""
+    say("davidson si8 mixed vs fp64")
+    s = system("si8")
+    sel = [0, 7, 20]
+    cpu, gpu, p = select_k(s, sel)
+    h = BatchedHamiltonian(gpu, s.shape, s.v_r.to(dev()), p.to(dev()))
+    x0 = rand_block(3, 24, cpu.npw_max, cpu.mask, seed=1).to(dev())
+    r64 = davidson_batched(h.apply, x0, gpu.t, gpu.mask, tol=TOL, max_iter=200)
+    rms = davidson_mixed(h.apply, x0, gpu.t, gpu.mask, tol=TOL, draft_tol=1e-3, max_iter=200)
+    assert float(r64.residual_norms.max()) < TOL and float(rms.residual_norms.max()) < TOL
+    assert float((rms.eigenvalues - r64.eigenvalues).abs().max()) < 1e-6 * HARTREE_EV
+    assert rms.n_apply_low > 0
+    # the reference's two-stage wrapper skips its draft when crossover >= tol (_ms.py:47)
+    r_skip = davidson_batched_ms(h.apply, x0, gpu.t, gpu.mask, tol=TOL, max_iter=200, crossover=1e-4)
+    assert r_skip.n_apply_low == 0 and float(r_skip.residual_norms.max()) < TOL

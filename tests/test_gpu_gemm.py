@@ -1,0 +1,86 @@
+"""GPU parity of the MFMA complex contractions against CPU complex128 products.
+
+Shapes cover tile edges (not multiples of 16/64), batches, strided views,
+the conjugated combine and beta accumulation.
+"""
+
+import pytest
+import torch
+
+from gpu_cases import dev, say
+
+pytestmark = pytest.mark.gpu
+
+from gradwave_amd import linalg as LA  # noqa: E402
+
+
+def _rand(*shape, seed=0, dtype=torch.complex128):
+    g = torch.Generator().manual_seed(seed)
+    return torch.randn(*shape, dtype=torch.complex128, generator=g).to(dtype)
+
+
+TOL = {torch.complex128: 1e-13, torch.complex64: 3e-6}
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+@pytest.mark.parametrize("nk,M,N,K", [(1, 8, 8, 763), (3, 17, 33, 1000), (2, 70, 5, 4097),
+                                      (1, 160, 160, 37073), (1, 256, 96, 2969), (4, 1, 1, 3)])
+def test_inner(dtype, nk, M, N, K):
+    say(f"inner {dtype} {nk}x{M}x{N}x{K}")
+    x, y = _rand(nk, M, K, seed=1, dtype=dtype), _rand(nk, N, K, seed=2, dtype=dtype)
+    want = torch.matmul(x.to(torch.complex128).conj(), y.to(torch.complex128).mT)
+    got = LA.inner(x.to(dev()), y.to(dev())).cpu().to(torch.complex128)
+    scale = float((x.abs() ** 2).sum(-1).max().sqrt() * (y.abs() ** 2).sum(-1).max().sqrt())
+    assert float((got - want).abs().max()) < TOL[dtype] * scale
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+@pytest.mark.parametrize("nk,A,G,J", [(1, 8, 763, 8), (3, 33, 1000, 17), (2, 5, 4097, 70),
+                                      (1, 160, 37073, 256), (1, 24, 2969, 96), (2, 1, 3, 1)])
+@pytest.mark.parametrize("conj_u,beta", [(False, 0.0), (True, 1.0)])
+def test_combine(dtype, nk, A, G, J, conj_u, beta):
+    say(f"combine {dtype} {nk}x{A}x{G}x{J} conj={conj_u} beta={beta}")
+    u, y, z0 = _rand(nk, J, A, seed=3, dtype=dtype), _rand(nk, J, G, seed=4, dtype=dtype), \
+        _rand(nk, A, G, seed=5, dtype=dtype)
+    uu = u.to(torch.complex128)
+    uu = uu.conj() if conj_u else uu
+    want = beta * z0.to(torch.complex128) + torch.einsum("kja,kjg->kag", uu, y.to(torch.complex128))
+    zd = z0.to(dev())
+    got = LA.combine(u.to(dev()), y.to(dev()), out=zd, beta=beta, conj_u=conj_u).cpu()
+    scale = float(u.abs().max() * y.abs().max() * J) + beta * float(z0.abs().max())
+    assert float((got.to(torch.complex128) - want).abs().max()) < TOL[dtype] * scale
+
+
+def test_strided_views():
+    say("inner/combine on strided views")
+    big = _rand(2, 40, 300, seed=6).to(dev())
+    x, y = big[:, 3:11], big[:, 20:37]
+    want = torch.matmul(x.cpu().conj(), y.cpu().mT)
+    assert float((LA.inner(x, y).cpu() - want).abs().max()) < 1e-12
+    u = _rand(2, 8, 17, seed=7).to(dev())
+    out = torch.zeros(2, 50, 300, dtype=torch.complex128, device=dev())
+    LA.combine(u, x, out=out[:, 30:47], beta=0.0)
+    want2 = torch.einsum("kja,kjg->kag", u.cpu(), x.cpu())
+    assert float((out[:, 30:47].cpu() - want2).abs().max()) < 1e-12
+    assert float(out[:, :30].abs().max()) == 0.0 and float(out[:, 47:].abs().max()) == 0.0
+
+
+def test_residual_and_teter_kernels():
+    say("residual + teter kernels")
+    from oracle.davidson import teter_b
+
+    nk, nb, m = 3, 6, 500
+    x, hx = _rand(nk, nb, m, seed=8), _rand(nk, nb, m, seed=9)
+    eig = torch.randn(nk, nb, dtype=torch.float64)
+    r, rn = LA.residual(x.to(dev()), hx.to(dev()), eig.to(dev()))
+    r_ref = hx - eig[..., None] * x
+    assert float((r.cpu() - r_ref).abs().max()) < 1e-13
+    assert float((rn.cpu() - torch.linalg.norm(r_ref, dim=-1)).abs().max()) < 1e-12
+    t = torch.rand(nk, m, dtype=torch.float64) * 50
+    t[:, -20:] = 0.0  # padded slots: K(0) = 1
+    sel = torch.argsort(rn.cpu(), dim=1, descending=True)[:, :4]
+    d = LA.teter_expand(x.to(dev()), r, t.to(dev()), sel.to(dev())).cpu()
+    tb = torch.einsum("kbg,kg,kbg->kb", x.conj(), t.to(x.dtype), x).real
+    dref = teter_b(torch.gather(r_ref, 1, sel[..., None].expand(-1, -1, m)), t, torch.gather(tb, 1, sel))
+    dref = dref / torch.linalg.norm(dref, dim=-1, keepdim=True)
+    assert float((d - dref).abs().max()) < 1e-13
