@@ -136,7 +136,7 @@ def main() -> None:
         roof = {"kernel": dom, "bound": "mfma", "achieved": work / t_dom / 1e12,
                 "peak": MFMA_PEAK["fp32"], "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = None
+    roof["traffic"] = profiled_traffic(args.workload, dom, "float")
     roof["avg_ms"] = st32[dom]
     roof["algorithmic_per_launch"] = work
     roof["stage_ms_fp32"] = st32
@@ -200,6 +200,31 @@ def main() -> None:
         result["cpu_baseline"] = cpu_baseline(args, wl)
 
     print(json.dumps(result), flush=True)
+
+
+STAGE_KERNEL = {"plane_inv": "k_plane_inv_w", "xpass": "k_xpass_wave", "plane_fwd": "k_plane_fwd_w",
+                "becp": "k_gemm_inner_lds", "dmul": "k_dmul", "backproj": "k_gemm_combine_lds"}
+PROFILE_OF = {"si64_gamma": "hpsi_apply_si64_gamma_90cubed.txt"}
+
+
+def profiled_traffic(workload: str, stage: str, real: str):
+    """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE) of the stage's
+    kernel, read from the rocprofv3 summary committed under profiles/, or None."""
+    name = PROFILE_OF.get(workload)
+    if name is None:
+        return None
+    path = os.path.join(HERE, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    key = f"gw::{STAGE_KERNEL[stage]}<{real}"
+    for line in open(path):
+        if line.startswith(key):
+            f = line.split()
+            try:
+                return (float(f[-3]) + float(f[-2])) * 1e6
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(args, wl) -> dict:

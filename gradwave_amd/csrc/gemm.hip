@@ -19,6 +19,7 @@
 // global memory straight into VGPRs (no LDS staging yet). MFMA 16x16x4 lane
 // layout: A[i=l%16][k=l/16], B[k=l/16][j=l%16], D col l%16 and row Mfma<T>::row.
 #include <algorithm>
+#include <cstdlib>
 
 #include "gw_internal.h"
 
@@ -165,6 +166,214 @@ __global__ void k_gemm_inner_reduce(int M, int N, int splits, const double* __re
   C[(long)k * sc + (long)i * ldc + j] = out;
 }
 
+
+// ---------------------------------------------------------------------------
+// LDS-staged versions: a workgroup stages a 64-row x 32-complex K-chunk of each
+// operand in padded LDS tiles (16-byte global loads, next chunk prefetched into
+// registers while the MFMAs of the current one run); each lane then feeds the
+// MFMAs from its contiguous 8-complex K segment of the tile.
+constexpr int kKC = 32;  // complex K elements per chunk
+
+template <typename T> struct Lds;
+template <> struct Lds<double> { static constexpr int PAD = 1; };  // 16-byte row skew
+template <> struct Lds<float> { static constexpr int PAD = 2; };
+template <typename T> constexpr int kLd = kKC + Lds<T>::PAD;  // LDS row stride (complex)
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_inner_lds(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long sx,
+                 const cpx<T>* __restrict__ Y, long ldy, long sy, int splits, int kchunk,
+                 double* __restrict__ part, int ntx, int nty, long total) {
+  using A4 = typename Mfma<T>::acc;
+  constexpr int LD = kLd<T>;
+  __shared__ cpx<T> As[64 * LD];
+  __shared__ cpx<T> Bs[32 * LD];
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const int bx = (int)(wid % ntx), by = (int)((wid / ntx) % nty), bz = (int)(wid / ((long)ntx * nty));
+  const int k = bz / splits, sp = bz - k * splits;
+  const int gb = sp * kchunk, ge = min(K, gb + kchunk);
+  const int row0 = by * 64, col0 = bx * 32;  // X rows, Y rows (complex columns of C)
+  const cpx<T>* Xk = X + (long)k * sx;
+  const cpx<T>* Yk = Y + (long)k * sy;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = w >> 1, wn = w & 1;
+  // global -> register staging: 64x32 of X (8 per thread) and 32x32 of Y (4 per thread)
+  cpx<T> ra[8], rb[4];
+  auto gload = [&](int g0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = tid + 256 * i, r = e >> 5, cc = e & 31;
+      const int row = row0 + r, g = g0 + cc;
+      ra[i] = (row < M && g < ge) ? Xk[(long)row * ldx + g] : cpx<T>{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int e = tid + 256 * i, r = e >> 5, cc = e & 31;
+      const int row = col0 + r, g = g0 + cc;
+      rb[i] = (row < N && g < ge) ? Yk[(long)row * ldy + g] : cpx<T>{0, 0};
+    }
+  };
+  A4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = A4{0, 0, 0, 0};
+  if (gb < ge) gload(gb);
+  for (int g0 = gb; g0 < ge; g0 += kKC) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) { const int e = tid + 256 * i; As[(e >> 5) * LD + (e & 31)] = ra[i]; }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { const int e = tid + 256 * i; Bs[(e >> 5) * LD + (e & 31)] = rb[i]; }
+    __syncthreads();
+    if (g0 + kKC < ge) gload(g0 + kKC);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {  // two halves of the lane's 8-complex K segment
+      T av[2][8], bv[2][8];
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm) {
+        const cpx<T>* ap = As + (wm * 32 + bm * 16 + li) * LD + lk * 8 + h * 4;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { const cpx<T> x = ap[v]; av[bm][2 * v] = x.x; av[bm][2 * v + 1] = x.y; }
+      }
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn) {
+        const int jj = wn * 32 + bn * 16 + li;
+        const cpx<T>* bp = Bs + (jj >> 1) * LD + lk * 8 + h * 4;
+        const bool odd = jj & 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const cpx<T> y = bp[v];
+          bv[bn][2 * v] = odd ? y.y : y.x;
+          bv[bn][2 * v + 1] = odd ? -y.x : y.y;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+          for (int bn = 0; bn < 2; ++bn) acc[bm][bn] = Mfma<T>::op(av[bm][v], bv[bn][v], acc[bm][bn]);
+    }
+  }
+  const int ncol = 2 * N;
+  double* pk = part + ((long)k * splits + sp) * M * ncol;
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + wm * 32 + bm * 16 + Mfma<T>::row(lk, r);
+        const int col = 2 * col0 + wn * 32 + bn * 16 + li;
+        if (row < M && col < ncol) pk[(long)row * ncol + col] = (double)acc[bm][bn][r];
+      }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_combine_lds(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu, long su, int conj_u,
+                   const cpx<T>* __restrict__ Y, long ldy, long sy, cpx<T>* __restrict__ Z, long ldz, long sz,
+                   T beta, int ntx, int nty, long total) {
+  using A4 = typename Mfma<T>::acc;
+  constexpr int LD = kLd<T>;
+  __shared__ cpx<T> Ut[64 * LD];  // [a][j]
+  __shared__ cpx<T> Yt[32 * LD];  // [g][j]
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const int by = (int)(wid % nty), bx = (int)((wid / nty) % ntx), k = (int)(wid / ((long)ntx * nty));
+  const int a0 = by * 64, g0c = bx * 32;
+  const cpx<T>* Uk = U + (long)k * su;
+  const cpx<T>* Yk = Y + (long)k * sy;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = w >> 1, wn = w & 1;
+  const T usign = conj_u ? T(-1) : T(1);
+  cpx<T> ru[8], ry[4];
+  auto gload = [&](int j0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {  // U rows j (32) x columns a (64), a fastest
+      const int e = tid + 256 * i, jr = e >> 6, ac = e & 63;
+      const int j = j0 + jr, a = a0 + ac;
+      ru[i] = (j < J && a < A) ? Uk[(long)j * ldu + a] : cpx<T>{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {  // Y rows j (32) x columns g (32), g fastest
+      const int e = tid + 256 * i, jr = e >> 5, gc = e & 31;
+      const int j = j0 + jr, g = g0c + gc;
+      ry[i] = (j < J && g < G) ? Yk[(long)j * ldy + g] : cpx<T>{0, 0};
+    }
+  };
+  A4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = A4{0, 0, 0, 0};
+  if (J > 0) gload(0);
+  for (int j0 = 0; j0 < J; j0 += kKC) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) { const int e = tid + 256 * i; Ut[(e & 63) * LD + (e >> 6)] = ru[i]; }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) { const int e = tid + 256 * i; Yt[(e & 31) * LD + (e >> 5)] = ry[i]; }
+    __syncthreads();
+    if (j0 + kKC < J) gload(j0 + kKC);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      T av[2][8], bv[2][8];
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm) {
+        const cpx<T>* up = Ut + (wm * 32 + bm * 16 + li) * LD + lk * 8 + h * 4;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { const cpx<T> u = up[v]; av[bm][2 * v] = u.x; av[bm][2 * v + 1] = usign * u.y; }
+      }
+#pragma unroll
+      for (int bn = 0; bn < 2; ++bn) {
+        const int col = wn * 32 + bn * 16 + li;
+        const cpx<T>* yp = Yt + (col >> 1) * LD + lk * 8 + h * 4;
+        const bool im = col & 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const cpx<T> y = yp[v];
+          bv[bn][2 * v] = im ? y.y : y.x;
+          bv[bn][2 * v + 1] = im ? y.x : -y.y;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+          for (int bn = 0; bn < 2; ++bn) acc[bm][bn] = Mfma<T>::op(av[bm][v], bv[bn][v], acc[bm][bn]);
+    }
+  }
+  const int ncol = 2 * G;
+  T* Zk = reinterpret_cast<T*>(Z + (long)k * sz);
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = a0 + wm * 32 + bm * 16 + Mfma<T>::row(lk, r);
+        const int col = 2 * g0c + wn * 32 + bn * 16 + li;
+        if (row < A && col < ncol) {
+          T* zp = Zk + (long)row * 2 * ldz + col;
+          const T v = acc[bm][bn][r];
+          *zp = (beta != T(0)) ? v + beta * (*zp) : v;
+        }
+      }
+}
+
+static bool gemm_simple() {
+  static const bool v = [] { const char* e = getenv("GW_GEMM_SIMPLE"); return e && e[0] == '1'; }();
+  return v;
+}
+
 static int inner_splits(int nk, int M, int N, int K) {
   const long tiles = (long)((2 * N + kTile - 1) / kTile) * ((M + kTile - 1) / kTile) * nk;
   long s = (2048 + tiles - 1) / tiles;
@@ -195,11 +404,15 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
   const size_t need = (size_t)nk * splits * M * 2 * N * sizeof(double);
   GW_REQUIRE(partials && part_bytes >= need, GW_ERR_WORKSPACE, "gemm_inner workspace %zu < %zu", part_bytes, need);
   int kchunk = (K + splits - 1) / splits;
-  kchunk = ((kchunk + kStepC - 1) / kStepC) * kStepC;
+  kchunk = ((kchunk + kKC - 1) / kKC) * kKC;  // multiple of both K granules (16, 32)
   const int ntx = (2 * N + kTile - 1) / kTile, nty = (M + kTile - 1) / kTile;
   const long total = (long)ntx * nty * nk * splits;
-  hipLaunchKernelGGL(k_gemm_inner<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X, ldx,
-                     sx, Y, ldy, sy, splits, kchunk, (double*)partials, ntx, nty, total);
+  if (gemm_simple())
+    hipLaunchKernelGGL(k_gemm_inner<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X, ldx,
+                       sx, Y, ldy, sy, splits, kchunk, (double*)partials, ntx, nty, total);
+  else
+    hipLaunchKernelGGL(k_gemm_inner_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X,
+                       ldx, sx, Y, ldy, sy, splits, kchunk, (double*)partials, ntx, nty, total);
   const long tot = (long)M * N;
   hipLaunchKernelGGL(k_gemm_inner_reduce<T>, dim3((tot + 255) / 256, nk), dim3(256), 0, st, M, N, splits,
                      (const double*)partials, C, ldc, sc);
@@ -298,8 +511,12 @@ int gemm_combine(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su
   if (A <= 0 || G <= 0 || nk <= 0) return GW_OK;
   const int ntx = (2 * G + kTile - 1) / kTile, nty = (A + kTile - 1) / kTile;
   const long total = (long)ntx * nty * nk;
-  hipLaunchKernelGGL(k_gemm_combine<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, A, G, J, U, ldu,
-                     su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, ntx, nty, total);
+  if (gemm_simple())
+    hipLaunchKernelGGL(k_gemm_combine<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, A, G, J, U,
+                       ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, ntx, nty, total);
+  else
+    hipLaunchKernelGGL(k_gemm_combine_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, A, G, J,
+                       U, ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, ntx, nty, total);
   GW_CHECK_HIP(hipGetLastError());
   return GW_OK;
 }

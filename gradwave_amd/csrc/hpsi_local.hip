@@ -143,6 +143,16 @@ static Geom geom_of(const Plan* p, int nb) {
 
 template <typename T> __device__ __forceinline__ cpx<T> czero() { cpx<T> z; z.x = 0; z.y = 0; return z; }
 
+// Linear workgroup id -> work id so that consecutive work ids share an XCD
+// (dispatch places workgroup L on XCD L % 8). Returns -1 for the padding ids.
+__device__ __forceinline__ long xcd_work_id(long total) {
+  const long L = blockIdx.x;
+  const long per = (total + 7) / 8;
+  const long w = (L % 8) * per + L / 8;
+  return w < total ? w : -1;
+}
+
+
 // P1 fused: scatter -> inverse z (sticks) -> inverse y (all columns) -> store plane.
 template <typename T>
 __global__ void __launch_bounds__(kPlaneThreads)
@@ -233,18 +243,22 @@ k_xpass(Geom g, int L, int nbt, cpx<T>* __restrict__ box, const T* __restrict__ 
 template <typename T, int N, int M, int S>
 __global__ void __launch_bounds__(256)
 k_xpass_wave(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __restrict__ vx,
-             const cpx<T>* __restrict__ twx) {
+             const cpx<T>* __restrict__ twx, int nchunk, long total) {
   using W = WaveLine<T, N, M, S>;
   constexpr int LW = W::LW;
   __shared__ cpx<T> E[4 * W::E_PER_WAVE];
   __shared__ cpx<T> tw[N];
+  // neighbouring line chunks share cache lines of every plane row: keep them on one XCD
+  const long wid = xcd_work_id(total);
+  if (wid < 0) return;
+  const int chunk = (int)(wid % nchunk), bgi = (int)(wid / nchunk);
   for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twx[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = lane / LW, l = lane - t * LW;
   const bool part = t < S;
   const int n23 = g.n2 * g.n3;
-  const int j = (blockIdx.x * 4 + wave) * LW + l;
+  const int j = (chunk * 4 + wave) * LW + l;
   const bool jok = j < n23;
   cpx<T>* Ew = E + wave * W::E_PER_WAVE;
   // the sphere's x-planes are [0, xlo) U [xhi, N)
@@ -252,22 +266,30 @@ k_xpass_wave(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __r
   T vr[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) vr[s] = (t < M && jok) ? vx[(long)(t + M * s) * n23 + j] : T(0);
-  for (int bg = 0; bg < bgroup; ++bg) {
-    const int bi = blockIdx.y * bgroup + bg;
-    if (bi >= nbt) break;
-    cpx<T>* bb = box + (long)bi * g.na * n23 + j;
-    cpx<T> x[M], y[S];
+  auto load = [&](int bi, cpx<T> (&v)[M]) {
+    const cpx<T>* bb = box + (long)bi * g.na * n23 + j;
 #pragma unroll
     for (int q = 0; q < M; ++q) {
       const int sl = part ? slot(S * q + t) : -1;
-      x[q] = (jok && sl >= 0) ? bb[(long)sl * n23] : czero<T>();
+      v[q] = (jok && sl >= 0) ? bb[(long)sl * n23] : czero<T>();
     }
+  };
+  const int b0 = bgi * bgroup;
+  const int nbg = min(bgroup, nbt - b0);
+  cpx<T> x[M], xn[M], y[S];
+  if (nbg > 0) load(b0, xn);
+  for (int bg = 0; bg < nbg; ++bg) {
+    const int bi = b0 + bg;
+#pragma unroll
+    for (int q = 0; q < M; ++q) x[q] = xn[q];
+    if (bg + 1 < nbg) load(bi + 1, xn);  // next band's loads overlap this band's transforms
 #pragma unroll
     for (int s = 0; s < S; ++s) y[s] = czero<T>();
     W::inverse(x, y, t, l, part, Ew, tw);
 #pragma unroll
     for (int s = 0; s < S; ++s) { y[s].x *= vr[s]; y[s].y *= vr[s]; }
     W::forward(y, x, t, l, part, Ew, tw);
+    cpx<T>* bb = box + (long)bi * g.na * n23 + j;
 #pragma unroll
     for (int q = 0; q < M; ++q) {
       const int sl = part ? slot(S * q + t) : -1;
@@ -493,26 +515,22 @@ __global__ void k_dmul(int nproj, int nb, const int* __restrict__ colptr, const 
 
 // ---- wave-line path: natural order on every axis, cubic y/z ----
 
-// Linear workgroup id -> work id so that consecutive work ids share an XCD
-// (dispatch places workgroup L on XCD L % 8). Returns -1 for the padding ids.
-__device__ __forceinline__ long xcd_work_id(long total) {
-  const long L = blockIdx.x;
-  const long per = (total + 7) / 8;
-  const long w = (L % 8) * per + L / 8;
-  return w < total ? w : -1;
-}
 
 // P1: scatter -> z inverse on compact stick rows (LDS) -> y inverse on columns,
 // natural-order rows stored straight to the box.
-template <typename T, int N, int M, int S>
-__global__ void __launch_bounds__(256)
+// waves per workgroup of the wave-line plane kernels: kPW, or 4 when the
+// compact stick rows plus kPW exchange areas exceed LDS (fp64, 120^3)
+constexpr int kPW = 8;
+
+template <typename T, int N, int M, int S, int NW>
+__global__ void __launch_bounds__(64 * NW)
 k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long sc,
               cpx<T>* __restrict__ box, const cpx<T>* __restrict__ twg) {
   using W = WaveLine<T, N, M, S>;
   constexpr int LW = W::LW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* E = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* tw = E + 4 * W::E_PER_WAVE;
+  cpx<T>* tw = E + NW * W::E_PER_WAVE;
   cpx<T>* rows = tw + N;
   int* map = reinterpret_cast<int*>(rows + (long)g.st_max_plane * N);
   const long wid = xcd_work_id(total);
@@ -531,7 +549,7 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) map[i1s[s]] = s - s0;
-  for (int s = s0 + wave; s < s1; s += 4)
+  for (int s = s0 + wave; s < s1; s += NW)
     for (int q = pp[s] + lane; q < pp[s + 1]; q += 64) rows[(s - s0) * N + yz[q] % N] = cb[src[q]];
   __syncthreads();
   const int t = lane / LW, l = lane - t * LW;
@@ -539,7 +557,7 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
   cpx<T>* Ew = E + wave * W::E_PER_WAVE;
   cpx<T> x[M], y[S];
   // z inverse on the plane's sticks (rows of `rows`)
-  for (int r0 = wave * LW; r0 < ns; r0 += 4 * LW) {
+  for (int r0 = wave * LW; r0 < ns; r0 += NW * LW) {
     const int row = r0 + l;
     const bool ok = part && row < ns;
 #pragma unroll
@@ -555,7 +573,7 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
   __syncthreads();
   // y inverse on all columns; column input is non-zero only at stick rows
   cpx<T>* dst = box + ((long)bi * g.na + a) * N * N;
-  for (int c0 = wave * LW; c0 < N; c0 += 4 * LW) {
+  for (int c0 = wave * LW; c0 < N; c0 += NW * LW) {
     const int col = c0 + l;
     const bool ok = part && col < N;
 #pragma unroll
@@ -574,8 +592,8 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
 }
 
 // P3: y forward on columns from the box -> stick rows (LDS) -> z forward -> gather (+ t c).
-template <typename T, int N, int M, int S, bool KIN>
-__global__ void __launch_bounds__(256)
+template <typename T, int N, int M, int S, bool KIN, int NW>
+__global__ void __launch_bounds__(64 * NW)
 k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ c, long ldc,
               long sc, const T* __restrict__ tk_all, cpx<T>* __restrict__ out, long ldo, long so, T scale,
               const cpx<T>* __restrict__ twg) {
@@ -583,7 +601,7 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
   constexpr int LW = W::LW;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* E = reinterpret_cast<cpx<T>*>(smem_raw);
-  cpx<T>* tw = E + 4 * W::E_PER_WAVE;
+  cpx<T>* tw = E + NW * W::E_PER_WAVE;
   cpx<T>* rows = tw + N;
   int* map = reinterpret_cast<int*>(rows + (long)g.st_max_plane * N);
   const long wid = xcd_work_id(total);
@@ -609,7 +627,7 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
   cpx<T>* Ew = E + wave * W::E_PER_WAVE;
   cpx<T> x[M], y[S];
   const cpx<T>* srcp = box + ((long)bi * g.na + a) * N * N;
-  for (int c0 = wave * LW; c0 < N; c0 += 4 * LW) {
+  for (int c0 = wave * LW; c0 < N; c0 += NW * LW) {
     const int col = c0 + l;
     const bool ok = col < N;
 #pragma unroll
@@ -624,7 +642,7 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
     }
   }
   __syncthreads();
-  for (int r0 = wave * LW; r0 < ns; r0 += 4 * LW) {
+  for (int r0 = wave * LW; r0 < ns; r0 += NW * LW) {
     const int row = r0 + l;
     const bool ok = row < ns;
 #pragma unroll
@@ -640,7 +658,7 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
   const int* src = g.pt_src + (long)k * g.npw_max;
   const int* yz = g.pt_yz + (long)k * g.npw_max;
   const T* tk = tk_all + (long)k * g.npw_max;
-  for (int s = s0 + wave; s < s1; s += 4)
+  for (int s = s0 + wave; s < s1; s += NW)
     for (int q = pp[s] + lane; q < pp[s + 1]; q += 64) {
       const int si = src[q];
       cpx<T> v = rows[(s - s0) * N + yz[q] % N];
@@ -664,14 +682,43 @@ template <> struct WaveCfg<120> { static constexpr int M = 10, S = 12; };
 
 static bool wave_length(int n) { return n == 24 || n == 36 || n == 90 || n == 120; }
 
-template <typename T, int N>
+template <typename T, int N, int NW>
 static size_t plane_w_lds(const Plan& P) {
   using C = WaveCfg<N>;
   using W = WaveLine<T, N, C::M, C::S>;
-  return (size_t)(4 * W::E_PER_WAVE + N + (size_t)P.st_max_plane * N) * sizeof(cpx<T>) + N * sizeof(int);
+  return (size_t)(NW * W::E_PER_WAVE + N + (size_t)std::max(1, P.st_max_plane) * N) * sizeof(cpx<T>) +
+         N * sizeof(int);
+}
+
+// waves per plane workgroup for this plan, 0 if even 4 waves do not fit
+template <typename T, int N>
+static int plane_waves(const Plan& P) {
+  const size_t cap = (size_t)std::min(P.lds_max, kMaxLds);
+  if (plane_w_lds<T, N, kPW>(P) <= cap) return kPW;
+  if (plane_w_lds<T, N, 4>(P) <= cap) return 4;
+  return 0;
 }
 
 // Launches P1/P2/P3 of the wave-line path; false if the box is not handled.
+template <typename T, int N, int NW>
+static void launch_planes(bool inv, Plan& P, const Geom& g, long total, const cpx<T>* c, long ldc, long sc,
+                          cpx<T>* box, const T* t, cpx<T>* out, long ldo, long so, T scale, bool kin, hipStream_t st) {
+  using C = WaveCfg<N>;
+  const int nblk = (int)(8 * ((total + 7) / 8));
+  const size_t lds = plane_w_lds<T, N, NW>(P);
+  const cpx<T>* tw = tw_of<T>(&P, 1);
+  if (inv)
+    hipLaunchKernelGGL((k_plane_inv_w<T, N, C::M, C::S, NW>), dim3(nblk), dim3(64 * NW), lds, st, g, total, c, ldc,
+                       sc, box, tw);
+  else if (kin)
+    hipLaunchKernelGGL((k_plane_fwd_w<T, N, C::M, C::S, true, NW>), dim3(nblk), dim3(64 * NW), lds, st, g, total,
+                       box, c, ldc, sc, t, out, ldo, so, scale, tw);
+  else
+    hipLaunchKernelGGL((k_plane_fwd_w<T, N, C::M, C::S, false, NW>), dim3(nblk), dim3(64 * NW), lds, st, g, total,
+                       box, c, ldc, sc, t, out, ldo, so, scale, tw);
+}
+
+// Launches P1/P2/P3 of the wave-line path.
 template <typename T, int N>
 static int launch_wave_path_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, long ldc, long sc, cpx<T>* box,
                               const T* vn, const T* t, cpx<T>* out, long ldo, long so, T scale, bool kin,
@@ -679,39 +726,44 @@ static int launch_wave_path_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, 
   using C = WaveCfg<N>;
   using W = WaveLine<T, N, C::M, C::S>;
   const long total = (long)P.na * nbt;
-  const int nblk = (int)(8 * ((total + 7) / 8));
-  const size_t lds = plane_w_lds<T, N>(P);
-  const cpx<T>* tw = tw_of<T>(&P, 1);
+  const int nw = plane_waves<T, N>(P);
+  GW_REQUIRE(nw > 0, GW_ERR_UNSUPPORTED, "x-plane sticks exceed LDS");
   hipEvent_t ev = tm.begin();
-  hipLaunchKernelGGL((k_plane_inv_w<T, N, C::M, C::S>), dim3(nblk), dim3(256), lds, st, g, total, c, ldc, sc, box,
-                     tw);
+  if (nw == kPW) launch_planes<T, N, kPW>(true, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
+  else launch_planes<T, N, 4>(true, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
   tm.end(ST_PLANE_INV, ev);
   ev = tm.begin();
   const int n23 = N * N;
   const int bgroup = 8;
   const int lines_per_wg = 4 * W::LW;
-  hipLaunchKernelGGL((k_xpass_wave<T, N, C::M, C::S>),
-                     dim3((n23 + lines_per_wg - 1) / lines_per_wg, (nbt + bgroup - 1) / bgroup), dim3(256), 0, st,
-                     g, nbt, bgroup, box, vn, tw_of<T>(&P, 0));
+  const int nchunk = (n23 + lines_per_wg - 1) / lines_per_wg;
+  const long xtotal = (long)nchunk * ((nbt + bgroup - 1) / bgroup);
+  hipLaunchKernelGGL((k_xpass_wave<T, N, C::M, C::S>), dim3((unsigned)(8 * ((xtotal + 7) / 8))), dim3(256), 0, st,
+                     g, nbt, bgroup, box, vn, tw_of<T>(&P, 0), nchunk, xtotal);
   tm.end(ST_XPASS, ev);
   ev = tm.begin();
-  if (kin)
-    hipLaunchKernelGGL((k_plane_fwd_w<T, N, C::M, C::S, true>), dim3(nblk), dim3(256), lds, st, g, total, box, c,
-                       ldc, sc, t, out, ldo, so, scale, tw);
-  else
-    hipLaunchKernelGGL((k_plane_fwd_w<T, N, C::M, C::S, false>), dim3(nblk), dim3(256), lds, st, g, total, box, c,
-                       ldc, sc, t, out, ldo, so, scale, tw);
+  if (nw == kPW) launch_planes<T, N, kPW>(false, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
+  else launch_planes<T, N, 4>(false, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
   tm.end(ST_PLANE_FWD, ev);
   GW_CHECK_HIP(hipGetLastError());
   return GW_OK;
 }
 
+template <int N>
+static bool plane_w_fits(const Plan& P, int precision) {
+  return (precision == GW_FP64 ? plane_waves<double, N>(P) : plane_waves<float, N>(P)) > 0;
+}
+
 static bool wave_path_ok(const Plan& P, int precision) {
   if (P.flags & GW_PLAN_LEGACY) return false;
   if (P.n[0] != P.n[1] || P.n[1] != P.n[2] || !wave_length(P.n[0]) || P.xlo < 0) return false;
-  const size_t e = precision == GW_FP64 ? 16 : 8;
-  const size_t need = (size_t)(P.st_max_plane + 1) * P.n[0] * e + 4 * 64 * 17 * e + P.n[0] * 4;
-  return need <= (size_t)std::min(P.lds_max, kMaxLds);
+  switch (P.n[0]) {
+    case 24: return plane_w_fits<24>(P, precision);
+    case 36: return plane_w_fits<36>(P, precision);
+    case 90: return plane_w_fits<90>(P, precision);
+    case 120: return plane_w_fits<120>(P, precision);
+    default: return false;
+  }
 }
 
 template <typename T>
@@ -727,12 +779,17 @@ static int launch_wave_path(Plan& P, const Geom& g, int nbt, const cpx<T>* c, lo
   }
 }
 
+template <typename T, int N, int NW>
+static void set_wave_attr_nw(int want) {
+  using C = WaveCfg<N>;
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_w<T, N, C::M, C::S, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, true, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, false, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+}
 template <typename T, int N>
 static void set_wave_attr_n(int want) {
-  using C = WaveCfg<N>;
-  (void)hipFuncSetAttribute((const void*)k_plane_inv_w<T, N, C::M, C::S>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
-  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, true>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
-  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, false>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  set_wave_attr_nw<T, N, kPW>(want);
+  set_wave_attr_nw<T, N, 4>(want);
 }
 static void set_wave_lds_attrs(int want) {
   set_wave_attr_n<double, 24>(want); set_wave_attr_n<float, 24>(want);
@@ -857,8 +914,7 @@ extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int
                               const int64_t* flat_idx, const int64_t* npw) {
   GW_REQUIRE(out && flat_idx && npw, GW_ERR_ARG, "null argument");
   GW_REQUIRE(n1 > 0 && n2 > 0 && n3 > 0 && nk > 0 && npw_max > 0, GW_ERR_ARG, "bad sizes");
-  gw_plan* h = new gw_plan();
-  std::memset(h, 0, sizeof(*h));
+  gw_plan* h = new gw_plan();  // value-initialised: every pointer null
   Plan& P = h->p;
   P.n[0] = n1; P.n[1] = n2; P.n[2] = n3;
   P.nk = nk; P.npw_max = npw_max;

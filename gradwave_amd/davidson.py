@@ -12,7 +12,8 @@ MI355X-native differences that leave the span and the fp64 contract unchanged:
 * the basis V and its images HV live in preallocated (nk, max_dim, npw) ring
   buffers instead of being rebuilt by torch.cat every round (763);
 * every contraction over npw (V^H HV, Ritz combination, projections, Gram
-  matrices) is an MFMA kernel of libgradwave_hip;
+  matrices) is an MFMA kernel of libgradwave_hip, and V^H HV grows by block
+  columns instead of being rebuilt each round;
 * residuals, norms and the Teter filter are fused kernels with fp64 reductions;
 * orthonormalisation is CholQR2 (the reference's GRADWAVE_CHOLQR path,
   davidson.py:195-222) with Householder QR as the breakdown fallback.
@@ -31,7 +32,10 @@ from gradwave_amd import linalg as LA
 
 _FP32_CERT_FLOOR_EPS = 32 * torch.finfo(torch.float32).eps  # davidson.py:258
 _FP32_CERT_STALL = 0.7  # davidson.py:259
-_SUBSPACE_CPU_MAX = int(os.environ.get("GW_EIGH_CPU_MAX", "0"))  # cf. davidson.py:58
+# Subspace eigensolves at or below this size go to host LAPACK (the reference
+# offloads m <= 256 on CUDA, davidson.py:58-75); measured on MI355X, rocSOLVER
+# wins from m ~ 160 up, host LAPACK below.
+_SUBSPACE_CPU_MAX = int(os.environ.get("GW_EIGH_CPU_MAX", "64"))
 
 
 class PhaseTimer:
@@ -137,27 +141,43 @@ def _orthonormalize_b(v: torch.Tensor, mask: torch.Tensor,
 
 
 class _Basis:
-    """Preallocated (nk, max_dim, npw) basis V and images HV."""
+    """Preallocated (nk, max_dim, npw) basis V, images HV and the fp64 subspace
+    matrix s = V^H H V, grown by block columns.
+
+    The reference rebuilds the whole m x m matrix every round
+    (solvers/davidson.py:648-649); here only the new block column
+    C = V^H (H D) is contracted when directions D are appended, and s is
+    completed Hermitian as [[s, C_top], [C_top^H, (C_bot + C_bot^H)/2]]."""
 
     def __init__(self, nk: int, max_dim: int, m: int, dtype, device) -> None:
         self.v = torch.zeros(nk, max_dim, m, dtype=dtype, device=device)
         self.hv = torch.zeros(nk, max_dim, m, dtype=dtype, device=device)
+        self.s = torch.zeros(nk, max_dim, max_dim, dtype=torch.complex128, device=device)
         self.n = 0
 
     def views(self):
         return self.v[:, : self.n], self.hv[:, : self.n]
+
+    def subspace(self) -> torch.Tensor:
+        return self.s[:, : self.n, : self.n]
 
     def set(self, v: torch.Tensor, hv: torch.Tensor) -> None:
         k = v.shape[1]
         self.v[:, :k].copy_(v)
         self.hv[:, :k].copy_(hv)
         self.n = k
+        self.s[:, :k, :k] = _hermitize(LA.inner(self.v[:, :k], self.hv[:, :k]))
 
     def append(self, v: torch.Tensor, hv: torch.Tensor) -> None:
         k = v.shape[1]
-        self.v[:, self.n: self.n + k].copy_(v)
-        self.hv[:, self.n: self.n + k].copy_(hv)
-        self.n += k
+        n0, n1 = self.n, self.n + k
+        self.v[:, n0:n1].copy_(v)
+        self.hv[:, n0:n1].copy_(hv)
+        c = LA.inner(self.v[:, :n1], self.hv[:, n0:n1]).to(torch.complex128)  # (nk, n1, k)
+        self.s[:, :n0, n0:n1] = c[:, :n0]
+        self.s[:, n0:n1, :n0] = c[:, :n0].mH
+        self.s[:, n0:n1, n0:n1] = _hermitize(c[:, n0:])
+        self.n = n1
 
 
 def _certify_fp64(h_apply, x):
@@ -215,10 +235,8 @@ def davidson_batched(h_apply: Callable[[torch.Tensor], torch.Tensor], x0: torch.
 
     for it in range(1, max_iter + 1):
         v, hv = basis.views()
-        tm("subspace_gemm")
-        s = _hermitize(LA.inner(v, hv))
         tm("eigh")
-        w, u = _eigh(s)
+        w, u = _eigh(basis.subspace())
         u = u[:, :, :nb].to(v.dtype).contiguous()
         eig = w[:, :nb].real.to(rdtype)
         tm("ritz_gemm")
@@ -264,7 +282,7 @@ def davidson_batched(h_apply: Callable[[torch.Tensor], torch.Tensor], x0: torch.
         d = _orthonormalize_b(d, mask, against=v)
         tm("h_apply")
         hd = _apply_exp(d)
-        tm("append")
+        tm("subspace_gemm")
         basis.append(d, hd)
 
     if use_fp32_exp:

@@ -23,9 +23,21 @@ def dev() -> torch.device:
     return torch.device("cuda:0")
 
 
+def _si8_box120():
+    """Si8 cell, 30 Ry, two k-points, in a 120^3 box (exercises the 120-point lines)."""
+    import numpy as np
+
+    cell, pos = S.si_conventional()
+    pos = S.jitter(pos, 0.05, 3)
+    k = np.array([[0.125, -0.25, 0.375], [0.0, 0.0, 0.0]])
+    return S.build_system("si8_box120", cell, pos, [S.SI] * 8, 30.0, k, np.full(2, 0.5), 8,
+                          shape=(120, 120, 120))
+
+
 def system(name: str):
     if name not in _CACHE:
-        _CACHE[name] = {"si2": S.si2_c1, "si8": S.si8_c4, "si64": S.si64_c2}[name]()
+        _CACHE[name] = {"si2": S.si2_c1, "si8": S.si8_c4, "si64": S.si64_c2,
+                        "si8_120": _si8_box120}[name]()
     return _CACHE[name]
 
 
