@@ -50,6 +50,14 @@ SIGNATURES = {
     "gw_residual": (_I, [_I, _I, _I, _I, _P, _P, _L, _L, _P, _P, _P, _P]),
     "gw_teter_expand": (_I, [_I, _I, _I, _I, _I, _P, _P, _L, _L, _P, _L, _P, _P, _L, _L, _P]),
     "gw_row_norms": (_I, [_I, _I, _I, _I, _P, _L, _L, _P, _P]),
+    "gw_hpsi_adjoint_v_workspace_bytes": (_Z, [_P, _I, _I]),
+    "gw_hpsi_adjoint_v": (_I, [_P, _I, _P, _L, _L, _P, _L, _L, _I, _P, _P, _Z, _P]),
+    "gw_proj_phase": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "gw_proj_phase_grad_workspace_bytes": (_Z, [_I, _I]),
+    "gw_proj_phase_grad": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _Z, _P]),
+    "gw_vloc_g": (_I, [_I, _L, _P, _P, _P, _P, _D, _P, _P]),
+    "gw_vloc_g_grad_workspace_bytes": (_Z, [_I]),
+    "gw_vloc_g_grad": (_I, [_I, _L, _P, _P, _P, _P, _D, _P, _P, _P, _Z, _P]),
 }
 
 _lib = None

@@ -521,6 +521,7 @@ __global__ void k_dmul(int nproj, int nb, const int* __restrict__ colptr, const 
 // waves per workgroup of the wave-line plane kernels: kPW, or 4 when the
 // compact stick rows plus kPW exchange areas exceed LDS (fp64, 120^3)
 constexpr int kPW = 8;
+constexpr int kAdjBandGroup = 16;  // bands reduced per x-pass workgroup in the V adjoint
 
 template <typename T, int N, int M, int S, int NW>
 __global__ void __launch_bounds__(64 * NW)
@@ -745,6 +746,107 @@ static int launch_wave_path_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, 
   if (nw == kPW) launch_planes<T, N, kPW>(false, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
   else launch_planes<T, N, 4>(false, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
   tm.end(ST_PLANE_FWD, ev);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
+
+// ---- adjoint of the local term with respect to V(r) ----
+// grad_V(r) += (1/N) sum_{k,b} Re[conj(phi(r)) psi(r)], psi = N ifftn(scatter c),
+// phi = N ifftn(scatter g): both boxes come from P1, this x-pass finishes the
+// two inverse transforms and reduces over a group of bands into partial sums
+// [group][x][y*n3 + z] (natural order), summed by k_adjv_reduce.
+template <typename T, int N, int M, int S>
+__global__ void __launch_bounds__(256)
+k_xpass_adjv_wave(Geom g, int nbt, int bgroup, const cpx<T>* __restrict__ boxc, const cpx<T>* __restrict__ boxg,
+                  const cpx<T>* __restrict__ twx, int nchunk, long total, double* __restrict__ part) {
+  using W = WaveLine<T, N, M, S>;
+  constexpr int LW = W::LW;
+  __shared__ cpx<T> E[4 * W::E_PER_WAVE];
+  __shared__ cpx<T> tw[N];
+  const long wid = xcd_work_id(total);
+  if (wid < 0) return;
+  const int chunk = (int)(wid % nchunk), bgi = (int)(wid / nchunk);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twx[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = lane / LW, l = lane - t * LW;
+  const bool part_ = t < S;
+  const int n23 = g.n2 * g.n3;
+  const int j = (chunk * 4 + wave) * LW + l;
+  const bool jok = j < n23;
+  cpx<T>* Ew = E + wave * W::E_PER_WAVE;
+  auto slot = [&](int i0) { return i0 < g.xlo ? i0 : (i0 >= g.xhi ? g.xlo + i0 - g.xhi : -1); };
+  double acc[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) acc[s] = 0.0;
+  const int b0 = bgi * bgroup;
+  const int nbg = min(bgroup, nbt - b0);
+  for (int bg = 0; bg < nbg; ++bg) {
+    const long off = (long)(b0 + bg) * g.na * n23 + j;
+    cpx<T> x[M], yc[S], yg[S];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int sl = part_ ? slot(S * q + t) : -1;
+      x[q] = (jok && sl >= 0) ? boxc[off + (long)sl * n23] : czero<T>();
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) yc[s] = czero<T>();
+    W::inverse(x, yc, t, l, part_, Ew, tw);
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int sl = part_ ? slot(S * q + t) : -1;
+      x[q] = (jok && sl >= 0) ? boxg[off + (long)sl * n23] : czero<T>();
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) yg[s] = czero<T>();
+    W::inverse(x, yg, t, l, part_, Ew, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] += (double)yg[s].x * yc[s].x + (double)yg[s].y * yc[s].y;
+  }
+  if (t < M && jok) {
+    double* pp = part + (long)bgi * N * n23;
+#pragma unroll
+    for (int s = 0; s < S; ++s) pp[(long)(t + M * s) * n23 + j] = acc[s];
+  }
+}
+
+__global__ void k_adjv_reduce(long n, int ngroups, double scale, const double* __restrict__ part,
+                              double* __restrict__ gv) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  double s = 0;
+  for (int i = 0; i < ngroups; ++i) s += part[(long)i * n + e];
+  gv[e] += scale * s;
+}
+
+template <typename T, int N>
+static int adjv_wave_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, long ldc, long sc, const cpx<T>* gr,
+                       long ldg, long sg, cpx<T>* boxc, cpx<T>* boxg, double* part, double* gv, hipStream_t st) {
+  using C = WaveCfg<N>;
+  using W = WaveLine<T, N, C::M, C::S>;
+  const long total = (long)P.na * nbt;
+  const int nw = plane_waves<T, N>(P);
+  GW_REQUIRE(nw > 0, GW_ERR_UNSUPPORTED, "x-plane sticks exceed LDS");
+  cpx<T>* nullout = nullptr;
+  if (nw == kPW) {
+    launch_planes<T, N, kPW>(true, P, g, total, c, ldc, sc, boxc, nullptr, nullout, 0, 0, T(1), false, st);
+    launch_planes<T, N, kPW>(true, P, g, total, gr, ldg, sg, boxg, nullptr, nullout, 0, 0, T(1), false, st);
+  } else {
+    launch_planes<T, N, 4>(true, P, g, total, c, ldc, sc, boxc, nullptr, nullout, 0, 0, T(1), false, st);
+    launch_planes<T, N, 4>(true, P, g, total, gr, ldg, sg, boxg, nullptr, nullout, 0, 0, T(1), false, st);
+  }
+  const int n23 = N * N;
+  const int lines_per_wg = 4 * W::LW;
+  const int nchunk = (n23 + lines_per_wg - 1) / lines_per_wg;
+  const int bgroup = kAdjBandGroup;
+  const int ngroups = (nbt + bgroup - 1) / bgroup;
+  const long xtotal = (long)nchunk * ngroups;
+  hipLaunchKernelGGL((k_xpass_adjv_wave<T, N, C::M, C::S>), dim3((unsigned)(8 * ((xtotal + 7) / 8))), dim3(256), 0,
+                     st, g, nbt, bgroup, boxc, boxg, tw_of<T>(&P, 0), nchunk, xtotal, part);
+  const long n = (long)N * n23;
+  hipLaunchKernelGGL(k_adjv_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, ngroups,
+                     1.0 / (double)n, part, gv);
   GW_CHECK_HIP(hipGetLastError());
   return GW_OK;
 }
@@ -1395,4 +1497,51 @@ extern "C" int gw_box_to_sphere(gw_plan* h, int precision, const void* box, void
   GW_REQUIRE(h && c_out && box && nb > 0, GW_ERR_ARG, "bad argument");
   return precision == GW_FP64 ? pack_t<double>(h, c_out, const_cast<void*>(box), nb, 1, (hipStream_t)stream)
                               : pack_t<float>(h, c_out, const_cast<void*>(box), nb, 1, (hipStream_t)stream);
+}
+
+extern "C" size_t gw_hpsi_adjoint_v_workspace_bytes(const gw_plan* h, int nb, int precision) {
+  if (!h || nb <= 0) return 0;
+  const Plan& P = h->p;
+  const size_t e = precision == GW_FP64 ? 16 : 8;
+  const size_t nbt = (size_t)P.nk * nb;
+  const size_t box = nbt * P.na * P.n[1] * P.n[2] * e;
+  const size_t ngroups = (nbt + kAdjBandGroup - 1) / kAdjBandGroup;
+  const size_t part = ngroups * (size_t)P.n[0] * P.n[1] * P.n[2] * sizeof(double);
+  return 2 * align256(box) + align256(part);
+}
+
+template <typename T>
+static int adjv_t(gw_plan* h, const cpx<T>* c, long ldc, long sc, const cpx<T>* gr, long ldg, long sg, int nb,
+                  double* gv, unsigned char* ws, hipStream_t st) {
+  Plan& P = h->p;
+  const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
+  GW_REQUIRE(wave_path_ok(P, prec), GW_ERR_UNSUPPORTED,
+             "V adjoint needs a cubic 24/36/90/120 box with the wave-line kernels");
+  const Geom g = geom_of(&P, nb);
+  const int nbt = P.nk * nb;
+  const size_t e = sizeof(cpx<T>);
+  const size_t box = align256((size_t)nbt * P.na * P.n[1] * P.n[2] * e);
+  cpx<T>* boxc = (cpx<T>*)ws;
+  cpx<T>* boxg = (cpx<T>*)(ws + box);
+  double* part = (double*)(ws + 2 * box);
+  switch (P.n[0]) {
+    case 24: return adjv_wave_n<T, 24>(P, g, nbt, c, ldc, sc, gr, ldg, sg, boxc, boxg, part, gv, st);
+    case 36: return adjv_wave_n<T, 36>(P, g, nbt, c, ldc, sc, gr, ldg, sg, boxc, boxg, part, gv, st);
+    case 90: return adjv_wave_n<T, 90>(P, g, nbt, c, ldc, sc, gr, ldg, sg, boxc, boxg, part, gv, st);
+    case 120: return adjv_wave_n<T, 120>(P, g, nbt, c, ldc, sc, gr, ldg, sg, boxc, boxg, part, gv, st);
+    default: return GW_ERR_UNSUPPORTED;
+  }
+}
+
+extern "C" int gw_hpsi_adjoint_v(gw_plan* h, int precision, const void* c, long ldc, long sc, const void* g,
+                                 long ldg, long sg, int nb, double* grad_v, void* ws, size_t ws_bytes,
+                                 void* stream) {
+  GW_REQUIRE(h && c && g && grad_v && nb > 0, GW_ERR_ARG, "bad argument");
+  const size_t need = gw_hpsi_adjoint_v_workspace_bytes(h, nb, precision);
+  GW_REQUIRE(ws && ws_bytes >= need, GW_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, need);
+  if (precision == GW_FP64)
+    return adjv_t<double>(h, (const double2*)c, ldc, sc, (const double2*)g, ldg, sg, nb, grad_v,
+                          (unsigned char*)ws, (hipStream_t)stream);
+  return adjv_t<float>(h, (const float2*)c, ldc, sc, (const float2*)g, ldg, sg, nb, grad_v, (unsigned char*)ws,
+                       (hipStream_t)stream);
 }

@@ -134,6 +134,41 @@ int gw_teter_expand(int precision, int nk, int nb, int nsel, int npw, const void
 int gw_row_norms(int precision, int nk, int m, int npw, const void* v, long ld, long s,
                  double* out, void* stream);
 
+/* ---- adjoints (vector-Jacobian products) of H|psi> ----
+ * The reference differentiates HamiltonianK.apply (core/hamiltonian.py:179-191),
+ * projectors (108-112) and local_potential_g (core/energies/local_pp.py:22-38)
+ * with torch autograd (postscf/forces.py:168-215). Gradients follow torch's
+ * complex convention: for a real loss L, grad z = dL/dRe z + i dL/dIm z.
+ *
+ * V(r): grad_v (n1, n2, n3) float64, natural order, is ACCUMULATED:
+ *   grad_v(r) += (1/N) sum_{k,b} Re[conj(phi_kb(r)) psi_kb(r)],
+ *   psi = N ifftn(scatter c), phi = N ifftn(scatter g); c, g as in gw_hpsi_apply_strided.
+ *   Needs a cubic 24/36/90/120 box (gw_plan_info fused_planes == 2). */
+size_t gw_hpsi_adjoint_v_workspace_bytes(const gw_plan* plan, int nb, int precision);
+int gw_hpsi_adjoint_v(gw_plan* plan, int precision, const void* c, long ldc, long sc, const void* g,
+                      long ldg, long sg, int nb, double* grad_v, void* workspace,
+                      size_t workspace_bytes, void* stream);
+/* projector phases (core/batch.py:229-235): p[k][q][g] = pf[k][q][g] exp(-i kpg[k][g] . tau[a(q)]),
+ * 0 for g >= npw[k]. pf, p (nk, nproj, npw_max) complex128; kpg (nk, npw_max, 3) float64 A^-1;
+ * positions (na, 3) A; atom_index (nproj,) int64; npw (nk,) int32 on the device. */
+int gw_proj_phase(int nk, int nproj, int npw_max, const int* npw, const void* pf, const double* kpg,
+                  const double* positions, const int64_t* atom_index, void* p, void* stream);
+/* its adjoint: grad_tau[a][x] += sum_{k, q: a(q)=a, g} Im[conj(grad_p) p] kpg_x (grad_tau (na, 3)) */
+size_t gw_proj_phase_grad_workspace_bytes(int nk, int nproj);
+int gw_proj_phase_grad(int nk, int nproj, int npw_max, int na, const void* grad_p, const void* p,
+                       const double* kpg, const int64_t* atom_index, double* grad_tau, void* workspace,
+                       size_t workspace_bytes, void* stream);
+/* local potential on the full box (local_pp.py:22-38): vloc_g (npts) complex128 =
+ * (1/volume) sum_a tables[species_index[a]][G] exp(-i G . tau_a);
+ * tables (nspecies, npts) float64 eV A^3; g_cart (npts, 3) A^-1; npts = n1 n2 n3. */
+int gw_vloc_g(int na, long npts, const double* positions, const int64_t* species_index,
+              const double* tables, const double* g_cart, double volume, void* vloc_g, void* stream);
+/* its adjoint: grad_tau[a][x] += (1/volume) sum_G tab(G) G_x Im[S_a(G) conj(grad_vloc_g(G))] */
+size_t gw_vloc_g_grad_workspace_bytes(int na);
+int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* species_index,
+                   const double* tables, const double* g_cart, double volume, const void* grad_vloc_g,
+                   double* grad_tau, void* workspace, size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
