@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--no-davidson", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-bands", type=int, default=8)
-    ap.add_argument("--davidson-max-iter", type=int, default=100)
+    ap.add_argument("--davidson-max-iter", type=int, default=300)
+    ap.add_argument("--no-forces", action="store_true")
     ap.add_argument("--profile-davidson", action="store_true")
     ap.add_argument("--tol-ha", type=float, default=1e-10)
     ap.add_argument("--draft-tol", type=float, default=1e-3, help="mixed draft tolerance [eV]")
@@ -195,11 +196,40 @@ def main() -> None:
             "band_energy_fp64_Ha": float((r64.eigenvalues[:, : int(wl.n_electrons // 2)].sum(-1)
                                           * torch.as_tensor(wl.kweights, device=dev)).sum() * 2
                                          / HARTREE_EV)}
+        if not args.no_forces:
+            result["forces"] = forces_check(wl, r64.eigenvectors, rms.eigenvectors)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, wl)
 
     print(json.dumps(result), flush=True)
+
+
+def forces_check(wl, psi64, psims) -> dict:
+    """Band-energy forces at fixed orbitals (adjoint kernels) from the FP64 and
+    the mixed solve's orbitals: time and max |dF| in Ha/bohr."""
+    import torch
+
+    from gradwave_amd.adjoint import band_energy_forces
+    from gradwave_amd.workloads import BOHR_ANG, HARTREE_EV
+
+    nocc = int(wl.n_electrons // 2)
+    w = torch.zeros(wl.bk.nk, psi64.shape[1], dtype=torch.float64, device=psi64.device)
+    w[:, :nocc] = 2.0 * torch.as_tensor(wl.kweights, device=psi64.device)[:, None]
+    args = (wl.bk, wl.shape, wl.positions, wl.species_index, wl.vloc_tables, wl.g_cart, wl.volume)
+    band_energy_forces(*args, psi64, w)  # warm-up (plans, workspaces)
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    e64, f64 = band_energy_forces(*args, psi64, w)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - a
+    _, fms = band_energy_forces(*args, psims, w)
+    to_ha_bohr = BOHR_ANG / HARTREE_EV
+    df = float((fms - f64).abs().max()) * to_ha_bohr
+    print(f"[bench] forces {3 * f64.shape[0]} coords {t * 1e3:.1f} ms, max|dF| mixed vs fp64 {df:.2e} Ha/bohr",
+          file=sys.stderr, flush=True)
+    return {"coords": 3 * int(f64.shape[0]), "s": t, "max_abs_Ha_per_bohr": float(f64.abs().max()) * to_ha_bohr,
+            "max_diff_mixed_vs_fp64_Ha_per_bohr": df, "band_energy_fixed_psi_eV": float(e64)}
 
 
 STAGE_KERNEL = {"plane_inv": "k_plane_inv_w", "xpass": "k_xpass_wave", "plane_fwd": "k_plane_fwd_w",

@@ -294,3 +294,35 @@ def local_potential_r(vloc_g: torch.Tensor) -> torch.Tensor:
     """V(r) = Re N ifftn V_loc(G) (scf/loop.py:532-544, one full-box FFT per geometry)."""
     n = vloc_g.shape[-3] * vloc_g.shape[-2] * vloc_g.shape[-1]
     return (torch.fft.ifftn(vloc_g, dim=(-3, -2, -1)) * n).real
+
+
+# ---------------------------------------------------------------- forces
+
+
+def band_energy_forces(bk, shape, positions: torch.Tensor, species_index: torch.Tensor,
+                       vloc_tables: torch.Tensor, g_cart: torch.Tensor, volume: float,
+                       psi: torch.Tensor, weights: torch.Tensor, v_frozen: torch.Tensor | None = None):
+    """Band energy at fixed orbitals and its position gradient.
+
+    E = sum_{k,b} weights[k,b] Re <psi_kb| H(tau) |psi_kb>   [eV]
+    F = -dE/dtau                                           [eV/A], (na, 3)
+
+    H(tau) carries V_loc(r; tau) (+ ``v_frozen``, a position-independent
+    potential such as Hartree + xc) and the projectors p(tau): the
+    Hellmann-Feynman pattern of postscf/forces.py:168-215 (orbitals detached,
+    gradient through the projector phases and the structure factor).
+    ``bk`` needs npw, mask, flat_idx, t, kpg, dij_full, proj_atom_index and
+    proj_phase_free on the GPU.
+    """
+    with torch.enable_grad():
+        pos = positions.detach().to(torch.float64).clone().requires_grad_(True)
+        v = local_potential_r(local_potential_g(pos, species_index, vloc_tables, g_cart, volume))
+        if v_frozen is not None:
+            v = v + v_frozen.detach()
+        p = projectors_b(bk, pos)
+        h = DifferentiableHamiltonian(bk, shape, v, p, bk.dij_full)
+        x = psi.detach()
+        hx = h.apply(x)
+        e = (weights.to(torch.float64)[..., None] * (x.conj() * hx).real).sum()
+        (g,) = torch.autograd.grad(e, pos)
+    return e.detach(), -g

@@ -57,6 +57,7 @@ class BatchedK:
     t: torch.Tensor
     dij_full: torch.Tensor
     proj_atom_index: torch.Tensor
+    proj_phase_free: torch.Tensor | None = None  # (nk, nproj, npw_max), e^{-i(k+G).tau} left out
 
     @property
     def nk(self) -> int:
@@ -81,6 +82,11 @@ class Workload:
     v_r: torch.Tensor  # (n1, n2, n3) float64
     nbands: int
     n_electrons: float
+    # geometry inputs of the position-differentiable path (local_pp.py:22-38)
+    volume: float = 0.0
+    species_index: torch.Tensor | None = None  # (na,) row of each atom in vloc_tables
+    vloc_tables: torch.Tensor | None = None  # (nspecies, n1, n2, n3) eV A^3, G=0 entry alpha
+    g_cart: torch.Tensor | None = None  # (n1, n2, n3, 3) A^-1
 
 
 def reciprocal(cell: np.ndarray) -> np.ndarray:
@@ -198,7 +204,7 @@ def build(name: str, cell, positions, species: list[Species], ecut_ry: float, kf
     c00 = 0.5 * math.sqrt(1.0 / math.pi)
     c1 = math.sqrt(3.0 / (4.0 * math.pi))
     pref = 4.0 * math.pi / math.sqrt(vol)
-    cols, atom_idx, dblocks = [], [], []
+    cols, free, atom_idx, dblocks = [], [], [], []
     for a, sp in enumerate(species):
         f0 = kb_form_factor(sp, 0, q)
         f1 = kb_form_factor(sp, 1, q)
@@ -209,10 +215,14 @@ def build(name: str, cell, positions, species: list[Species], ecut_ry: float, kf
                -1j * (c1 * unit[..., 1] * f1)]
         for col in ang:
             cols.append(pref * col * phase)
+            free.append(pref * col)
             atom_idx.append(a)
         dblocks.append(torch.diag(torch.tensor([sp.d_s_ha, sp.d_p_ha, sp.d_p_ha, sp.d_p_ha],
                                                dtype=f64) * HARTREE_EV))
-    p = torch.stack(cols, dim=1) * torch.as_tensor(mask, device=dev)[:, None, :]
+    mk = torch.as_tensor(mask, device=dev)[:, None, :]
+    p = torch.stack(cols, dim=1) * mk
+    pf = torch.stack(free, dim=1) * mk
+    del cols, free
     dij = torch.block_diag(*dblocks)
     # local potential on the full box, separable structure factor per atom
     n1, n2, n3 = shape
@@ -226,11 +236,13 @@ def build(name: str, cell, positions, species: list[Species], ecut_ry: float, kf
     by_species: dict[Species, list[int]] = {}
     for a, sp in enumerate(species):
         by_species.setdefault(sp, []).append(a)
+    tables = []
     for sp, atoms in by_species.items():
         rc = sp.rc_bohr * BOHR_ANG
         gz = torch.where(gabs > 1e-10, gabs, torch.ones_like(gabs))
         v = -4.0 * math.pi * sp.z * E2 * torch.exp(-0.25 * (gz * rc) ** 2) / gz**2
         v = torch.where(gabs > 1e-10, v, torch.full_like(v, math.pi * sp.z * E2 * rc**2))
+        tables.append(v)
         s = torch.zeros(shape, dtype=torch.complex128, device=dev)
         for a in atoms:
             e = [torch.exp(-2j * math.pi * freqs[i].to(torch.complex128) * frac[a, i]) for i in range(3)]
@@ -242,9 +254,14 @@ def build(name: str, cell, positions, species: list[Species], ecut_ry: float, kf
                   mask=torch.as_tensor(mask, device=dev),
                   flat_idx=torch.as_tensor(flat, device=dev), kpg=kpg,
                   t=torch.as_tensor(t, dtype=f64, device=dev), dij_full=dij.to(dev),
-                  proj_atom_index=torch.as_tensor(atom_idx, dtype=torch.int64, device=dev))
+                  proj_atom_index=torch.as_tensor(atom_idx, dtype=torch.int64, device=dev),
+                  proj_phase_free=pf.contiguous())
+    order = list(by_species)
+    sidx = torch.tensor([order.index(sp) for sp in species], dtype=torch.int64, device=dev)
     return Workload(name, cell, pos, tuple(shape), ecut, kfrac, np.asarray(kweights, dtype=np.float64),
-                    bk, p.contiguous(), v_r, nbands, float(sum(sp.z for sp in species)))
+                    bk, p.contiguous(), v_r, nbands, float(sum(sp.z for sp in species)),
+                    volume=vol, species_index=sidx, vloc_tables=torch.stack(tables).contiguous(),
+                    g_cart=g.contiguous())
 
 
 def jitter(pos: np.ndarray, delta: float, seed: int) -> np.ndarray:

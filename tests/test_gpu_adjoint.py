@@ -239,3 +239,82 @@ def test_adjoint_identities_at_si64_size():
     lin_d = float((wg.conj() * hd.apply_terms(cg, L.GW_TERM_NONLOCAL)).real.sum())
     assert abs(float((gv * dv).sum()) - lin_v) < 1e-10 * abs(lin_v)
     assert abs(float((gd.cpu() * dd).sum()) - lin_d) < 1e-10 * abs(lin_d)
+
+
+def test_band_energy_forces_si8_all_k_against_oracle():
+    """C4 shape: Si8, 36^3, all 36 k-points, 24 bands, k-weighted occupations."""
+    from gradwave_amd.adjoint import band_energy_forces
+
+    say("band-energy forces si8 36 k x 24 bands vs oracle autograd")
+    s, cpu, gpu, p, c, _ = _setup("si8", None, 24, seed=17)
+    tab, sidx = _tables(s)
+    d = dev()
+    nocc = int(s.n_electrons // 2)
+    w = torch.zeros(c.shape[0], c.shape[1], dtype=torch.float64)
+    w[:, :nocc] = 2.0 * torch.as_tensor(s.kweights)[:, None]
+    e, f = band_energy_forces(gpu, s.shape, s.positions.to(d), sidx.to(d), tab.to(d),
+                              s.grid.g_cart.to(d), s.grid.volume, c.to(d), w.to(d))
+    from oracle.hamiltonian import HamiltonianK, build_projector_data, projectors
+    from oracle.pseudo import local_potential_g, local_potential_r
+
+    pos = s.positions.clone().requires_grad_(True)
+    v = local_potential_r(local_potential_g(pos, s.species, s.grid.g_cart, s.grid.volume))
+    e_ref = 0.0
+    for ik in range(c.shape[0]):
+        n = int(cpu.npw[ik])
+        pd = build_projector_data(s.spheres[ik], s.species, s.grid.volume)
+        h = HamiltonianK(s.spheres[ik], s.shape, v, pd, projectors(pd, pos))
+        ck = c[ik, :, :n]
+        e_ref = e_ref + (w[ik][:, None] * (ck.conj() * h.apply(ck)).real).sum()
+    (g_ref,) = torch.autograd.grad(e_ref, pos)
+    assert abs(float(e) - float(e_ref)) < 1e-11 * abs(float(e_ref))
+    assert float((f.cpu() + g_ref).norm() / g_ref.norm()) < 1e-6
+
+
+def test_workload_geometry_tables_reproduce_p_and_v():
+    """The bench's forces use the workload's phase-free projectors and V_loc tables."""
+    from gradwave_amd import adjoint as A
+    from gradwave_amd import workloads as W
+
+    say("workload geometry tables si8_mp")
+    wl = W.si8_mp(device=dev())
+    p = A.projectors_b(wl.bk, wl.positions)
+    v = A.local_potential_r(A.local_potential_g(wl.positions, wl.species_index, wl.vloc_tables, wl.g_cart,
+                                                wl.volume))
+    assert _rel(p, wl.p) < 1e-13
+    assert _rel(v, wl.v_r) < 1e-12
+
+
+def test_single_k_hamiltonian_api_matches_oracle():
+    """adjoint.HamiltonianK(sphere, shape, v, pd, p).apply(c (nb, npw)) with gradients,
+    against the oracle's HamiltonianK under autograd (core/hamiltonian.py:120-191)."""
+    from gradwave_amd import adjoint as A
+    from oracle.hamiltonian import HamiltonianK as RefK
+    from oracle.hamiltonian import build_projector_data, projectors
+
+    say("single-k HamiltonianK api si8 k=3")
+    s = system("si8")
+    ik = 3
+    sph = s.spheres[ik]
+    pd = build_projector_data(sph, s.species, s.grid.volume)
+    p = projectors(pd, s.positions)
+    g = torch.Generator().manual_seed(12)
+    c = torch.randn(5, sph.npw, dtype=torch.complex128, generator=g) / sph.npw**0.5
+    w = torch.randn(5, sph.npw, dtype=torch.complex128, generator=g) / sph.npw**0.5
+    d = dev()
+    sph_d = SimpleNamespace(flat_idx=sph.flat_idx.to(d), kpg2=sph.kpg2.to(d))
+    pd_d = SimpleNamespace(dij_full=pd.dij_full.to(d).requires_grad_(True))
+    v = s.v_r.to(d).requires_grad_(True)
+    pg = p.to(d).requires_grad_(True)
+    cg = c.to(d).requires_grad_(True)
+    out = A.HamiltonianK(sph_d, s.shape, v, pd_d, pg).apply(cg)
+    grads = torch.autograd.grad((w.to(d).conj() * out).real.sum(), [cg, v, pg, pd_d.dij_full])
+    vr = s.v_r.clone().requires_grad_(True)
+    pr = p.clone().requires_grad_(True)
+    cr = c.clone().requires_grad_(True)
+    pd.dij_full = pd.dij_full.clone().requires_grad_(True)
+    ref_out = RefK(sph, s.shape, vr, pd, pr).apply(cr)
+    ref = torch.autograd.grad((w.conj() * ref_out).real.sum(), [cr, vr, pr, pd.dij_full])
+    assert _rel(out, ref_out) < 1e-12
+    for a, b in zip(grads, ref):
+        assert _rel(a, b) < 1e-10
