@@ -22,6 +22,8 @@ import statistics
 import sys
 import time
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
@@ -41,6 +43,7 @@ def parse():
     ap.add_argument("--cpu-bands", type=int, default=8)
     ap.add_argument("--davidson-max-iter", type=int, default=300)
     ap.add_argument("--no-forces", action="store_true")
+    ap.add_argument("--no-gamma", action="store_true")
     ap.add_argument("--profile-davidson", action="store_true")
     ap.add_argument("--tol-ha", type=float, default=1e-10)
     ap.add_argument("--draft-tol", type=float, default=1e-3, help="mixed draft tolerance [eV]")
@@ -198,11 +201,74 @@ def main() -> None:
                                          / HARTREE_EV)}
         if not args.no_forces:
             result["forces"] = forces_check(wl, r64.eigenvectors, rms.eigenvectors)
+        if not args.no_gamma and nk == 1 and not np.any(wl.kfrac):
+            result["gamma"] = gamma_form(args, wl, x64, r64)
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, wl)
 
     print(json.dumps(result), flush=True)
+
+
+def gamma_form(args, wl, x64, r64) -> dict:
+    """Gamma real layout (core/gamma.py): half-sphere apply at two bands per
+    complex apply, real-symmetric Davidson (FP64 and mixed) from the same start
+    vectors; eigenvalues against the complex FP64 solve."""
+    import torch
+
+    from gradwave_amd import gamma as G
+    from gradwave_amd.workloads import HARTREE_EV
+
+    sph = type("Sphere", (), {})()
+    sph.flat_idx = wl.bk.flat_idx[0]
+    sph.kpg2 = (wl.bk.kpg[0] ** 2).sum(-1)
+    gb = G.build_gamma_basis(sph, wl.shape)
+    gh = G.GammaHamiltonian(gb, wl.v_r, wl.p[0], wl.bk.dij_full)
+    xh = G.full_to_half(gb, x64[0])
+    xh[:, 0] = xh[:, 0].real.to(xh.dtype)
+    nb = xh.shape[0]
+    out = {}
+    for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+        phi = G.embed_real(gb, xh).to(dt)
+        for _ in range(3):
+            gh.apply_real(phi)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(5, args.steps // 2)):
+            a = time.perf_counter()
+            gh.apply_real(phi)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - a)
+        ms = statistics.median(ts)
+        out[f"{name}_band_applies_per_s"] = nb / ms
+        out[f"{name}_ms_per_step"] = ms * 1e3
+    tol = args.tol_ha * HARTREE_EV
+    prof64, profms = {}, {}
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    g64 = G.davidson_gamma(gh, xh, tol=tol, max_iter=args.davidson_max_iter,
+                           profile_out=prof64 if args.profile_davidson else None)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - a
+    a = time.perf_counter()
+    gms = G.davidson_gamma_mixed(gh, xh, tol=tol, draft_tol=args.draft_tol, max_iter=args.davidson_max_iter,
+                                 profile_out=profms if args.profile_davidson else None)
+    torch.cuda.synchronize()
+    tms = time.perf_counter() - a
+    ref = r64.eigenvalues[0]
+    out.update({
+        "fp64_s_per_kpt": t64, "mixed_s_per_kpt": tms, "fp64_iters": g64.n_iter,
+        "mixed_draft_iters": gms.n_draft_iter, "mixed_polish_iters": gms.n_iter,
+        "fp64_max_residual_eV": float(g64.residual_norms.max()),
+        "mixed_max_residual_eV": float(gms.residual_norms.max()),
+        "max_eig_diff_fp64_vs_complex_Ha": float((g64.eigenvalues - ref).abs().max()) / HARTREE_EV,
+        "max_eig_diff_mixed_vs_complex_Ha": float((gms.eigenvalues - ref).abs().max()) / HARTREE_EV,
+        "phase_s_fp64": prof64 or None, "phase_s_mixed": profms or None})
+    print(f"[bench] gamma apply fp32 {out['fp32_ms_per_step']:.3f} ms fp64 {out['fp64_ms_per_step']:.3f} ms; "
+          f"davidson fp64 {t64:.2f}s ({g64.n_iter} it) mixed {tms:.2f}s; max|de| "
+          f"{out['max_eig_diff_fp64_vs_complex_Ha']:.1e} / {out['max_eig_diff_mixed_vs_complex_Ha']:.1e} Ha",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def forces_check(wl, psi64, psims) -> dict:

@@ -58,6 +58,11 @@ SIGNATURES = {
     "gw_vloc_g": (_I, [_I, _L, _P, _P, _P, _P, _D, _P, _P]),
     "gw_vloc_g_grad_workspace_bytes": (_Z, [_I]),
     "gw_vloc_g_grad": (_I, [_I, _L, _P, _P, _P, _P, _D, _P, _P, _P, _Z, _P]),
+    "gw_rgemm_inner_workspace_bytes": (_Z, [_I, _I, _I, _I, _I]),
+    "gw_rgemm_inner": (_I, [_I, _I, _I, _I, _I, _P, _L, _L, _P, _L, _L, _P, _L, _L, _P, _Z, _P]),
+    "gw_rgemm_combine": (_I, [_I, _I, _I, _I, _I, _P, _L, _L, _P, _L, _L, _P, _L, _L, _D, _D, _P]),
+    "gw_gamma_pack": (_I, [_I, _I, _I, _I, _P, _L, _P, _P, _P, _L, _P]),
+    "gw_gamma_unpack": (_I, [_I, _I, _I, _P, _L, _P, _P, _P, _L, _P]),
 }
 
 _lib = None

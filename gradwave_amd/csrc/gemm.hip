@@ -557,3 +557,267 @@ extern "C" int gw_zgemm_combine(int precision, int nk, int A, int G, int J, cons
   return gemm_combine<float>(nk, A, G, J, (const float2*)U, ldu, su, conj_u, (const float2*)Y, ldy, sy,
                              (float2*)Z, ldz, sz, (float)beta, (hipStream_t)stream);
 }
+
+// ---------------------------------------------------------------------------
+// Real contractions (Gamma layout, core/gamma.py: the real-embedded Davidson):
+//   inner   C[i][j] = sum_g X[i][g] Y[j][g]              split-K, fp64 partials
+//   combine Z[a][g] = beta Z + alpha sum_j U[j][a] Y[j][g]
+// Same structure as the LDS-staged complex kernels: 64x64 workgroup tiles of
+// four 32x32 wave tiles, 32-wide K chunks staged through padded LDS with the
+// next chunk prefetched into registers.
+constexpr int kKR = 32;
+template <typename T> constexpr int kLdR = kKR + (sizeof(T) == 8 ? 2 : 4);
+
+namespace {
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_rgemm_inner_lds(int M, int N, int K, const T* __restrict__ X, long ldx, long sx, const T* __restrict__ Y,
+                  long ldy, long sy, int splits, int kchunk, double* __restrict__ part, int ntx, int nty,
+                  long total) {
+  using A4 = typename Mfma<T>::acc;
+  constexpr int LD = kLdR<T>;
+  __shared__ T As[64 * LD];
+  __shared__ T Bs[64 * LD];
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const int bx = (int)(wid % ntx), by = (int)((wid / ntx) % nty), bz = (int)(wid / ((long)ntx * nty));
+  const int k = bz / splits, sp = bz - k * splits;
+  const int gb = sp * kchunk, ge = min(K, gb + kchunk);
+  const int row0 = by * 64, col0 = bx * 64;
+  const T* Xk = X + (long)k * sx;
+  const T* Yk = Y + (long)k * sy;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = w >> 1, wn = w & 1;
+  T ra[8], rb[8];
+  auto gload = [&](int g0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = tid + 256 * i, r = e >> 5, cc = e & 31;
+      const int g = g0 + cc;
+      const int xr = row0 + r, yr = col0 + r;
+      ra[i] = (xr < M && g < ge) ? Xk[(long)xr * ldx + g] : T(0);
+      rb[i] = (yr < N && g < ge) ? Yk[(long)yr * ldy + g] : T(0);
+    }
+  };
+  A4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = A4{0, 0, 0, 0};
+  if (gb < ge) gload(gb);
+  for (int g0 = gb; g0 < ge; g0 += kKR) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = tid + 256 * i;
+      As[(e >> 5) * LD + (e & 31)] = ra[i];
+      Bs[(e >> 5) * LD + (e & 31)] = rb[i];
+    }
+    __syncthreads();
+    if (g0 + kKR < ge) gload(g0 + kKR);
+    T av[2][8], bv[2][8];
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm) {
+      const T* ap = As + (wm * 32 + bm * 16 + li) * LD + lk * 8;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) av[bm][v] = ap[v];
+    }
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn) {
+      const T* bp = Bs + (wn * 32 + bn * 16 + li) * LD + lk * 8;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) bv[bn][v] = bp[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn) acc[bm][bn] = Mfma<T>::op(av[bm][v], bv[bn][v], acc[bm][bn]);
+  }
+  double* pk = part + ((long)k * splits + sp) * M * N;
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + wm * 32 + bm * 16 + Mfma<T>::row(lk, r);
+        const int col = col0 + wn * 32 + bn * 16 + li;
+        if (row < M && col < N) pk[(long)row * N + col] = (double)acc[bm][bn][r];
+      }
+}
+
+template <typename T>
+__global__ void k_rgemm_inner_reduce(int M, int N, int splits, const double* __restrict__ part, T* __restrict__ C,
+                                     long ldc, long sc) {
+  const int k = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long tot = (long)M * N;
+  if (e >= tot) return;
+  const int i = (int)(e / N), j = (int)(e - (long)i * N);
+  const double* pk = part + (long)k * splits * tot + e;
+  double s = 0;
+  for (int q = 0; q < splits; ++q) s += pk[(long)q * tot];
+  C[(long)k * sc + (long)i * ldc + j] = (T)s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_rgemm_combine_lds(int A, int G, int J, const T* __restrict__ U, long ldu, long su, const T* __restrict__ Y,
+                    long ldy, long sy, T* __restrict__ Z, long ldz, long sz, T alpha, T beta, int ntx, int nty,
+                    long total) {
+  using A4 = typename Mfma<T>::acc;
+  constexpr int LD = kLdR<T>;
+  __shared__ T Ut[64 * LD];  // [a][j]
+  __shared__ T Yt[64 * LD];  // [g][j]
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const int by = (int)(wid % nty), bx = (int)((wid / nty) % ntx), k = (int)(wid / ((long)ntx * nty));
+  const int a0 = by * 64, g0 = bx * 64;
+  const T* Uk = U + (long)k * su;
+  const T* Yk = Y + (long)k * sy;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = w >> 1, wn = w & 1;
+  T ru[8], ry[8];
+  auto gload = [&](int j0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {  // rows j (32) x columns (64), column fastest
+      const int e = tid + 256 * i, jr = e >> 6, c = e & 63;
+      const int j = j0 + jr;
+      ru[i] = (j < J && a0 + c < A) ? Uk[(long)j * ldu + a0 + c] : T(0);
+      ry[i] = (j < J && g0 + c < G) ? Yk[(long)j * ldy + g0 + c] : T(0);
+    }
+  };
+  A4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = A4{0, 0, 0, 0};
+  if (J > 0) gload(0);
+  for (int j0 = 0; j0 < J; j0 += kKR) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = tid + 256 * i;
+      Ut[(e & 63) * LD + (e >> 6)] = ru[i];
+      Yt[(e & 63) * LD + (e >> 6)] = ry[i];
+    }
+    __syncthreads();
+    if (j0 + kKR < J) gload(j0 + kKR);
+    T av[2][8], bv[2][8];
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm) {
+      const T* up = Ut + (wm * 32 + bm * 16 + li) * LD + lk * 8;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) av[bm][v] = up[v];
+    }
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn) {
+      const T* yp = Yt + (wn * 32 + bn * 16 + li) * LD + lk * 8;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) bv[bn][v] = yp[v];
+    }
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+#pragma unroll
+      for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int bn = 0; bn < 2; ++bn) acc[bm][bn] = Mfma<T>::op(av[bm][v], bv[bn][v], acc[bm][bn]);
+  }
+  T* Zk = Z + (long)k * sz;
+#pragma unroll
+  for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < 2; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = a0 + wm * 32 + bm * 16 + Mfma<T>::row(lk, r);
+        const int col = g0 + wn * 32 + bn * 16 + li;
+        if (row < A && col < G) {
+          T* zp = Zk + (long)row * ldz + col;
+          const T v = alpha * acc[bm][bn][r];
+          *zp = (beta != T(0)) ? v + beta * (*zp) : v;
+        }
+      }
+}
+
+int rinner_splits(int nk, int M, int N, int K) {
+  const long tiles = (long)((N + 63) / 64) * ((M + 63) / 64) * nk;
+  long s = (2048 + tiles - 1) / tiles;
+  const long smax = std::max<long>(1, (K + 4 * kKR - 1) / (4 * kKR));  // >= 4 chunks per split
+  s = std::max<long>(1, std::min<long>(s, smax));
+  return (int)std::min<long>(s, 4096);
+}
+
+template <typename T>
+int rgemm_inner(int nk, int M, int N, int K, const T* X, long ldx, long sx, const T* Y, long ldy, long sy, T* C,
+                long ldc, long sc, void* partials, size_t part_bytes, hipStream_t st) {
+  if (M <= 0 || N <= 0 || nk <= 0) return GW_OK;
+  GW_REQUIRE(K > 0, GW_ERR_ARG, "K must be positive");
+  const int splits = rinner_splits(nk, M, N, K);
+  const size_t need = (size_t)nk * splits * M * N * sizeof(double);
+  GW_REQUIRE(partials && part_bytes >= need, GW_ERR_WORKSPACE, "rgemm_inner workspace %zu < %zu", part_bytes,
+             need);
+  GW_REQUIRE(nk <= 65535, GW_ERR_UNSUPPORTED, "nk too large");
+  int kchunk = (K + splits - 1) / splits;
+  kchunk = ((kchunk + kKR - 1) / kKR) * kKR;
+  const int ntx = (N + 63) / 64, nty = (M + 63) / 64;
+  const long total = (long)ntx * nty * nk * splits;
+  hipLaunchKernelGGL(k_rgemm_inner_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X,
+                     ldx, sx, Y, ldy, sy, splits, kchunk, (double*)partials, ntx, nty, total);
+  const long tot = (long)M * N;
+  hipLaunchKernelGGL(k_rgemm_inner_reduce<T>, dim3((unsigned)((tot + 255) / 256), nk), dim3(256), 0, st, M, N,
+                     splits, (const double*)partials, C, ldc, sc);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
+template <typename T>
+int rgemm_combine(int nk, int A, int G, int J, const T* U, long ldu, long su, const T* Y, long ldy, long sy, T* Z,
+                  long ldz, long sz, T alpha, T beta, hipStream_t st) {
+  if (A <= 0 || G <= 0 || nk <= 0) return GW_OK;
+  const int ntx = (G + 63) / 64, nty = (A + 63) / 64;
+  const long total = (long)ntx * nty * nk;
+  hipLaunchKernelGGL(k_rgemm_combine_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, A, G, J,
+                     U, ldu, su, Y, ldy, sy, Z, ldz, sz, alpha, beta, ntx, nty, total);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
+}  // namespace
+
+extern "C" size_t gw_rgemm_inner_workspace_bytes(int precision, int nk, int M, int N, int K) {
+  (void)precision;
+  if (M <= 0 || N <= 0 || K <= 0 || nk <= 0) return 0;
+  return (size_t)nk * rinner_splits(nk, M, N, K) * M * N * sizeof(double);
+}
+
+extern "C" int gw_rgemm_inner(int precision, int nk, int M, int N, int K, const void* X, long ldx, long sx,
+                              const void* Y, long ldy, long sy, void* C, long ldc, long sc, void* ws,
+                              size_t ws_bytes, void* stream) {
+  GW_REQUIRE(X && Y && C && nk >= 0 && M >= 0 && N >= 0 && K >= 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(ldx >= K && ldy >= K && ldc >= N, GW_ERR_ARG, "leading dimension too small");
+  if (precision == GW_FP64)
+    return rgemm_inner<double>(nk, M, N, K, (const double*)X, ldx, sx, (const double*)Y, ldy, sy, (double*)C, ldc,
+                               sc, ws, ws_bytes, (hipStream_t)stream);
+  return rgemm_inner<float>(nk, M, N, K, (const float*)X, ldx, sx, (const float*)Y, ldy, sy, (float*)C, ldc, sc,
+                            ws, ws_bytes, (hipStream_t)stream);
+}
+
+extern "C" int gw_rgemm_combine(int precision, int nk, int A, int G, int J, const void* U, long ldu, long su,
+                                const void* Y, long ldy, long sy, void* Z, long ldz, long sz, double alpha,
+                                double beta, void* stream) {
+  GW_REQUIRE(U && Y && Z && nk >= 0 && A >= 0 && G >= 0 && J >= 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(ldu >= A && ldy >= G && ldz >= G, GW_ERR_ARG, "leading dimension too small");
+  if (precision == GW_FP64)
+    return rgemm_combine<double>(nk, A, G, J, (const double*)U, ldu, su, (const double*)Y, ldy, sy, (double*)Z,
+                                 ldz, sz, alpha, beta, (hipStream_t)stream);
+  return rgemm_combine<float>(nk, A, G, J, (const float*)U, ldu, su, (const float*)Y, ldy, sy, (float*)Z, ldz,
+                              sz, (float)alpha, (float)beta, (hipStream_t)stream);
+}

@@ -110,3 +110,52 @@ def row_norms(v: torch.Tensor) -> torch.Tensor:
                                  v.stride(0), out.data_ptr(), L.stream_ptr(v.device)),
             "gw_row_norms")
     return out
+
+
+def rinner(x: torch.Tensor, y: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+    """Real C = X Y^T over the long last axis (split-K MFMA, fp64 partial sums):
+    x (M, K) or (nk, M, K), y (N, K) or (nk, N, K) -> (M, N) / (nk, M, N)."""
+    L.require_gpu(x, "x")
+    x, y = _rows(x), _rows(y)
+    batched = x.dim() == 3
+    xb = x if batched else x[None]
+    yb = y if batched else y[None]
+    nk, M, K = xb.shape
+    N = yb.shape[1]
+    assert yb.shape[0] == nk and yb.shape[2] == K and x.dtype == y.dtype
+    prec = L.precision_of(x.dtype)
+    if out is None:
+        out = torch.empty(nk, M, N, dtype=x.dtype, device=x.device) if batched else \
+            torch.empty(M, N, dtype=x.dtype, device=x.device)
+    ob = out if batched else out[None]
+    if M == 0 or N == 0:
+        return out
+    need = int(L.lib().gw_rgemm_inner_workspace_bytes(prec, nk, M, N, K))
+    ws = _scratch(x.device, need)
+    L.check(L.lib().gw_rgemm_inner(prec, nk, M, N, K, xb.data_ptr(), xb.stride(1), xb.stride(0), yb.data_ptr(),
+                                   yb.stride(1), yb.stride(0), ob.data_ptr(), ob.stride(1), ob.stride(0),
+                                   ws.data_ptr(), ws.numel(), L.stream_ptr(x.device)), "gw_rgemm_inner")
+    return out
+
+
+def rcombine(u: torch.Tensor, y: torch.Tensor, out: torch.Tensor | None = None, alpha: float = 1.0,
+             beta: float = 0.0) -> torch.Tensor:
+    """Real Z = beta Z + alpha U^T Y: u (J, A), y (J, G) -> (A, G) (batched with a leading nk)."""
+    L.require_gpu(y, "y")
+    u, y = _rows(u), _rows(y)
+    batched = y.dim() == 3
+    ub = u if batched else u[None]
+    yb = y if batched else y[None]
+    nk, J, A = ub.shape
+    G = yb.shape[2]
+    assert yb.shape[0] == nk and yb.shape[1] == J and u.dtype == y.dtype
+    prec = L.precision_of(y.dtype)
+    if out is None:
+        out = torch.empty(nk, A, G, dtype=y.dtype, device=y.device) if batched else \
+            torch.empty(A, G, dtype=y.dtype, device=y.device)
+        beta = 0.0
+    ob = out if batched else out[None]
+    L.check(L.lib().gw_rgemm_combine(prec, nk, A, G, J, ub.data_ptr(), ub.stride(1), ub.stride(0), yb.data_ptr(),
+                                     yb.stride(1), yb.stride(0), ob.data_ptr(), ob.stride(1), ob.stride(0),
+                                     float(alpha), float(beta), L.stream_ptr(y.device)), "gw_rgemm_combine")
+    return out

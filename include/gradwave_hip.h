@@ -118,6 +118,19 @@ int gw_zgemm_combine(int precision, int nk, int A, int G, int J, const void* U, 
                      long su, int conj_u, const void* Y, long ldy, long sy, void* Z, long ldz,
                      long sz, double beta, void* stream);
 
+/* ---- real contractions on MFMA (the Gamma layout's real-embedded Davidson,
+ * core/gamma.py:291-320 over solvers/davidson.py:329-395) ----
+ * inner:   C[k][i][j] = sum_g X[k][i][g] Y[k][j][g]      (split-K, fp64 partial sums)
+ * combine: Z[k][a][g] = beta Z[k][a][g] + alpha sum_j U[k][j][a] Y[k][j][g]
+ * Real float64 / float32 data, strides in elements. */
+size_t gw_rgemm_inner_workspace_bytes(int precision, int nk, int M, int N, int K);
+int gw_rgemm_inner(int precision, int nk, int M, int N, int K, const void* X, long ldx, long sx,
+                   const void* Y, long ldy, long sy, void* C, long ldc, long sc,
+                   void* workspace, size_t workspace_bytes, void* stream);
+int gw_rgemm_combine(int precision, int nk, int A, int G, int J, const void* U, long ldu, long su,
+                     const void* Y, long ldy, long sy, void* Z, long ldz, long sz, double alpha,
+                     double beta, void* stream);
+
 /* ---- Davidson vector kernels ----
  * residual (davidson.py:656-657): r = hx - eig*x, rn[k][b] = ||r[k][b]|| (float64 out).
  * x, hx, r: (nk, nb, npw) complex with row stride ld and batch stride s; eig (nk, nb) float64. */
@@ -168,6 +181,22 @@ size_t gw_vloc_g_grad_workspace_bytes(int na);
 int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* species_index,
                    const double* tables, const double* g_cart, double volume, const void* grad_vloc_g,
                    double* grad_tau, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ---- Gamma-point real layout (core/gamma.py) ----
+ * Half-sphere vectors travel in the real embedding of core/gamma.py:242-262:
+ * phi[0] = Re c(0), phi[2h-1] = sqrt2 Re c(h), phi[2h] = sqrt2 Im c(h), length npw = 2 nhalf - 1.
+ * pack: z[j] = full(phi[2j]) + i full(phi[2j+1]) on the full sphere (npw points,
+ *   zero up to npw_max), full(c)[g] = c[src_half[g]], conjugated where conj_full[g];
+ *   z ((nb+1)/2, ldz) complex; a missing odd partner band counts as zero.
+ * unpack: H phi[2j] / H phi[2j+1] from Hz = H z: (Hz(G) +- conj Hz(-G)) / (2 or 2i)
+ *   on the half sphere (rep_full[h] = full index of half slot h, partner[g] = index of -G),
+ *   written in the embedding, out (nb, ldo) real. Together with gw_hpsi_apply_strided
+ *   on a one-k plan of the full sphere this is GammaHamiltonian.apply (core/gamma.py:206-228)
+ *   at two bands per complex apply. int32 maps, uint8 flags. */
+int gw_gamma_pack(int precision, int nb, int npw, int npw_max, const void* phi, long ldp,
+                  const int* src_half, const unsigned char* conj_full, void* z, long ldz, void* stream);
+int gw_gamma_unpack(int precision, int nb, int nhalf, const void* hz, long ldz, const int* rep_full,
+                    const int* partner, void* out, long ldo, void* stream);
 
 #ifdef __cplusplus
 }

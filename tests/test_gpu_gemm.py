@@ -84,3 +84,41 @@ def test_residual_and_teter_kernels():
     dref = teter_b(torch.gather(r_ref, 1, sel[..., None].expand(-1, -1, m)), t, torch.gather(tb, 1, sel))
     dref = dref / torch.linalg.norm(dref, dim=-1, keepdim=True)
     assert float((d - dref).abs().max()) < 1e-13
+
+
+RTOL = {torch.float64: 1e-13, torch.float32: 3e-6}
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("nk,M,N,K", [(1, 8, 8, 763), (3, 17, 33, 1000), (1, 640, 160, 37073),
+                                      (1, 70, 65, 4097), (2, 1, 1, 3)])
+def test_real_inner(dtype, nk, M, N, K):
+    say(f"rinner {dtype} {nk}x{M}x{N}x{K}")
+    g = torch.Generator().manual_seed(3)
+    x = torch.randn(nk, M, K, dtype=torch.float64, generator=g)
+    y = torch.randn(nk, N, K, dtype=torch.float64, generator=g)
+    want = torch.matmul(x.to(dtype).double(), y.to(dtype).double().mT)
+    got = LA.rinner(x.to(dev(), dtype), y.to(dev(), dtype)).cpu().double()
+    scale = float(x.norm(dim=-1).max() * y.norm(dim=-1).max())
+    assert float((got - want).abs().max()) < RTOL[dtype] * scale
+    if nk == 1:  # unbatched views with a row stride
+        xs = x[0].to(dev(), dtype)[::2]
+        got2 = LA.rinner(xs, y[0].to(dev(), dtype)).cpu().double()
+        assert float((got2 - want[0][::2]).abs().max()) < RTOL[dtype] * scale
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("nk,A,G,J", [(1, 8, 763, 8), (3, 33, 1000, 17), (1, 160, 37073, 640),
+                                      (1, 65, 4097, 70), (2, 1, 3, 1)])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.0, 1.0)])
+def test_real_combine(dtype, nk, A, G, J, alpha, beta):
+    say(f"rcombine {dtype} {nk}x{A}x{G}x{J} a={alpha} b={beta}")
+    g = torch.Generator().manual_seed(4)
+    u = torch.randn(nk, J, A, dtype=torch.float64, generator=g).to(dtype)
+    y = torch.randn(nk, J, G, dtype=torch.float64, generator=g).to(dtype)
+    z0 = torch.randn(nk, A, G, dtype=torch.float64, generator=g).to(dtype)
+    want = alpha * torch.matmul(u.double().mT, y.double()) + beta * z0.double()
+    z = z0.to(dev())
+    LA.rcombine(u.to(dev()), y.to(dev()), out=z, alpha=alpha, beta=beta)
+    scale = float(u.double().norm(dim=-2).max() * y.double().norm(dim=-2).max()) + float(z0.abs().max())
+    assert float((z.cpu().double() - want).abs().max()) < RTOL[dtype] * scale
