@@ -165,6 +165,8 @@ def main() -> None:
         "fp64_ms_per_step": ms64 * 1e3,
         "roofline": roof,
     }
+    if nk * m * m * 16 <= (4 << 30):
+        result["toeplitz"] = toeplitz_rates(args, wl, x32, x64)
 
     if not args.no_davidson:
         from gradwave_amd.workloads import HARTREE_EV
@@ -208,6 +210,33 @@ def main() -> None:
         result["cpu_baseline"] = cpu_baseline(args, wl)
 
     print(json.dumps(result), flush=True)
+
+
+def toeplitz_rates(args, wl, x32, x64) -> dict:
+    """Small-cell local term as a matrix (core/batch.py:333-464): full apply
+    band-applies/s with the path forced on, next to the FFT path."""
+    import torch
+
+    from gradwave_amd.hamiltonian import BatchedHamiltonian
+
+    ht = BatchedHamiltonian(wl.bk, wl.shape, wl.v_r, wl.p, toeplitz="on")
+    out = {}
+    for name, x in (("fp32", x32), ("fp64", x64)):
+        for _ in range(3):
+            ht.apply(x)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(max(5, args.steps // 2)):
+            a = time.perf_counter()
+            ht.apply(x)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - a)
+        ms = statistics.median(ts)
+        out[f"{name}_band_applies_per_s"] = x.shape[0] * x.shape[1] / ms
+        out[f"{name}_ms_per_step"] = ms * 1e3
+    print(f"[bench] toeplitz apply fp32 {out['fp32_ms_per_step']:.3f} ms fp64 {out['fp64_ms_per_step']:.3f} ms",
+          file=sys.stderr, flush=True)
+    return out
 
 
 def gamma_form(args, wl, x64, r64) -> dict:

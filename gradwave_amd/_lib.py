@@ -61,6 +61,7 @@ SIGNATURES = {
     "gw_rgemm_inner_workspace_bytes": (_Z, [_I, _I, _I, _I, _I]),
     "gw_rgemm_inner": (_I, [_I, _I, _I, _I, _I, _P, _L, _L, _P, _L, _L, _P, _L, _L, _P, _Z, _P]),
     "gw_rgemm_combine": (_I, [_I, _I, _I, _I, _I, _P, _L, _L, _P, _L, _L, _P, _L, _L, _D, _D, _P]),
+    "gw_toeplitz_build": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "gw_gamma_pack": (_I, [_I, _I, _I, _I, _P, _L, _P, _P, _P, _L, _P]),
     "gw_gamma_unpack": (_I, [_I, _I, _I, _P, _L, _P, _P, _P, _L, _P]),
 }

@@ -11,12 +11,26 @@ only owns the plan and the device workspace.
 from __future__ import annotations
 
 import ctypes as C
+import os
+import time
 
 import torch
 
 from gradwave_amd import _lib as L
 
 _HAPPLY_TALLY = {"on": False, "count": 0}
+
+# Small-cell local term as one dense GEMM per k (core/batch.py:48-107, 333-464):
+# out(G_i) = sum_j Vhat(G_i - G_j) c(G_j). GRADWAVE_TOEPLITZ in {auto, on, off}
+# as in the reference; like the reference on CUDA, "auto" keeps the FFT path on
+# the GPU unless GW_TOEPLITZ_ON_GPU=1 lets the timing trial decide (adopt when
+# >= 30 % faster). The matrix cap is larger than the reference's 256 MiB because
+# an MI355X has 288 GB of HBM.
+_TOEPLITZ_MODE = os.environ.get("GRADWAVE_TOEPLITZ", "auto").strip().lower()
+_TOEPLITZ_ON_GPU = os.environ.get("GW_TOEPLITZ_ON_GPU", "0") == "1"
+_TOEPLITZ_M_BUDGET_BYTES = int(os.environ.get("GW_TOEPLITZ_BUDGET", str(4 << 30)))
+_TOEP_TRIAL_MARGIN = 0.7
+_TOEP_VERDICT: dict[tuple, bool] = {}
 
 
 def reset_happly_tally() -> None:
@@ -94,9 +108,12 @@ class BatchedHamiltonian:
     """
 
     def __init__(self, bk, shape, v_eff_r: torch.Tensor, p: torch.Tensor, hub_q=None,
-                 hub_dij=None, smooth=None) -> None:
+                 hub_dij=None, smooth=None, toeplitz: str | None = None) -> None:
         if hub_q is not None or hub_dij is not None or smooth is not None:
             raise NotImplementedError("DFT+U and the USPP dual grid are not part of this operator")
+        mode = (toeplitz or _TOEPLITZ_MODE).strip().lower()
+        if mode not in ("auto", "on", "off"):
+            raise ValueError(f"toeplitz mode must be auto, on or off, got {mode!r}")
         L.require_gpu(bk.mask, "bk.mask")
         self.bk = bk
         self.shape = tuple(int(s) for s in shape)
@@ -109,6 +126,10 @@ class BatchedHamiltonian:
         self.plan.set_projectors(p, bk.dij_full)
         self._ws: dict[int, torch.Tensor] = {}
         self.device = bk.mask.device
+        nk, npw_max = self.plan.nk, self.plan.npw_max
+        self._toep_mode = mode
+        self._toep_eligible = mode != "off" and nk * npw_max * npw_max * 16 <= _TOEPLITZ_M_BUDGET_BYTES
+        self._toep_M: dict[torch.dtype, torch.Tensor] = {}
 
     def _workspace(self, nb: int, prec: int) -> torch.Tensor:
         need = int(L.lib().gw_hpsi_workspace_bytes(self.plan.handle, nb, prec))
@@ -143,7 +164,69 @@ class BatchedHamiltonian:
         """(nk, nb, npw_max) -> H c, mask preserved (core/batch.py:480-505)."""
         if _HAPPLY_TALLY["on"]:
             _HAPPLY_TALLY["count"] += int(c.shape[0] * c.shape[1])
+        if self._use_toeplitz(c):
+            out = self.apply_terms(c, L.GW_TERM_KINETIC | L.GW_TERM_NONLOCAL)
+            out += self.local_toeplitz(c)
+            return out
         return self.apply_terms(c, L.GW_TERM_ALL)
+
+    # -- Toeplitz local term (core/batch.py:333-464) --
+
+    def toeplitz_matrix(self, cdtype: torch.dtype) -> torch.Tensor:
+        """M[k][i][j] = Vhat(G_i - G_j), Vhat = fftn(v_eff) / N, zero in padding;
+        cached per precision for the operator's lifetime (batch.py:380-393)."""
+        M = self._toep_M.get(cdtype)
+        if M is None:
+            n1, n2, n3 = self.shape
+            vhat = (torch.fft.fftn(self.v_eff_r.to(self.device, torch.float64)) / self.n).to(cdtype).contiguous()
+            nk, m = self.plan.nk, self.plan.npw_max
+            M = torch.empty(nk, m, m, dtype=cdtype, device=self.device)
+            npw32 = self.bk.npw.to(self.device, torch.int32).contiguous()
+            flat32 = self.bk.flat_idx.to(self.device, torch.int32).contiguous()
+            L.check(L.lib().gw_toeplitz_build(L.precision_of(cdtype), nk, m, n1, n2, n3, npw32.data_ptr(),
+                                              flat32.data_ptr(), vhat.data_ptr(), M.data_ptr(),
+                                              L.stream_ptr(self.device)), "gw_toeplitz_build")
+            self._toep_M[cdtype] = M
+        return M
+
+    def local_toeplitz(self, c: torch.Tensor) -> torch.Tensor:
+        """Local V psi as one MFMA contraction per k: out[b][i] = sum_j M[i][j] c[b][j]
+        (batch.py:395-403), computed as inner(conj(c), M); padded input ignored."""
+        from gradwave_amd import linalg as LA
+
+        M = self.toeplitz_matrix(c.dtype)
+        cm = (c * self.bk.mask[:, None, :]).conj().resolve_conj()
+        return LA.inner(cm, M)
+
+    def _use_toeplitz(self, c: torch.Tensor) -> bool:
+        """The reference's gate (batch.py:430-464): "on" forces the path within the
+        memory cap; "auto" times both local terms once per (nk, npw_max, shape,
+        dtype) signature and adopts Toeplitz only when >= 30 % faster -- on the
+        GPU only with GW_TOEPLITZ_ON_GPU=1, as the reference keeps it off on CUDA."""
+        if not self._toep_eligible:
+            return False
+        if self._toep_mode == "on":
+            return True
+        if not _TOEPLITZ_ON_GPU:
+            return False
+        key = (self.plan.nk, self.plan.npw_max, self.shape, c.dtype)
+        verdict = _TOEP_VERDICT.get(key)
+        if verdict is None:
+            def timed(f):
+                f()
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                f()
+                torch.cuda.synchronize()
+                return time.perf_counter() - t0
+
+            t_fft = timed(lambda: self.apply_terms(c, L.GW_TERM_LOCAL))
+            t_toep = timed(lambda: self.local_toeplitz(c))
+            verdict = t_toep < _TOEP_TRIAL_MARGIN * t_fft
+            _TOEP_VERDICT[key] = verdict
+            if not verdict:
+                self._toep_M.clear()
+        return verdict
 
     __call__ = apply
 

@@ -182,6 +182,14 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
                    const double* tables, const double* g_cart, double volume, const void* grad_vloc_g,
                    double* grad_tau, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- small-cell local term as a matrix (core/batch.py:333-464) ----
+ * M[k][i][j] = vhat[(G_i - G_j) wrapped into the box] for i, j < npw[k], else 0;
+ * vhat (n1 n2 n3) complex = fftn(v_eff) / N; npw (nk) and flat_idx (nk, npw_max)
+ * int32 on the device; M (nk, npw_max, npw_max) complex. The local term is then
+ * out[k][b][i] = sum_j M[k][i][j] c[k][b][j] (gw_zgemm_inner of conj(c) with M). */
+int gw_toeplitz_build(int precision, int nk, int npw_max, int n1, int n2, int n3, const int* npw,
+                      const int* flat_idx, const void* vhat, void* M, void* stream);
+
 /* ---- Gamma-point real layout (core/gamma.py) ----
  * Half-sphere vectors travel in the real embedding of core/gamma.py:242-262:
  * phi[0] = Re c(0), phi[2h-1] = sqrt2 Re c(h), phi[2h] = sqrt2 Im c(h), length npw = 2 nhalf - 1.

@@ -202,3 +202,29 @@ def test_si216_gamma_workload_against_oracle():
     assert int(bk.npw[0]) == 80797
     assert _rel(got, want) < 1e-12
     assert _rel(got32, want) < 2e-5
+
+
+@pytest.mark.parametrize("name,dtype", [("si2", torch.complex128), ("si2", torch.complex64),
+                                        ("si8", torch.complex128)])
+def test_toeplitz_local_matches_oracle_and_fft(name, dtype):
+    """Small-cell matrix local term (core/batch.py:333-464) against the oracle's
+    toeplitz_local and the FFT path, plus the full apply with the path forced on."""
+    from oracle.hamiltonian import toeplitz_local
+
+    say(f"toeplitz {name} {dtype}")
+    s = system(name)
+    sel = None if name == "si2" else [0, 3]
+    cpu, gpu, p = select_k(s, sel)
+    d = dev()
+    x = rand_block(cpu.npw.shape[0], 6, cpu.mask.shape[1], cpu.mask, seed=5)
+    ht = BatchedHamiltonian(gpu, s.shape, s.v_r.to(d), p.to(d), toeplitz="on")
+    hf = BatchedHamiltonian(gpu, s.shape, s.v_r.to(d), p.to(d), toeplitz="off")
+    xd = x.to(d, dtype)
+    loc = ht.local_toeplitz(xd).cpu().to(torch.complex128)
+    want = toeplitz_local(cpu, s.shape, s.v_r, x)
+    tol = 1e-12 if dtype == torch.complex128 else 2e-5
+    assert _rel(loc, want) < tol
+    assert _rel(hf.apply_terms(xd, L.GW_TERM_LOCAL).cpu().to(torch.complex128), want) < tol
+    full = ht.apply(xd).cpu().to(torch.complex128)
+    ref = ref_h(s, sel).apply(x)
+    assert _rel(full, ref) < tol
