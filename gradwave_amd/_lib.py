@@ -61,6 +61,12 @@ SIGNATURES = {
     "gw_rgemm_inner_workspace_bytes": (_Z, [_I, _I, _I, _I, _I]),
     "gw_rgemm_inner": (_I, [_I, _I, _I, _I, _I, _P, _L, _L, _P, _L, _L, _P, _L, _L, _P, _Z, _P]),
     "gw_rgemm_combine": (_I, [_I, _I, _I, _I, _I, _P, _L, _L, _P, _L, _L, _P, _L, _L, _D, _D, _P]),
+    "gw_batch_create": (_I, [C.POINTER(_P), _P, _I]),
+    "gw_batch_destroy": (_I, [_P]),
+    "gw_batch_workspace_bytes": (_Z, [_P, _P, _I]),
+    "gw_batch_bind": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _Z]),
+    "gw_batch_apply_bound": (_I, [_P, _P]),
+    "gw_batch_apply": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P]),
     "gw_toeplitz_build": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "gw_gamma_pack": (_I, [_I, _I, _I, _I, _P, _L, _P, _P, _P, _L, _P]),
     "gw_gamma_unpack": (_I, [_I, _I, _I, _P, _L, _P, _P, _P, _L, _P]),

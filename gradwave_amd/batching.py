@@ -1,0 +1,288 @@
+"""Ragged batches of independent plane-wave problems on MI355X (the batching form).
+
+The reference solves one system at a time: one ``BatchedHamiltonian`` per
+cell (core/batch.py:238-505) and one ``davidson_batched`` call per solve
+(solvers/davidson.py:527-774), so many small cells leave a large GPU idle
+between tiny launches. Here:
+
+* ``RaggedBatch`` applies many operators -- different boxes, G-spheres, V(r),
+  projector counts and band counts -- as ONE launch per local-term pass over the
+  concatenated work items of every instance (``gw_batch_apply``), followed by
+  each instance's MFMA projector GEMMs; ``capture`` records a fixed-shape
+  batched apply in a HIP graph and ``replay`` re-launches it.
+* ``davidson_ragged`` runs one ``davidson_steps`` generator per instance (the
+  same solver as ``davidson_batched``) in lockstep, decode-style: each round it
+  gathers every pending apply request into one ragged apply and every pending
+  subspace eigensolve into one batched eigensolve, so the per-solve results are
+  those of ``davidson_batched`` on each instance alone.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from gradwave_amd import _lib as L
+from gradwave_amd import workloads as W
+from gradwave_amd.davidson import BatchedDavidsonResult, davidson_steps
+from gradwave_amd.hamiltonian import BatchedHamiltonian
+
+
+class RaggedBatch:
+    """Independent operators applied together; ``hams`` are BatchedHamiltonian
+    objects (their plans hold the geometry, V(r), projectors and D)."""
+
+    def __init__(self, hams: list[BatchedHamiltonian]) -> None:
+        self.hams = list(hams)
+        n = len(self.hams)
+        arr = (C.c_void_p * n)(*[h.plan.handle.value for h in self.hams])
+        self.handle = C.c_void_p()
+        self._lib = L.lib()
+        L.check(self._lib.gw_batch_create(C.byref(self.handle), arr, n), "gw_batch_create")
+        self._ws: dict[int, torch.Tensor] = {}
+        self._keep = None
+        self.device = self.hams[0].device
+
+    def __len__(self) -> int:
+        return len(self.hams)
+
+    def __del__(self) -> None:
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self._lib.gw_batch_destroy(h)
+            except Exception:
+                pass
+            self.handle = C.c_void_p()
+
+    def _args(self, cs, outs):
+        n = len(self.hams)
+        cp, op = (C.c_void_p * n)(), (C.c_void_p * n)()
+        ldc, sc, ldo, so = (C.c_long * n)(), (C.c_long * n)(), (C.c_long * n)(), (C.c_long * n)()
+        nb = (C.c_int * n)()
+        prec = None
+        for i, (c, o) in enumerate(zip(cs, outs)):
+            if c is None or c.shape[1] == 0:
+                continue
+            h = self.hams[i]
+            if c.dim() != 3 or c.shape[0] != h.plan.nk or c.shape[2] != h.plan.npw_max or c.stride(2) != 1:
+                raise ValueError(f"instance {i}: block must be (nk={h.plan.nk}, nb, npw_max={h.plan.npw_max}) "
+                                 f"with a contiguous last axis, got {tuple(c.shape)}")
+            p = L.precision_of(c.dtype)
+            if prec is None:
+                prec = p
+            elif p != prec:
+                raise TypeError("all blocks of one batched apply must share a precision")
+            cp[i], op[i] = c.data_ptr(), o.data_ptr()
+            ldc[i], sc[i], ldo[i], so[i] = c.stride(1), c.stride(0), o.stride(1), o.stride(0)
+            nb[i] = c.shape[1]
+        return prec, cp, ldc, sc, op, ldo, so, nb
+
+    def _workspace(self, nb, prec: int) -> torch.Tensor:
+        need = int(self._lib.gw_batch_workspace_bytes(self.handle, nb, prec))
+        ws = self._ws.get(prec)
+        if ws is None or ws.numel() < need:
+            ws = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
+            self._ws[prec] = ws
+        return ws
+
+    def apply(self, cs: list, terms: int = L.GW_TERM_ALL) -> list:
+        """H_i c_i for every instance i with a block (None or 0 bands: skipped)."""
+        outs = [None if c is None else torch.empty_like(c) for c in cs]
+        prec, cp, ldc, sc, op, ldo, so, nb = self._args(cs, outs)
+        if prec is None:
+            return outs
+        ws = self._workspace(nb, prec)
+        L.check(self._lib.gw_batch_apply(self.handle, prec, terms, cp, ldc, sc, op, ldo, so, nb, ws.data_ptr(),
+                                         ws.numel(), L.stream_ptr(self.device)), "gw_batch_apply")
+        return outs
+
+    def bind(self, cs: list, outs: list, terms: int = L.GW_TERM_ALL) -> None:
+        """Fix the blocks of later ``apply_bound`` calls (synchronous upload)."""
+        prec, cp, ldc, sc, op, ldo, so, nb = self._args(cs, outs)
+        if prec is None:
+            raise ValueError("nothing to bind")
+        ws = self._workspace(nb, prec)
+        L.check(self._lib.gw_batch_bind(self.handle, prec, terms, cp, ldc, sc, op, ldo, so, nb, ws.data_ptr(),
+                                        ws.numel()), "gw_batch_bind")
+        self._keep = (list(cs), list(outs), ws)
+
+    def apply_bound(self) -> None:
+        L.check(self._lib.gw_batch_apply_bound(self.handle, L.stream_ptr(self.device)), "gw_batch_apply_bound")
+
+    def capture(self, cs: list, outs: list, terms: int = L.GW_TERM_ALL) -> torch.cuda.CUDAGraph:
+        """Bind fixed blocks and record the batched apply in a HIP graph; the
+        graph's ``replay()`` re-runs it on whatever the bound inputs hold."""
+        self.bind(cs, outs, terms)
+        self.apply_bound()  # warm (first-use module loads stay out of the graph)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.apply_bound()
+        torch.cuda.current_stream().wait_stream(s)
+        return g
+
+
+# ---------------------------------------------------------------- lockstep solves
+
+
+def _eigh_batched(mats: list[torch.Tensor]) -> list[tuple[torch.Tensor, torch.Tensor]]:
+    """Eigenpairs of many Hermitian subspace matrices (each (nk, m_i, m_i)) in one
+    batched FP64 eigensolve: every matrix is embedded in the top-left corner of an
+    (m_max, m_max) one whose remaining diagonal is a value above its spectrum, so
+    its own eigenpairs are the lowest m_i, with eigenvectors zero below m_i."""
+    m_max = max(int(s.shape[-1]) for s in mats)
+    total = sum(int(s.shape[0]) for s in mats)
+    dev = mats[0].device
+    big = torch.empty(total, m_max, m_max, dtype=torch.complex128, device=dev)
+    big.zero_()
+    row = 0
+    for s in mats:
+        nk, m = int(s.shape[0]), int(s.shape[-1])
+        blk = big[row:row + nk]
+        blk[:, :m, :m] = s.to(torch.complex128)
+        if m < m_max:
+            shift = 2.0 * float(torch.linalg.matrix_norm(s.to(torch.complex128), ord="fro").max()) + 1.0
+            idx = torch.arange(m, m_max, device=dev)
+            blk[:, idx, idx] = shift
+        row += nk
+    w, u = torch.linalg.eigh(big)
+    out, row = [], 0
+    for s in mats:
+        nk, m = int(s.shape[0]), int(s.shape[-1])
+        out.append((w[row:row + nk, :m].contiguous(), u[row:row + nk, :m, :m].contiguous()))
+        row += nk
+    return out
+
+
+@dataclass
+class RaggedSolveStats:
+    rounds: int = 0
+    batched_applies: int = 0
+    batched_eighs: int = 0
+
+
+def davidson_ragged(batch: RaggedBatch, x0s: list[torch.Tensor], ts: list[torch.Tensor],
+                    masks: list[torch.Tensor], tol: float = 1e-9, max_iter: int = 40,
+                    max_dim_factor: int = 4, stats: RaggedSolveStats | None = None) -> list[BatchedDavidsonResult]:
+    """One ``davidson_steps`` solve per instance, advanced in lockstep: each round
+    serves all pending apply requests with one ragged apply (one per precision)
+    and all pending subspace eigensolves with one batched eigensolve."""
+    gens = [davidson_steps(x0, t, m, tol=tol, max_iter=max_iter, max_dim_factor=max_dim_factor)
+            for x0, t, m in zip(x0s, ts, masks)]
+    results: list = [None] * len(gens)
+    pending: dict[int, tuple] = {}
+
+    def advance(i, resp=None, first=False):
+        try:
+            pending[i] = next(gens[i]) if first else gens[i].send(resp)
+        except StopIteration as stop:
+            results[i] = stop.value
+            pending.pop(i, None)
+
+    for i in range(len(gens)):
+        advance(i, first=True)
+    st = stats if stats is not None else RaggedSolveStats()
+    while pending:
+        st.rounds += 1
+        resp: dict[int, object] = {}
+        applies = {i: a for i, (k, a) in pending.items() if k == "apply"}
+        by_dtype: dict[torch.dtype, dict[int, torch.Tensor]] = {}
+        for i, z in applies.items():
+            by_dtype.setdefault(z.dtype, {})[i] = z if z.stride(-1) == 1 else z.contiguous()
+        for group in by_dtype.values():
+            cs = [group.get(i) for i in range(len(batch))]
+            outs = batch.apply(cs)
+            st.batched_applies += 1
+            for i in group:
+                resp[i] = outs[i]
+        eighs = {i: s for i, (k, s) in pending.items() if k == "eigh"}
+        if eighs:
+            keys = list(eighs)
+            for i, wu in zip(keys, _eigh_batched([eighs[i] for i in keys])):
+                resp[i] = wu
+            st.batched_eighs += 1
+        for i, r in resp.items():
+            advance(i, r)
+    return results
+
+
+def davidson_ragged_mixed(batch: RaggedBatch, x0s, ts, masks, tol: float = 1e-9, draft_tol: float = 1e-4,
+                          max_iter: int = 100, max_dim_factor: int = 4,
+                          stats: RaggedSolveStats | None = None) -> list[BatchedDavidsonResult]:
+    """Precision form of the batch: all instances draft in complex64 (FP32 FFTs,
+    V multiply and MFMA GEMMs) to ``draft_tol`` in lockstep, then polish in
+    complex128 to ``tol`` from the drafted Ritz vectors, FP64 subspace algebra
+    and residual norms throughout (the scaffold of solvers/_ms.py:26-52)."""
+    hi = [x.dtype for x in x0s]
+    d = davidson_ragged(batch, [x.to(torch.complex64) for x in x0s], [t.to(torch.float32) for t in ts], masks,
+                        tol=draft_tol, max_iter=max_iter, max_dim_factor=max_dim_factor, stats=stats)
+    p = davidson_ragged(batch, [r.eigenvectors.to(h) for r, h in zip(d, hi)], ts, masks, tol=tol,
+                        max_iter=max_iter, max_dim_factor=max_dim_factor, stats=stats)
+    for pi, di in zip(p, d):
+        pi.n_apply_low += di.n_apply_full
+        pi.n_draft_iter = di.n_iter
+    return p
+
+
+# ---------------------------------------------------------------- the ragged workload
+
+
+def fft_shape(cell: np.ndarray, ecut: float) -> tuple[int, int, int]:
+    """Box sizing of grids.py:68-126: n_i = good size >= 2 m_i + 1 for the density
+    cutoff 4 ecut, 7-smooth."""
+    gmax = 2.0 * math.sqrt(ecut / W.HBAR2_2M)
+    dims = []
+    for i in range(3):
+        n = 2 * int(np.floor(gmax * np.linalg.norm(cell[i]) / (2.0 * np.pi))) + 1
+        while True:
+            m = n
+            for p in (2, 3, 5, 7):
+                while m % p == 0:
+                    m //= p
+            if m == 1:
+                break
+            n += 1
+        dims.append(n)
+    return tuple(dims)
+
+
+def ragged_instances(device="cuda", n: int = 32, seed: int = 0) -> list[W.Workload]:
+    """32 independent cells: fcc Al (a = 4.05 A, Z = 3, rc = 0.6 bohr), Si2 fcc,
+    Si8 (conventional) and Si16 (2x2x2 fcc primitive) with seeded U(+-0.03 A) jitter, each at a
+    seeded random k-point, Ecut uniform in [25, 40] Ry, boxes sized by the
+    reference's rule, 8 (Al1), 8 (Si2), 32 (Si8) and 64 (Si16) bands."""
+    rng = np.random.default_rng(seed)
+    out = []
+    kinds = ["al1", "si2", "si8", "si16"]
+    for i in range(n):
+        kind = kinds[i % 4]
+        ecut_ry = float(rng.uniform(25.0, 40.0))
+        k = rng.uniform(-0.5, 0.5, 3)
+        if kind == "al1":
+            cell = W.fcc_primitive(W.A_AL)
+            pos, species, nb = np.zeros((1, 3)), [W.AL], 8
+        elif kind == "si2":
+            cell = W.fcc_primitive(W.A_SI)
+            pos, species, nb = np.array([[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]]) * W.A_SI, [W.SI] * 2, 8
+        elif kind == "si8":
+            cell, pos = W.si_conventional()
+            pos, species, nb = W.jitter(pos, 0.03, seed + i), [W.SI] * 8, 32
+        else:
+            prim = W.fcc_primitive(W.A_SI)
+            cell = 2.0 * prim
+            basis = np.array([[0.0, 0.0, 0.0], [0.25, 0.25, 0.25]]) * W.A_SI
+            shifts = np.array([[a, b, c] for a in (0, 1) for b in (0, 1) for c in (0, 1)], dtype=np.float64) @ prim
+            pos = (shifts[:, None, :] + basis[None]).reshape(-1, 3)
+            pos, species, nb = W.jitter(pos, 0.03, seed + i), [W.SI] * 16, 64
+        shape = fft_shape(cell, ecut_ry * W.RY_EV)
+        out.append(W.build(f"{kind}_{ecut_ry:.1f}Ry_{shape[0]}x{shape[1]}x{shape[2]}", cell, pos, species, ecut_ry,
+                           k[None], [1.0], nb, shape, device))
+    return out

@@ -153,14 +153,12 @@ __device__ __forceinline__ long xcd_work_id(long total) {
 }
 
 
-// P1 fused: scatter -> inverse z (sticks) -> inverse y (all columns) -> store plane.
+// P1 fused: scatter -> inverse z (sticks) -> inverse y (all columns) -> store plane,
+// for x-plane a of band row bi (= k * nb + b); `tile` holds n2 * n3 elements.
 template <typename T>
-__global__ void __launch_bounds__(kPlaneThreads)
-k_plane_inv(Geom g, const cpx<T>* __restrict__ c, long ldc, long sc, cpx<T>* __restrict__ box,
-            const cpx<T>* __restrict__ twy, const cpx<T>* __restrict__ twz) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
-  const int a = blockIdx.x, bi = blockIdx.y;
+__device__ void plane_inv_body(const Geom& g, int a, int bi, const cpx<T>* __restrict__ c, long ldc, long sc,
+                               cpx<T>* __restrict__ box, const cpx<T>* __restrict__ twy,
+                               const cpx<T>* __restrict__ twz, cpx<T>* tile) {
   const int k = bi / g.nb, b = bi - k * g.nb;
   const int n23 = g.n2 * g.n3;
   for (int e = threadIdx.x; e < n23; e += blockDim.x) tile[e] = czero<T>();
@@ -185,15 +183,21 @@ k_plane_inv(Geom g, const cpx<T>* __restrict__ c, long ldc, long sc, cpx<T>* __r
   for (int e = threadIdx.x; e < n23; e += blockDim.x) dst[e] = tile[e];
 }
 
-// P2: inverse x-FFT, x V(r), forward x-FFT over a chunk of L yz-lines.
 template <typename T>
-__global__ void __launch_bounds__(kXThreads)
-k_xpass(Geom g, int L, int nbt, cpx<T>* __restrict__ box, const T* __restrict__ vp,
-        const cpx<T>* __restrict__ twx) {
+__global__ void __launch_bounds__(kPlaneThreads)
+k_plane_inv(Geom g, const cpx<T>* __restrict__ c, long ldc, long sc, cpx<T>* __restrict__ box,
+            const cpx<T>* __restrict__ twy, const cpx<T>* __restrict__ twz) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);  // [n1][L]
+  plane_inv_body<T>(g, blockIdx.x, blockIdx.y, c, ldc, sc, box, twy, twz, reinterpret_cast<cpx<T>*>(smem_raw));
+}
+
+// P2: inverse x-FFT, x V(r), forward x-FFT over chunk `chunk` of L yz-lines for
+// band group `bgrp` (kXBandGroup bands); `tile` holds n1 * L elements.
+template <typename T>
+__device__ void xpass_body(const Geom& g, int L, int nbt, int chunk, int bgrp, cpx<T>* __restrict__ box,
+                           const T* __restrict__ vp, const cpx<T>* __restrict__ twx, cpx<T>* tile) {
   const int n23 = g.n2 * g.n3;
-  const int j0 = blockIdx.x * L;
+  const int j0 = chunk * L;
   const int nl = min(L, n23 - j0);
   const int ne = g.n1 * L;
   T vr[kXVpt];
@@ -208,7 +212,7 @@ k_xpass(Geom g, int L, int nbt, cpx<T>* __restrict__ box, const T* __restrict__ 
     vr[i] = v;
   }
   for (int bg = 0; bg < kXBandGroup; ++bg) {
-    const int bi = blockIdx.y * kXBandGroup + bg;
+    const int bi = bgrp * kXBandGroup + bg;
     if (bi >= nbt) break;
     for (int e = threadIdx.x; e < ne; e += blockDim.x) tile[e] = czero<T>();
     __syncthreads();
@@ -234,6 +238,14 @@ k_xpass(Geom g, int L, int nbt, cpx<T>* __restrict__ box, const T* __restrict__ 
     }
     __syncthreads();
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kXThreads)
+k_xpass(Geom g, int L, int nbt, cpx<T>* __restrict__ box, const T* __restrict__ vp,
+        const cpx<T>* __restrict__ twx) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  xpass_body<T>(g, L, nbt, blockIdx.x, blockIdx.y, box, vp, twx, reinterpret_cast<cpx<T>*>(smem_raw));
 }
 
 // P2 with wave-cooperative lines (gw_regfft.h): each x-line is spread over S
@@ -300,13 +312,10 @@ k_xpass_wave(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __r
 
 // P3 fused: load plane -> forward y (all columns) -> forward z (sticks) -> gather (+ t c).
 template <typename T, bool KIN>
-__global__ void __launch_bounds__(kPlaneThreads)
-k_plane_fwd(Geom g, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ c, long ldc,
-            long sc, const T* __restrict__ t, cpx<T>* __restrict__ out, long ldo, long so,
-            T scale, const cpx<T>* __restrict__ twy, const cpx<T>* __restrict__ twz) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
-  const int a = blockIdx.x, bi = blockIdx.y;
+__device__ void plane_fwd_body(const Geom& g, int a, int bi, const cpx<T>* __restrict__ box,
+                               const cpx<T>* __restrict__ c, long ldc, long sc, const T* __restrict__ t,
+                               cpx<T>* __restrict__ out, long ldo, long so, T scale, const cpx<T>* __restrict__ twy,
+                               const cpx<T>* __restrict__ twz, cpx<T>* tile) {
   const int k = bi / g.nb, b = bi - k * g.nb;
   const int n23 = g.n2 * g.n3;
   const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
@@ -341,6 +350,16 @@ k_plane_fwd(Geom g, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ c
     }
     ob[s] = v;
   }
+}
+
+template <typename T, bool KIN>
+__global__ void __launch_bounds__(kPlaneThreads)
+k_plane_fwd(Geom g, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ c, long ldc,
+            long sc, const T* __restrict__ t, cpx<T>* __restrict__ out, long ldo, long so,
+            T scale, const cpx<T>* __restrict__ twy, const cpx<T>* __restrict__ twz) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  plane_fwd_body<T, KIN>(g, blockIdx.x, blockIdx.y, box, c, ldc, sc, t, out, ldo, so, scale, twy, twz,
+                         reinterpret_cast<cpx<T>*>(smem_raw));
 }
 
 // ---- unfused path (plane tile larger than LDS) ----
@@ -1544,4 +1563,267 @@ extern "C" int gw_hpsi_adjoint_v(gw_plan* h, int precision, const void* c, long 
                           (unsigned char*)ws, (hipStream_t)stream);
   return adjv_t<float>(h, (const float2*)c, ldc, sc, (const float2*)g, ldg, sg, nb, grad_v, (unsigned char*)ws,
                        (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ ragged batches
+// Many independent operators (different boxes, spheres, potentials, projector
+// counts and band counts) applied together: each pass of the local term is ONE
+// launch over the concatenated work items of all instances (a workgroup looks up
+// its instance in a device table of descriptors), so a batch of small cells
+// fills the chip instead of issuing a few dozen under-occupied launches.
+
+namespace gw {
+
+template <typename T>
+struct RagInst {
+  Geom g;
+  const cpx<T>* c;
+  long ldc, sc;
+  cpx<T>* out;
+  long ldo, so;
+  const T* t;
+  const T* vp;
+  const cpx<T>* twx;
+  const cpx<T>* twy;
+  const cpx<T>* twz;
+  cpx<T>* box;
+  T scale;
+  int XL, nbt, nchunk, kin;
+  long w13, w2;  // first work item of this instance in the plane passes / in the x pass
+};
+
+template <typename T>
+__device__ __forceinline__ int rag_find(const RagInst<T>* R, int n, long w, bool xpass) {
+  int i = 0;
+  while (i + 1 < n && (xpass ? R[i + 1].w2 : R[i + 1].w13) <= w) ++i;
+  return i;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kPlaneThreads) k_plane_inv_rag(const RagInst<T>* __restrict__ R, int n, long total) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const long w = xcd_work_id(total);
+  if (w < 0) return;
+  const RagInst<T>& d = R[rag_find(R, n, w, false)];
+  const long local = w - d.w13;
+  const int a = (int)(local % d.g.na), bi = (int)(local / d.g.na);
+  plane_inv_body<T>(d.g, a, bi, d.c, d.ldc, d.sc, d.box, d.twy, d.twz, reinterpret_cast<cpx<T>*>(smem_raw));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kXThreads) k_xpass_rag(const RagInst<T>* __restrict__ R, int n, long total) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const long w = xcd_work_id(total);
+  if (w < 0) return;
+  const RagInst<T>& d = R[rag_find(R, n, w, true)];
+  const long local = w - d.w2;
+  const int chunk = (int)(local % d.nchunk), bgrp = (int)(local / d.nchunk);
+  xpass_body<T>(d.g, d.XL, d.nbt, chunk, bgrp, d.box, d.vp, d.twx, reinterpret_cast<cpx<T>*>(smem_raw));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kPlaneThreads) k_plane_fwd_rag(const RagInst<T>* __restrict__ R, int n, long total) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const long w = xcd_work_id(total);
+  if (w < 0) return;
+  const RagInst<T>& d = R[rag_find(R, n, w, false)];
+  const long local = w - d.w13;
+  const int a = (int)(local % d.g.na), bi = (int)(local / d.g.na);
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
+  if (d.kin)
+    plane_fwd_body<T, true>(d.g, a, bi, d.box, d.c, d.ldc, d.sc, d.t, d.out, d.ldo, d.so, d.scale, d.twy, d.twz, tile);
+  else
+    plane_fwd_body<T, false>(d.g, a, bi, d.box, d.c, d.ldc, d.sc, d.t, d.out, d.ldo, d.so, d.scale, d.twy, d.twz,
+                             tile);
+}
+
+}  // namespace gw
+
+struct gw_batch {
+  std::vector<gw_plan*> plans;
+  void* table = nullptr;  // device copy of the descriptors
+  size_t table_cap = 0;
+  // bound call
+  int prec = -1, terms = 0;
+  long tot13 = 0, tot2 = 0;
+  size_t lds13 = 0, lds2 = 0;
+  struct Bound {
+    const void* c; long ldc, sc;
+    void* out; long ldo, so;
+    int nb;
+    unsigned char* ws;
+  };
+  std::vector<Bound> bound;
+};
+
+static size_t batch_ws_bytes(const gw_batch* b, const int* nb, int precision) {
+  size_t tot = 0;
+  for (size_t i = 0; i < b->plans.size(); ++i)
+    if (nb[i] > 0) tot += align256(apply_layout(&b->plans[i]->p, nb[i], precision).total);
+  return tot;
+}
+
+extern "C" int gw_batch_create(gw_batch** out, gw_plan* const* plans, int n) {
+  GW_REQUIRE(out && plans && n > 0, GW_ERR_ARG, "bad argument");
+  auto* b = new gw_batch();
+  for (int i = 0; i < n; ++i) {
+    if (!plans[i]) {
+      delete b;
+      GW_REQUIRE(false, GW_ERR_ARG, "plan %d is null", i);
+    }
+    b->plans.push_back(plans[i]);
+  }
+  b->table_cap = (size_t)n * sizeof(RagInst<double>);
+  if (hipMalloc(&b->table, b->table_cap) != hipSuccess) {
+    delete b;
+    GW_REQUIRE(false, GW_ERR_HIP, "hipMalloc of the batch table failed");
+  }
+  const int want = kMaxLds;
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_rag<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_rag<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_rag<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_rag<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_xpass_rag<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_xpass_rag<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  *out = b;
+  return GW_OK;
+}
+
+extern "C" int gw_batch_destroy(gw_batch* b) {
+  if (!b) return GW_OK;
+  if (b->table) (void)hipFree(b->table);
+  delete b;
+  return GW_OK;
+}
+
+extern "C" size_t gw_batch_workspace_bytes(const gw_batch* b, const int* nb, int precision) {
+  if (!b || !nb) return 0;
+  return batch_ws_bytes(b, nb, precision);
+}
+
+template <typename T>
+static int batch_bind_t(gw_batch* b, int terms, const void* const* c, const long* ldc, const long* sc,
+                        void* const* out, const long* ldo, const long* so, const int* nb, unsigned char* ws) {
+  const int n = (int)b->plans.size();
+  const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
+  std::vector<RagInst<T>> R(n);
+  b->bound.assign(n, gw_batch::Bound{});
+  long w13 = 0, w2 = 0;
+  size_t lds13 = 0, lds2 = 0, off = 0;
+  for (int i = 0; i < n; ++i) {
+    Plan& P = b->plans[i]->p;
+    GW_REQUIRE(nb[i] >= 0, GW_ERR_ARG, "instance %d: negative band count", i);
+    if (nb[i] == 0) {  // instance sits this call out: an empty work range
+      R[i] = RagInst<T>{};
+      R[i].g.na = 1;
+      R[i].nchunk = 1;
+      R[i].w13 = w13;
+      R[i].w2 = w2;
+      b->bound[i] = gw_batch::Bound{nullptr, 0, 0, nullptr, 0, 0, 0, nullptr};
+      continue;
+    }
+    GW_REQUIRE(c[i] && out[i], GW_ERR_ARG, "instance %d: bad block", i);
+    GW_REQUIRE(ldc[i] >= P.npw_max && ldo[i] >= P.npw_max, GW_ERR_ARG, "instance %d: row stride < npw_max", i);
+    GW_REQUIRE(P.have_v, GW_ERR_ARG, "instance %d: no potential set", i);
+    GW_REQUIRE(plane_fits_lds(&P, prec), GW_ERR_UNSUPPORTED, "instance %d: x-plane exceeds LDS", i);
+    const ApplyLayout L = apply_layout(&P, nb[i], prec);
+    RagInst<T>& d = R[i];
+    d.g = geom_of(&P, nb[i]);
+    d.c = (const cpx<T>*)c[i];
+    d.ldc = ldc[i];
+    d.sc = sc[i];
+    d.out = (cpx<T>*)out[i];
+    d.ldo = ldo[i];
+    d.so = so[i];
+    d.t = prec == GW_FP64 ? (const T*)P.t64 : (const T*)P.t32;
+    d.vp = prec == GW_FP64 ? (const T*)P.vp64 : (const T*)P.vp32;
+    d.twx = tw_of<T>(&P, 0);
+    d.twy = tw_of<T>(&P, 1);
+    d.twz = tw_of<T>(&P, 2);
+    d.box = (cpx<T>*)(ws + off + L.box_off);
+    d.scale = T(1.0 / ((double)P.n[0] * P.n[1] * P.n[2]));
+    d.XL = x_lines(&P);
+    d.nbt = P.nk * nb[i];
+    d.nchunk = (P.n[1] * P.n[2] + d.XL - 1) / d.XL;
+    d.kin = (terms & GW_TERM_KINETIC) ? 1 : 0;
+    d.w13 = w13;
+    d.w2 = w2;
+    w13 += (long)P.na * d.nbt;
+    w2 += (long)d.nchunk * ((d.nbt + kXBandGroup - 1) / kXBandGroup);
+    lds13 = std::max(lds13, (size_t)P.n[1] * P.n[2] * sizeof(cpx<T>));
+    lds2 = std::max(lds2, (size_t)P.n[0] * d.XL * sizeof(cpx<T>));
+    b->bound[i] = gw_batch::Bound{c[i], ldc[i], sc[i], out[i], ldo[i], so[i], nb[i], ws + off};
+    off += align256(L.total);
+  }
+  GW_CHECK_HIP(hipMemcpy(b->table, R.data(), (size_t)n * sizeof(RagInst<T>), hipMemcpyHostToDevice));
+  b->prec = prec;
+  b->terms = terms;
+  b->tot13 = w13;
+  b->tot2 = w2;
+  b->lds13 = lds13;
+  b->lds2 = lds2;
+  return GW_OK;
+}
+
+template <typename T>
+static int batch_apply_t(gw_batch* b, hipStream_t st) {
+  const int n = (int)b->plans.size();
+  const RagInst<T>* R = (const RagInst<T>*)b->table;
+  if (b->tot13 == 0) return GW_OK;
+  hipLaunchKernelGGL(k_plane_inv_rag<T>, dim3((unsigned)(8 * ((b->tot13 + 7) / 8))), dim3(kPlaneThreads), b->lds13,
+                     st, R, n, b->tot13);
+  hipLaunchKernelGGL(k_xpass_rag<T>, dim3((unsigned)(8 * ((b->tot2 + 7) / 8))), dim3(kXThreads), b->lds2, st, R, n,
+                     b->tot2);
+  hipLaunchKernelGGL(k_plane_fwd_rag<T>, dim3((unsigned)(8 * ((b->tot13 + 7) / 8))), dim3(kPlaneThreads), b->lds13,
+                     st, R, n, b->tot13);
+  GW_CHECK_HIP(hipGetLastError());
+  if (!(b->terms & GW_TERM_NONLOCAL)) return GW_OK;
+  const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
+  for (int i = 0; i < n; ++i) {
+    Plan& P = b->plans[i]->p;
+    const gw_batch::Bound& B = b->bound[i];
+    if (!P.nproj || B.nb == 0) continue;
+    const ApplyLayout L = apply_layout(&P, B.nb, prec);
+    const cpx<T>* pr = prec == GW_FP64 ? (const cpx<T>*)P.p64 : (const cpx<T>*)P.p32;
+    cpx<T>* becp = (cpx<T>*)(B.ws + L.becp_off);
+    cpx<T>* u = (cpx<T>*)(B.ws + L.u_off);
+    const long pstride = (long)P.nproj * P.npw_max;
+    int rc = gemm_inner<T>(P.nk, P.nproj, B.nb, P.npw_max, pr, P.npw_max, pstride, (const cpx<T>*)B.c, B.ldc, B.sc,
+                           becp, B.nb, (long)P.nproj * B.nb, B.ws + L.part_off, L.part_bytes, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dmul<T>, dim3(grid_blocks((long)P.nproj * B.nb, 256), P.nk), dim3(256), 0, st, P.nproj,
+                       B.nb, P.d_colptr, P.d_row, P.d_val, becp, u);
+    rc = gemm_combine<T>(P.nk, B.nb, P.npw_max, P.nproj, u, B.nb, (long)P.nproj * B.nb, 0, pr, P.npw_max, pstride,
+                         (cpx<T>*)B.out, B.ldo, B.so, T(1), st);
+    if (rc) return rc;
+  }
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
+extern "C" int gw_batch_bind(gw_batch* b, int precision, int terms, const void* const* c, const long* ldc,
+                             const long* sc, void* const* out, const long* ldo, const long* so, const int* nb,
+                             void* ws, size_t ws_bytes) {
+  GW_REQUIRE(b && c && ldc && sc && out && ldo && so && nb && ws, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(precision == GW_FP64 || precision == GW_FP32, GW_ERR_ARG, "bad precision");
+  GW_REQUIRE(terms & GW_TERM_LOCAL, GW_ERR_UNSUPPORTED, "batched apply needs the local term");
+  const size_t need = batch_ws_bytes(b, nb, precision);
+  GW_REQUIRE(ws_bytes >= need, GW_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, need);
+  if (precision == GW_FP64)
+    return batch_bind_t<double>(b, terms, c, ldc, sc, out, ldo, so, nb, (unsigned char*)ws);
+  return batch_bind_t<float>(b, terms, c, ldc, sc, out, ldo, so, nb, (unsigned char*)ws);
+}
+
+extern "C" int gw_batch_apply_bound(gw_batch* b, void* stream) {
+  GW_REQUIRE(b && b->prec >= 0, GW_ERR_ARG, "batch not bound");
+  return b->prec == GW_FP64 ? batch_apply_t<double>(b, (hipStream_t)stream)
+                            : batch_apply_t<float>(b, (hipStream_t)stream);
+}
+
+extern "C" int gw_batch_apply(gw_batch* b, int precision, int terms, const void* const* c, const long* ldc,
+                              const long* sc, void* const* out, const long* ldo, const long* so, const int* nb,
+                              void* ws, size_t ws_bytes, void* stream) {
+  const int rc = gw_batch_bind(b, precision, terms, c, ldc, sc, out, ldo, so, nb, ws, ws_bytes);
+  if (rc) return rc;
+  return gw_batch_apply_bound(b, stream);
 }

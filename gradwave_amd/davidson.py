@@ -180,17 +180,40 @@ class _Basis:
         self.n = n1
 
 
-def _certify_fp64(h_apply, x):
-    """davidson.py:287-308."""
-    hx = h_apply(x)
+def _certify_steps(x, counts):
+    """davidson.py:287-308: re-measure the block with the FP64 operator (a step
+    generator: yields ("apply", z) / ("eigh", s) requests)."""
+    counts["full"] += x.shape[0] * x.shape[1]
+    hx = yield ("apply", x)
     s = _hermitize(LA.inner(x, hx))
-    w, u = _eigh(s)
+    w, u = yield ("eigh", s)
     eig = w.real.to(x.real.dtype)
     u = u.to(x.dtype).contiguous()
     x = LA.combine(u, x)
     hx = LA.combine(u, hx)
     r, rn = LA.residual(x, hx, eig)
     return eig, x, hx, r, rn
+
+
+def _apply_steps(z, low, counts):
+    if not low:
+        counts["full"] += z.shape[0] * z.shape[1]
+        return (yield ("apply", z))
+    counts["low"] += z.shape[0] * z.shape[1]
+    hz = yield ("apply", z.to(torch.complex64))
+    return hz.to(z.dtype)
+
+
+def drive(steps, h_apply):
+    """Run a Davidson step generator to completion, serving its apply requests
+    with ``h_apply`` and its subspace eigensolves with ``_eigh``."""
+    try:
+        req = next(steps)
+        while True:
+            kind, arg = req
+            req = steps.send(h_apply(arg) if kind == "apply" else _eigh(arg))
+    except StopIteration as stop:
+        return stop.value
 
 
 @torch.no_grad()
@@ -205,6 +228,18 @@ def davidson_batched(h_apply: Callable[[torch.Tensor], torch.Tensor], x0: torch.
     always used (the reference measured the delayed gate as a loss,
     davidson.py:545-554). ``profile_out`` (not in the reference) collects wall
     seconds per phase."""
+    return drive(davidson_steps(x0, t, mask, tol=tol, max_iter=max_iter, max_dim_factor=max_dim_factor,
+                                history_out=history_out, profile_out=profile_out), h_apply)
+
+
+@torch.no_grad()
+def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: float = 1e-9,
+                   max_iter: int = 40, max_dim_factor: int = 4, history_out: list | None = None,
+                   profile_out: dict | None = None):
+    """The solver of ``davidson_batched`` as a step generator: it yields
+    ("apply", block) and ("eigh", subspace matrix) requests and returns the
+    BatchedDavidsonResult, so a driver can serve many independent solves in
+    lockstep with one batched apply and one batched eigensolve per round."""
     nk, nb, m = x0.shape
     tm = PhaseTimer(profile_out)
     device = x0.device
@@ -213,22 +248,13 @@ def davidson_batched(h_apply: Callable[[torch.Tensor], torch.Tensor], x0: torch.
     use_fp32_exp = _fp32_expansion_active(x0)
     counts = {"low": 0, "full": 0}
 
-    def _apply_full(z):
-        counts["full"] += z.shape[0] * z.shape[1]
-        return h_apply(z)
-
-    def _apply_exp(z):
-        if not use_fp32_exp:
-            return _apply_full(z)
-        counts["low"] += z.shape[0] * z.shape[1]
-        return h_apply(z.to(torch.complex64)).to(z.dtype)
-
     cert_floor = _FP32_CERT_FLOOR_EPS * max(float(t.max()), 1.0) if use_fp32_exp else 0.0
     prev_rnmax = float("inf")
     basis = _Basis(nk, max(max_dim, nb), m, x0.dtype, device)
 
     v0 = _orthonormalize_b(x0.contiguous(), mask)
-    basis.set(v0, _apply_exp(v0))
+    hv0 = yield from _apply_steps(v0, use_fp32_exp, counts)
+    basis.set(v0, hv0)
     eig = torch.zeros(nk, nb, dtype=rdtype, device=device)
     x = v0[:, :nb]
     rn = torch.full((nk, nb), float("inf"), dtype=torch.float64, device=device)
@@ -236,7 +262,7 @@ def davidson_batched(h_apply: Callable[[torch.Tensor], torch.Tensor], x0: torch.
     for it in range(1, max_iter + 1):
         v, hv = basis.views()
         tm("eigh")
-        w, u = _eigh(basis.subspace())
+        w, u = yield ("eigh", basis.subspace())
         u = u[:, :, :nb].to(v.dtype).contiguous()
         eig = w[:, :nb].real.to(rdtype)
         tm("ritz_gemm")
@@ -249,7 +275,7 @@ def davidson_batched(h_apply: Callable[[torch.Tensor], torch.Tensor], x0: torch.
             stalled = it > 1 and rnmax > _FP32_CERT_STALL * prev_rnmax
             prev_rnmax = rnmax
             if rnmax < max(tol, cert_floor) or stalled:
-                eig, x, hx, r, rn = _certify_fp64(_apply_full, x)
+                eig, x, hx, r, rn = yield from _certify_steps(x, counts)
                 if float(rn.max()) >= tol:
                     basis.set(x, hx)
                     use_fp32_exp = False
@@ -281,12 +307,12 @@ def davidson_batched(h_apply: Callable[[torch.Tensor], torch.Tensor], x0: torch.
         tm("orthonormalize")
         d = _orthonormalize_b(d, mask, against=v)
         tm("h_apply")
-        hd = _apply_exp(d)
+        hd = yield from _apply_steps(d, use_fp32_exp, counts)
         tm("subspace_gemm")
         basis.append(d, hd)
 
     if use_fp32_exp:
-        eig, x, hx, r, rn = _certify_fp64(_apply_full, x)
+        eig, x, hx, r, rn = yield from _certify_steps(x, counts)
     tm(None)
     return BatchedDavidsonResult(eig, x, max_iter, rn.to(rdtype), counts["low"], counts["full"])
 

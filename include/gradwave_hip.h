@@ -182,6 +182,28 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
                    const double* tables, const double* g_cart, double volume, const void* grad_vloc_g,
                    double* grad_tau, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- ragged batches of independent operators ----
+ * No reference counterpart: the reference applies one operator per call
+ * (core/batch.py:480-505); a batch of independent cells (different boxes,
+ * spheres, potentials, projector and band counts) runs here as ONE launch per
+ * local-term pass over all instances' work items, then each instance's
+ * projector GEMMs. Plans must take the generic-size plane kernels
+ * (x-plane fits LDS). bind uploads the per-instance block pointers, strides and
+ * band counts (nb[i] = 0 leaves instance i out of the call) and is synchronous;
+ * apply_bound only launches (so it can be captured in a HIP graph); apply = both.
+ * Terms must include GW_TERM_LOCAL. Workspace: gw_batch_workspace_bytes. */
+typedef struct gw_batch gw_batch;
+int gw_batch_create(gw_batch** out, gw_plan* const* plans, int n);
+int gw_batch_destroy(gw_batch* batch);
+size_t gw_batch_workspace_bytes(const gw_batch* batch, const int* nb, int precision);
+int gw_batch_bind(gw_batch* batch, int precision, int terms, const void* const* c, const long* ldc,
+                  const long* sc, void* const* out, const long* ldo, const long* so, const int* nb,
+                  void* workspace, size_t workspace_bytes);
+int gw_batch_apply_bound(gw_batch* batch, void* stream);
+int gw_batch_apply(gw_batch* batch, int precision, int terms, const void* const* c, const long* ldc,
+                   const long* sc, void* const* out, const long* ldo, const long* so, const int* nb,
+                   void* workspace, size_t workspace_bytes, void* stream);
+
 /* ---- small-cell local term as a matrix (core/batch.py:333-464) ----
  * M[k][i][j] = vhat[(G_i - G_j) wrapped into the box] for i, j < npw[k], else 0;
  * vhat (n1 n2 n3) complex = fftn(v_eff) / N; npw (nk) and flat_idx (nk, npw_max)
