@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libgradwave_hip.so")
 GW_FP64, GW_FP32 = 0, 1
 GW_TERM_KINETIC, GW_TERM_LOCAL, GW_TERM_NONLOCAL, GW_TERM_ALL = 1, 2, 4, 7
 GW_PLAN_FORCE_UNFUSED, GW_PLAN_TIMING, GW_PLAN_LEGACY = 1, 2, 4
+GW_TUNE_RADIX, GW_TUNE_GEMM_WGS = 1, 2
 STAGES = ("plane_inv", "xpass", "plane_fwd", "becp", "dmul", "backproj")
 
 _P = C.c_void_p
@@ -61,6 +62,8 @@ SIGNATURES = {
     "gw_rgemm_inner_workspace_bytes": (_Z, [_I, _I, _I, _I, _I]),
     "gw_rgemm_inner": (_I, [_I, _I, _I, _I, _I, _P, _L, _L, _P, _L, _L, _P, _L, _L, _P, _Z, _P]),
     "gw_rgemm_combine": (_I, [_I, _I, _I, _I, _I, _P, _L, _L, _P, _L, _L, _P, _L, _L, _D, _D, _P]),
+    "gw_tune_set": (_I, [_I, _I, _I]),
+    "gw_tune_get": (_I, [_I, _I]),
     "gw_batch_create": (_I, [C.POINTER(_P), _P, _I]),
     "gw_batch_destroy": (_I, [_P]),
     "gw_batch_workspace_bytes": (_Z, [_P, _P, _I]),
@@ -91,6 +94,9 @@ def lib() -> C.CDLL:
             f.restype = res
             f.argtypes = args
         _lib = h
+        from gradwave_amd import autotune  # saved radix / split-K choices, if any
+
+        autotune.load()
     return _lib
 
 

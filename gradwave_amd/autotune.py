@@ -1,0 +1,134 @@
+"""Autotuner for the FFT radix splits and the split-K width of the MFMA inner
+products (no reference counterpart: the reference uses torch.fft and BLAS).
+
+* Radix: every box length n is transformed as two codelet passes n = a * b
+  (a, b in {1..10, 12, 14, 15, 16}); plans take the most balanced split unless
+  ``gw_tune_set(GW_TUNE_RADIX, n, a)`` says otherwise. ``tune_radix`` times
+  the generic-size local-term kernels of real operators for each candidate a
+  and keeps the fastest.
+* Split-K: ``gw_zgemm_inner`` / ``gw_rgemm_inner`` split the long axis until
+  the launch has about ``GW_TUNE_GEMM_WGS`` workgroups; ``tune_gemm`` times
+  the projector products of real operators for a few targets.
+
+Choices persist as JSON (``GW_TUNE_FILE``, default
+``~/.cache/gradwave_amd/tune.json``) and are loaded with the library.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import statistics
+import time
+
+import torch
+
+from gradwave_amd import _lib as L
+
+CODELETS = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16}
+GEMM_TARGETS = (512, 1024, 2048, 4096, 8192)
+
+
+def tune_file() -> str:
+    return os.environ.get("GW_TUNE_FILE", os.path.join(os.path.expanduser("~"), ".cache", "gradwave_amd",
+                                                       "tune.json"))
+
+
+def candidate_splits(n: int) -> list[int]:
+    return [a for a in range(1, 17) if n % a == 0 and a in CODELETS and n // a in CODELETS]
+
+
+def current() -> dict:
+    lib = L.lib()
+    radix = {n: lib.gw_tune_get(L.GW_TUNE_RADIX, n) for n in range(1, 1024)}
+    return {"radix": {str(n): a for n, a in radix.items() if a > 0},
+            "gemm_wgs": lib.gw_tune_get(L.GW_TUNE_GEMM_WGS, 0)}
+
+
+def apply(choices: dict) -> None:
+    lib = L.lib()
+    for n, a in choices.get("radix", {}).items():
+        L.check(lib.gw_tune_set(L.GW_TUNE_RADIX, int(n), int(a)), "gw_tune_set")
+    if "gemm_wgs" in choices:
+        L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, int(choices["gemm_wgs"])), "gw_tune_set")
+
+
+def load(path: str | None = None) -> dict | None:
+    path = path or tune_file()
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        choices = json.load(f)
+    apply(choices)
+    return choices
+
+
+def save(path: str | None = None) -> str:
+    path = path or tune_file()
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(current(), f, indent=1, sort_keys=True)
+    return path
+
+
+def _median_ms(fn, reps: int = 5) -> float:
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts) * 1e3
+
+
+def tune_radix(workloads, nb: int = 8, reps: int = 5) -> dict[int, int]:
+    """For every box length of ``workloads`` (objects with bk, shape, v_r, p),
+    time the generic-size local term of the operators that use it under each
+    candidate split and keep the fastest (lengths tuned one at a time)."""
+    from gradwave_amd.hamiltonian import BatchedHamiltonian
+
+    lib = L.lib()
+    lengths = sorted({n for w in workloads for n in w.shape})
+    chosen = {}
+    for n in lengths:
+        cands = candidate_splits(n)
+        users = [w for w in workloads if n in w.shape]
+        if len(cands) < 2 or not users:
+            continue
+        g = torch.Generator().manual_seed(n)
+        blocks = [(torch.randn(w.bk.nk, nb, w.bk.npw_max, dtype=torch.complex128, generator=g).to(w.v_r.device)
+                   * w.bk.mask[:, None, :]).to(torch.complex64) for w in users]
+        best, best_ms = None, float("inf")
+        for a in cands:
+            L.check(lib.gw_tune_set(L.GW_TUNE_RADIX, n, a), "gw_tune_set")
+            hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in users]
+            for h in hams:
+                h.set_flags(L.GW_PLAN_LEGACY)
+            ms = _median_ms(lambda: [h.apply_terms(x, L.GW_TERM_LOCAL) for h, x in zip(hams, blocks)], reps)
+            if ms < best_ms:
+                best, best_ms = a, ms
+        L.check(lib.gw_tune_set(L.GW_TUNE_RADIX, n, best), "gw_tune_set")
+        chosen[n] = best
+    return chosen
+
+
+def tune_gemm(workloads, nb: int = 32, reps: int = 5) -> int:
+    """Time the projector products (nonlocal term) of ``workloads`` for each
+    split-K workgroup target and keep the fastest."""
+    from gradwave_amd.hamiltonian import BatchedHamiltonian
+
+    lib = L.lib()
+    hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in workloads]
+    g = torch.Generator().manual_seed(7)
+    blocks = [(torch.randn(w.bk.nk, nb, w.bk.npw_max, dtype=torch.complex128, generator=g).to(w.v_r.device)
+               * w.bk.mask[:, None, :]).to(torch.complex64) for w in workloads]
+    best, best_ms = None, float("inf")
+    for target in GEMM_TARGETS:
+        L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, target), "gw_tune_set")
+        ms = _median_ms(lambda: [h.apply_terms(x, L.GW_TERM_NONLOCAL) for h, x in zip(hams, blocks)], reps)
+        if ms < best_ms:
+            best, best_ms = target, ms
+    L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, best), "gw_tune_set")
+    return best

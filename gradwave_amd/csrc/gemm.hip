@@ -374,9 +374,13 @@ static bool gemm_simple() {
   return v;
 }
 
+static int g_target_wgs = 2048;  // split-K aims at this many workgroups (autotuned)
+void set_gemm_target_wgs(int v) { g_target_wgs = v > 0 ? v : 2048; }
+int gemm_target_wgs() { return g_target_wgs; }
+
 static int inner_splits(int nk, int M, int N, int K) {
   const long tiles = (long)((2 * N + kTile - 1) / kTile) * ((M + kTile - 1) / kTile) * nk;
-  long s = (2048 + tiles - 1) / tiles;
+  long s = (g_target_wgs + tiles - 1) / tiles;
   const long smax = std::max<long>(1, (K + 4 * kStepC - 1) / (4 * kStepC));  // >= 4 groups per split
   s = std::max<long>(1, std::min<long>(s, smax));
   return (int)std::min<long>(s, 4096);
@@ -749,7 +753,7 @@ k_rgemm_combine_lds(int A, int G, int J, const T* __restrict__ U, long ldu, long
 
 int rinner_splits(int nk, int M, int N, int K) {
   const long tiles = (long)((N + 63) / 64) * ((M + 63) / 64) * nk;
-  long s = (2048 + tiles - 1) / tiles;
+  long s = (g_target_wgs + tiles - 1) / tiles;
   const long smax = std::max<long>(1, (K + 4 * kKR - 1) / (4 * kKR));  // >= 4 chunks per split
   s = std::max<long>(1, std::min<long>(s, smax));
   return (int)std::min<long>(s, 4096);

@@ -40,7 +40,18 @@ void set_error(const char* fmt, ...) {
   va_end(ap);
 }
 
+// Preferred first factor per box length, set by the autotuner (0: none).
+static int g_radix_pref[1024] = {0};
+
 bool split_length(int n, int* a, int* b) {
+  if (n > 0 && n < 1024 && g_radix_pref[n] > 0) {
+    const int x = g_radix_pref[n];
+    if (n % x == 0 && codelet_ok(x) && codelet_ok(n / x)) {
+      *a = x;
+      *b = n / x;
+      return true;
+    }
+  }
   // most balanced split with both factors in the codelet set
   int best_a = -1, best_b = -1;
   for (int x = 1; x <= 16; ++x) {
@@ -1029,6 +1040,29 @@ struct gw_plan {
 static int grid_blocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 extern "C" const char* gw_last_error(void) { return gw::g_err; }
+
+// Tuning knobs written by the autotuner (gradwave_amd/autotune.py).
+extern "C" int gw_tune_set(int key, int arg, int value) {
+  if (key == GW_TUNE_RADIX) {
+    GW_REQUIRE(arg > 0 && arg < 1024, GW_ERR_ARG, "box length %d out of range", arg);
+    GW_REQUIRE(value == 0 || (arg % value == 0 && codelet_ok(value) && codelet_ok(arg / value)), GW_ERR_ARG,
+               "%d does not split %d into two codelets", value, arg);
+    g_radix_pref[arg] = value;
+    return GW_OK;
+  }
+  if (key == GW_TUNE_GEMM_WGS) {
+    GW_REQUIRE(value >= 0 && value <= (1 << 20), GW_ERR_ARG, "bad workgroup target %d", value);
+    set_gemm_target_wgs(value);
+    return GW_OK;
+  }
+  GW_REQUIRE(false, GW_ERR_ARG, "unknown tuning key %d", key);
+}
+
+extern "C" int gw_tune_get(int key, int arg) {
+  if (key == GW_TUNE_RADIX) return (arg > 0 && arg < 1024) ? g_radix_pref[arg] : -1;
+  if (key == GW_TUNE_GEMM_WGS) return gemm_target_wgs();
+  return -1;
+}
 extern "C" int gw_version(void) { return 1; }
 
 extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int npw_max,

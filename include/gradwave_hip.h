@@ -182,6 +182,15 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
                    const double* tables, const double* g_cart, double volume, const void* grad_vloc_g,
                    double* grad_tau, void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---- tuning knobs (no reference counterpart; written by gradwave_amd/autotune.py) ----
+ * GW_TUNE_RADIX: for box length `arg`, the first factor `value` of the two-codelet
+ *   split used by plans created afterwards (0 = the default most balanced split).
+ * GW_TUNE_GEMM_WGS: the number of workgroups the split-K inner products aim at
+ *   (`arg` ignored; 0 = default 2048). */
+enum { GW_TUNE_RADIX = 1, GW_TUNE_GEMM_WGS = 2 };
+int gw_tune_set(int key, int arg, int value);
+int gw_tune_get(int key, int arg);
+
 /* ---- ragged batches of independent operators ----
  * No reference counterpart: the reference applies one operator per call
  * (core/batch.py:480-505); a batch of independent cells (different boxes,

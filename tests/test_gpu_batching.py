@@ -117,3 +117,36 @@ def test_lockstep_davidson_matches_independent_solves():
     for r, m in zip(res, mixed):
         assert float(m.residual_norms.max()) < tol
         assert float((m.eigenvalues - r.eigenvalues).abs().max()) / HARTREE_EV < 1e-10
+
+
+def test_autotuner_choices_are_valid_and_exact(tmp_path, monkeypatch):
+    from gradwave_amd import _lib as L
+    from gradwave_amd import autotune
+    from gradwave_amd import batching as B
+
+    say("autotune radix + split-K on 4 cells")
+    wls = _instances()[:4]
+    radix = autotune.tune_radix(wls, reps=2)
+    wgs = autotune.tune_gemm(wls, reps=2)
+    assert wgs in autotune.GEMM_TARGETS
+    for n, a in radix.items():
+        assert a in autotune.candidate_splits(n)
+    hams = _hams(wls)
+    for h, w in zip(hams, wls):
+        info = h.info(torch.complex64)
+        for ax, n in enumerate(w.shape):
+            if n in radix:
+                assert info[("split_x", "split_y", "split_z")[ax]][0] == radix[n]
+    cs = _blocks(wls, torch.complex128, seed=9)
+    outs = B.RaggedBatch(hams).apply(cs)
+    for h, c, o in zip(hams, cs, outs):
+        assert _rel(o, h.apply_terms(c)) < 1e-12
+    path = str(tmp_path / "tune.json")
+    monkeypatch.setenv("GW_TUNE_FILE", path)
+    autotune.save()
+    L.lib().gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, 0)
+    assert autotune.load()["gemm_wgs"] == wgs
+    assert L.lib().gw_tune_get(L.GW_TUNE_GEMM_WGS, 0) == wgs
+    for n in radix:  # leave the default splits for later tests
+        L.lib().gw_tune_set(L.GW_TUNE_RADIX, n, 0)
+    L.lib().gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, 0)
