@@ -12,9 +12,9 @@ between tiny launches. Here:
   batched apply in a HIP graph and ``replay`` re-launches it.
 * ``davidson_ragged`` runs one ``davidson_steps`` generator per instance (the
   same solver as ``davidson_batched``) in lockstep, decode-style: each round it
-  gathers every pending apply request into one ragged apply and every pending
-  subspace eigensolve into one batched eigensolve, so the per-solve results are
-  those of ``davidson_batched`` on each instance alone.
+  gathers every pending apply request into one ragged apply and the pending
+  subspace eigensolves into one batched eigensolve per matrix size, so the
+  per-solve results are those of ``davidson_batched`` on each instance alone.
 """
 
 from __future__ import annotations
@@ -134,31 +134,21 @@ class RaggedBatch:
 
 
 def _eigh_batched(mats: list[torch.Tensor]) -> list[tuple[torch.Tensor, torch.Tensor]]:
-    """Eigenpairs of many Hermitian subspace matrices (each (nk, m_i, m_i)) in one
-    batched FP64 eigensolve: every matrix is embedded in the top-left corner of an
-    (m_max, m_max) one whose remaining diagonal is a value above its spectrum, so
-    its own eigenpairs are the lowest m_i, with eigenvectors zero below m_i."""
-    m_max = max(int(s.shape[-1]) for s in mats)
-    total = sum(int(s.shape[0]) for s in mats)
-    dev = mats[0].device
-    big = torch.empty(total, m_max, m_max, dtype=torch.complex128, device=dev)
-    big.zero_()
-    row = 0
-    for s in mats:
-        nk, m = int(s.shape[0]), int(s.shape[-1])
-        blk = big[row:row + nk]
-        blk[:, :m, :m] = s.to(torch.complex128)
-        if m < m_max:
-            shift = 2.0 * float(torch.linalg.matrix_norm(s.to(torch.complex128), ord="fro").max()) + 1.0
-            idx = torch.arange(m, m_max, device=dev)
-            blk[:, idx, idx] = shift
-        row += nk
-    w, u = torch.linalg.eigh(big)
-    out, row = [], 0
-    for s in mats:
-        nk, m = int(s.shape[0]), int(s.shape[-1])
-        out.append((w[row:row + nk, :m].contiguous(), u[row:row + nk, :m, :m].contiguous()))
-        row += nk
+    """Eigenpairs of many Hermitian subspace matrices (each (nk, m_i, m_i)) with
+    one batched FP64 eigensolve per distinct size m_i (instances of one kind
+    move through the same sizes, so a round needs a handful of calls)."""
+    groups: dict[int, list[int]] = {}
+    for j, s in enumerate(mats):
+        groups.setdefault(int(s.shape[-1]), []).append(j)
+    out: list = [None] * len(mats)
+    for idx in groups.values():
+        stack = torch.cat([mats[j].to(torch.complex128) for j in idx])
+        w, u = torch.linalg.eigh(stack)
+        row = 0
+        for j in idx:
+            nk = int(mats[j].shape[0])
+            out[j] = (w[row:row + nk], u[row:row + nk])
+            row += nk
     return out
 
 
@@ -169,12 +159,24 @@ class RaggedSolveStats:
     batched_eighs: int = 0
 
 
+def _check_finite(kind: str, i: int, req, resp) -> None:
+    tensors = resp if isinstance(resp, tuple) else (resp,)
+    for t in tensors:
+        if not bool(torch.isfinite(t).all()):
+            src = req if isinstance(req, torch.Tensor) else None
+            raise FloatingPointError(f"non-finite {kind} result for instance {i}: request finite="
+                                     f"{None if src is None else bool(torch.isfinite(src).all())}, "
+                                     f"shape {tuple(src.shape) if src is not None else None}")
+
+
 def davidson_ragged(batch: RaggedBatch, x0s: list[torch.Tensor], ts: list[torch.Tensor],
                     masks: list[torch.Tensor], tol: float = 1e-9, max_iter: int = 40,
-                    max_dim_factor: int = 4, stats: RaggedSolveStats | None = None) -> list[BatchedDavidsonResult]:
+                    max_dim_factor: int = 4, stats: RaggedSolveStats | None = None,
+                    check_finite: bool = False) -> list[BatchedDavidsonResult]:
     """One ``davidson_steps`` solve per instance, advanced in lockstep: each round
     serves all pending apply requests with one ragged apply (one per precision)
-    and all pending subspace eigensolves with one batched eigensolve."""
+    and the pending subspace eigensolves with one batched eigensolve per size.
+    ``check_finite`` raises on the first non-finite result (debugging aid)."""
     gens = [davidson_steps(x0, t, m, tol=tol, max_iter=max_iter, max_dim_factor=max_dim_factor)
             for x0, t, m in zip(x0s, ts, masks)]
     results: list = [None] * len(gens)
@@ -209,6 +211,9 @@ def davidson_ragged(batch: RaggedBatch, x0s: list[torch.Tensor], ts: list[torch.
             for i, wu in zip(keys, _eigh_batched([eighs[i] for i in keys])):
                 resp[i] = wu
             st.batched_eighs += 1
+        if check_finite:
+            for i, r in resp.items():
+                _check_finite(pending[i][0], i, pending[i][1], r)
         for i, r in resp.items():
             advance(i, r)
     return results

@@ -124,7 +124,10 @@ extern "C" int gw_residual(int precision, int nk, int nb, int npw, const void* x
 extern "C" int gw_teter_expand(int precision, int nk, int nb, int nsel, int npw, const void* x, const void* r,
                                long ld, long s, const double* t, long ldt, const int64_t* sel, void* d, long ldd,
                                long sd, void* stream) {
-  GW_REQUIRE(x && r && t && sel && d && ld >= npw && ldd >= npw && ldt >= npw, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(x && r && t && sel && d, GW_ERR_ARG, "teter_expand: null pointer (x %p r %p t %p sel %p d %p)", x, r,
+             t, sel, d);
+  GW_REQUIRE(ld >= npw && ldd >= npw && ldt >= npw, GW_ERR_ARG,
+             "teter_expand: row stride below npw (ld %ld ldd %ld ldt %ld npw %d)", ld, ldd, ldt, npw);
   if (!nk || !nsel) return GW_OK;
   hipStream_t st = (hipStream_t)stream;
   if (precision == GW_FP64)

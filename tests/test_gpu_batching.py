@@ -105,7 +105,7 @@ def test_lockstep_davidson_matches_independent_solves():
     x0s = [start_vectors(w, seed=i) for i, w in enumerate(wls)]
     stats = B.RaggedSolveStats()
     res = B.davidson_ragged(batch, x0s, [w.bk.t for w in wls], [w.bk.mask for w in wls], tol=tol, max_iter=200,
-                            stats=stats)
+                            stats=stats, check_finite=True)
     say(f"rounds {stats.rounds} applies {stats.batched_applies} eighs {stats.batched_eighs}")
     for i, (w, h, x0, r) in enumerate(zip(wls, hams, x0s, res)):
         ref = davidson_batched(h.apply, x0, w.bk.t, w.bk.mask, tol=tol, max_iter=200)
