@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--davidson-max-iter", type=int, default=300)
     ap.add_argument("--no-forces", action="store_true")
     ap.add_argument("--no-gamma", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile and split-K width")
+    ap.add_argument("--only", choices=["gamma", "forces"], default=None,
+                    help="profiling runs: time just the Gamma-layout apply or just the adjoint forces")
     ap.add_argument("--profile-davidson", action="store_true")
     ap.add_argument("--tol-ha", type=float, default=1e-10)
     ap.add_argument("--draft-tol", type=float, default=1e-3, help="mixed draft tolerance [eV]")
@@ -77,6 +80,18 @@ def main() -> None:
           file=sys.stderr, flush=True)
     x64 = W.start_vectors(wl, seed=0)
     x32 = x64.to(torch.complex64)
+    tuned = None
+    if not args.no_autotune:
+        from gradwave_amd import autotune
+
+        t1 = time.perf_counter()
+        tile, wgs = autotune.tune_gemm([wl], nb=nb)
+        tuned = {"gemm_tile_variant": tile, "gemm_split_k_target_wgs": wgs, "seconds": time.perf_counter() - t1}
+        print(f"[bench] autotune GEMM: tile {tile}, split-K target {wgs} ({tuned['seconds']:.1f}s)",
+              file=sys.stderr, flush=True)
+    if args.only is not None:
+        profile_only(args, wl, x64)
+        return
 
     def timed_applies(x, steps, warmup):
         out = torch.empty_like(x)
@@ -167,6 +182,7 @@ def main() -> None:
         "fp64_band_applies_per_s": nbt / ms64,
         "fp64_ms_per_step": ms64 * 1e3,
         "roofline": roof,
+        "autotune": tuned,
     }
     if nk * m * m * 16 <= (4 << 30):
         result["toeplitz"] = toeplitz_rates(args, wl, x32, x64)
@@ -215,6 +231,52 @@ def main() -> None:
     print(json.dumps(result), flush=True)
 
 
+def profile_only(args, wl, x64) -> None:
+    """Short runs for rocprofv3: ``--only gamma`` times the Gamma-layout apply
+    (FP32 and FP64 embeddings), ``--only forces`` the adjoint forces on the
+    seeded start block; one JSON line with the medians."""
+    import torch
+
+    from gradwave_amd.adjoint import band_energy_forces
+
+    out = {"workload": wl.name, "only": args.only}
+    if args.only == "gamma":
+        from gradwave_amd import gamma as G
+
+        sph = type("Sphere", (), {})()
+        sph.flat_idx = wl.bk.flat_idx[0]
+        sph.kpg2 = (wl.bk.kpg[0] ** 2).sum(-1)
+        gb = G.build_gamma_basis(sph, wl.shape)
+        gh = G.GammaHamiltonian(gb, wl.v_r, wl.p[0], wl.bk.dij_full)
+        xh = G.full_to_half(gb, x64[0])
+        for name, dt in (("fp32", torch.float32), ("fp64", torch.float64)):
+            phi = G.embed_real(gb, xh).to(dt)
+            ts = []
+            for i in range(args.warmup + args.steps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                gh.apply_real(phi)
+                torch.cuda.synchronize()
+                if i >= args.warmup:
+                    ts.append(time.perf_counter() - t0)
+            out[f"{name}_ms"] = statistics.median(ts) * 1e3
+    else:
+        nocc = int(wl.n_electrons // 2)
+        w = torch.zeros(wl.bk.nk, x64.shape[1], dtype=torch.float64, device=x64.device)
+        w[:, :nocc] = 2.0 * torch.as_tensor(wl.kweights, device=x64.device)[:, None]
+        fargs = (wl.bk, wl.shape, wl.positions, wl.species_index, wl.vloc_tables, wl.g_cart, wl.volume)
+        ts = []
+        for i in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            band_energy_forces(*fargs, x64, w)
+            torch.cuda.synchronize()
+            if i >= args.warmup:
+                ts.append(time.perf_counter() - t0)
+        out["forces_ms"] = statistics.median(ts) * 1e3
+    print(json.dumps(out), flush=True)
+
+
 def bench_ragged(args) -> None:
     """Batching form: 32 independent cells (Al1, Si2, Si8, Si16 at seeded random
     k, 25-40 Ry, 8-64 bands) applied as one ragged batch and solved by lockstep
@@ -238,11 +300,12 @@ def bench_ragged(args) -> None:
 
     t0 = time.perf_counter()
     radix = autotune.tune_radix(wls)
-    gemm_wgs = autotune.tune_gemm(wls)
+    gemm_tile, gemm_wgs = autotune.tune_gemm(wls)
     t_tune = time.perf_counter() - t0
     hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in wls]  # plans with the tuned splits
     batch = B.RaggedBatch(hams)
-    print(f"[bench] autotune {t_tune:.1f}s: radix {radix}, split-K target {gemm_wgs} workgroups",
+    print(f"[bench] autotune {t_tune:.1f}s: radix {radix}, GEMM tile {gemm_tile}, split-K target {gemm_wgs} "
+          "workgroups",
           file=sys.stderr, flush=True)
     x64 = [start_vectors(w, seed=i) for i, w in enumerate(wls)]
     x32 = [x.to(torch.complex64) for x in x64]
@@ -272,11 +335,14 @@ def bench_ragged(args) -> None:
                        "graph_band_applies_per_s": nbt / med_g}
         print(f"[bench] ragged apply {name}: batched {med_b * 1e3:.3f} ms, graph {med_g * 1e3:.3f} ms, "
               f"one-by-one {med_s * 1e3:.3f} ms", file=sys.stderr, flush=True)
+    if args.no_davidson:
+        print(json.dumps({"workload": "ragged32", "apply": rates}), flush=True)
+        return
     tol = args.tol_ha * HARTREE_EV
     ts_, ms_ = [w.bk.t for w in wls], [w.bk.mask for w in wls]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    st = B.RaggedSolveStats()
+    st = B.RaggedSolveStats(seconds={} if args.profile_davidson else None)
     r64 = B.davidson_ragged(batch, x64, ts_, ms_, tol=tol, max_iter=args.davidson_max_iter, stats=st)
     torch.cuda.synchronize()
     t_lock = time.perf_counter() - t0
@@ -306,9 +372,10 @@ def bench_ragged(args) -> None:
                    "instances": len(wls), "band_rows": nbt, "shapes": [list(w.shape) for w in wls]},
         "apply": rates,
         "autotune": {"seconds": t_tune, "radix_first_factor": {str(k): v for k, v in radix.items()},
-                     "gemm_split_k_target_wgs": gemm_wgs},
+                     "gemm_tile_variant": gemm_tile, "gemm_split_k_target_wgs": gemm_wgs},
         "davidson": {"lockstep_fp64_s_per_kpt": t_lock / nkp, "lockstep_mixed_s_per_kpt": t_mixed / nkp,
                      "one_by_one_fp64_s_per_kpt": t_seq / nkp, "rounds": st.rounds,
+                     "lockstep_phase_s": st.seconds,
                      "max_eig_diff_lockstep_vs_one_by_one_Ha": de, "max_eig_diff_mixed_vs_one_by_one_Ha": dm,
                      "max_residual_eV": rn, "tol_Ha": args.tol_ha},
         "roofline": None, "cpu_baseline": None,

@@ -1,14 +1,16 @@
-"""Autotuner for the FFT radix splits and the split-K width of the MFMA inner
-products (no reference counterpart: the reference uses torch.fft and BLAS).
+"""Autotuner for the FFT radix splits, the MFMA tile shape and the split-K width of
+the GEMMs (no reference counterpart: the reference uses torch.fft and BLAS).
 
 * Radix: every box length n is transformed as two codelet passes n = a * b
   (a, b in {1..10, 12, 14, 15, 16}); plans take the most balanced split unless
   ``gw_tune_set(GW_TUNE_RADIX, n, a)`` says otherwise. ``tune_radix`` times
   the generic-size local-term kernels of real operators for each candidate a
   and keeps the fastest.
-* Split-K: ``gw_zgemm_inner`` / ``gw_rgemm_inner`` split the long axis until
-  the launch has about ``GW_TUNE_GEMM_WGS`` workgroups; ``tune_gemm`` times
-  the projector products of real operators for a few targets.
+* GEMM tiles and split-K: the complex MFMA kernels run 2x2, 4x2, 2x4 or 4x4
+  16x16 blocks per wave (``GW_TUNE_GEMM_TILE``), and ``gw_zgemm_inner`` /
+  ``gw_rgemm_inner`` split the long axis until the launch has about
+  ``GW_TUNE_GEMM_WGS`` workgroups; ``tune_gemm`` times the projector products
+  of real operators for each tile shape and target.
 
 Choices persist as JSON (``GW_TUNE_FILE``, default
 ``~/.cache/gradwave_amd/tune.json``) and are loaded with the library.
@@ -42,7 +44,8 @@ def current() -> dict:
     lib = L.lib()
     radix = {n: lib.gw_tune_get(L.GW_TUNE_RADIX, n) for n in range(1, 1024)}
     return {"radix": {str(n): a for n, a in radix.items() if a > 0},
-            "gemm_wgs": lib.gw_tune_get(L.GW_TUNE_GEMM_WGS, 0)}
+            "gemm_wgs": lib.gw_tune_get(L.GW_TUNE_GEMM_WGS, 0),
+            "gemm_tile": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, 0)}
 
 
 def apply(choices: dict) -> None:
@@ -51,6 +54,8 @@ def apply(choices: dict) -> None:
         L.check(lib.gw_tune_set(L.GW_TUNE_RADIX, int(n), int(a)), "gw_tune_set")
     if "gemm_wgs" in choices:
         L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, int(choices["gemm_wgs"])), "gw_tune_set")
+    if "gemm_tile" in choices:
+        L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, int(choices["gemm_tile"])), "gw_tune_set")
 
 
 def load(path: str | None = None) -> dict | None:
@@ -114,21 +119,36 @@ def tune_radix(workloads, nb: int = 8, reps: int = 5) -> dict[int, int]:
     return chosen
 
 
-def tune_gemm(workloads, nb: int = 32, reps: int = 5) -> int:
-    """Time the projector products (nonlocal term) of ``workloads`` for each
-    split-K workgroup target and keep the fastest."""
+GEMM_TILES = (0, 1, 2, 3)  # MFMA blocks per wave: 2x2, 4x2, 2x4, 4x4
+
+
+def tune_gemm(workloads, nb: int = 32, reps: int = 5, dtype=torch.complex64) -> tuple[int, int]:
+    """Time the projector products (nonlocal term) of ``workloads``: first each
+    MFMA tile shape at the current split-K target, then each split-K workgroup
+    target with the best tile; keep the fastest. Returns (tile, target)."""
     from gradwave_amd.hamiltonian import BatchedHamiltonian
 
     lib = L.lib()
     hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in workloads]
     g = torch.Generator().manual_seed(7)
     blocks = [(torch.randn(w.bk.nk, nb, w.bk.npw_max, dtype=torch.complex128, generator=g).to(w.v_r.device)
-               * w.bk.mask[:, None, :]).to(torch.complex64) for w in workloads]
+               * w.bk.mask[:, None, :]).to(dtype) for w in workloads]
+
+    def run():
+        return [h.apply_terms(x, L.GW_TERM_NONLOCAL) for h, x in zip(hams, blocks)]
+
+    best_tile, best_ms = 0, float("inf")
+    for tile in GEMM_TILES:
+        L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, tile), "gw_tune_set")
+        ms = _median_ms(run, reps)
+        if ms < best_ms:
+            best_tile, best_ms = tile, ms
+    L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, best_tile), "gw_tune_set")
     best, best_ms = None, float("inf")
     for target in GEMM_TARGETS:
         L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, target), "gw_tune_set")
-        ms = _median_ms(lambda: [h.apply_terms(x, L.GW_TERM_NONLOCAL) for h, x in zip(hams, blocks)], reps)
+        ms = _median_ms(run, reps)
         if ms < best_ms:
             best, best_ms = target, ms
     L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, best), "gw_tune_set")
-    return best
+    return best_tile, best

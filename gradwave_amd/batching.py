@@ -12,16 +12,21 @@ between tiny launches. Here:
   batched apply in a HIP graph and ``replay`` re-launches it.
 * ``davidson_ragged`` runs one ``davidson_steps`` generator per instance (the
   same solver as ``davidson_batched``) in lockstep, decode-style: each round it
-  gathers every pending apply request into one ragged apply and the pending
-  subspace eigensolves into one batched eigensolve per matrix size, so the
-  per-solve results are those of ``davidson_batched`` on each instance alone.
+  gathers every pending apply request into one ragged apply, the pending
+  subspace eigensolves into a few batched eigensolves (sizes bucketed), and
+  every pending host read (convergence gate, breakdown tests) into one copy,
+  so the per-solve results are those of ``davidson_batched`` on each instance
+  alone.
 """
 
 from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from dataclasses import dataclass
+
+import time
 
 import numpy as np
 import torch
@@ -133,17 +138,17 @@ class RaggedBatch:
 # ---------------------------------------------------------------- lockstep solves
 
 
-def _eigh_batched(mats: list[torch.Tensor]) -> list[tuple[torch.Tensor, torch.Tensor]]:
-    """Eigenpairs of many Hermitian subspace matrices (each (nk, m_i, m_i)) with
-    one batched FP64 eigensolve per distinct size m_i (instances of one kind
-    move through the same sizes, so a round needs a handful of calls)."""
+_EIGH_BUCKET = 32
+_EIGH_HOST_MAX = int(os.environ.get("GW_EIGH_CPU_MAX", "64"))
+
+
+def _eigh_exact_groups(mats: list[torch.Tensor]) -> list:
     groups: dict[int, list[int]] = {}
     for j, s in enumerate(mats):
         groups.setdefault(int(s.shape[-1]), []).append(j)
     out: list = [None] * len(mats)
     for idx in groups.values():
-        stack = torch.cat([mats[j].to(torch.complex128) for j in idx])
-        w, u = torch.linalg.eigh(stack)
+        w, u = torch.linalg.eigh(torch.cat([mats[j].to(torch.complex128) for j in idx]))
         row = 0
         for j in idx:
             nk = int(mats[j].shape[0])
@@ -152,11 +157,53 @@ def _eigh_batched(mats: list[torch.Tensor]) -> list[tuple[torch.Tensor, torch.Te
     return out
 
 
+_EIGH_STREAMS: list = []
+
+
+def _eigh_batched(mats: list[torch.Tensor]) -> list[tuple[torch.Tensor, torch.Tensor]]:
+    """Eigenpairs of many Hermitian subspace matrices (each (nk, m_i, m_i)).
+
+    Matrices up to GW_EIGH_CPU_MAX go to host LAPACK, one batched call per size
+    (the reference offloads small subspace solves to the CPU). Larger ones are
+    latency-bound rocSOLVER solves: each runs on one of eight side streams so
+    that independent solves overlap on the GPU. (Padding every matrix into one
+    common-size batch gave non-finite eigenvectors from the batched GPU solver.)"""
+    out: list = [None] * len(mats)
+    small = [j for j, s in enumerate(mats) if int(s.shape[-1]) <= _EIGH_HOST_MAX]
+    large = [j for j, s in enumerate(mats) if int(s.shape[-1]) > _EIGH_HOST_MAX]
+    if small:
+        host = [mats[j].to(torch.complex128).cpu() for j in small]
+        for j, (w, u) in zip(small, _eigh_exact_groups(host)):
+            out[j] = (w.to(mats[j].device), u.to(mats[j].device))
+    if large:
+        if not _EIGH_STREAMS:
+            _EIGH_STREAMS.extend(torch.cuda.Stream() for _ in range(8))  # default priority
+        cur = torch.cuda.current_stream()
+        used = []
+        for n, j in enumerate(large):
+            st = _EIGH_STREAMS[n % len(_EIGH_STREAMS)]
+            st.wait_stream(cur)
+            s = mats[j].to(torch.complex128)
+            s.record_stream(st)
+            with torch.cuda.stream(st):
+                w, u = torch.linalg.eigh(s)
+            w.record_stream(cur)
+            u.record_stream(cur)
+            out[j] = (w, u)
+            if st not in used:
+                used.append(st)
+        for st in used:
+            cur.wait_stream(st)
+    return out
+
+
 @dataclass
 class RaggedSolveStats:
     rounds: int = 0
     batched_applies: int = 0
     batched_eighs: int = 0
+    host_reads: int = 0
+    seconds: dict | None = None  # phase wall times when profiling (synchronising)
 
 
 def _check_finite(kind: str, i: int, req, resp) -> None:
@@ -192,9 +239,20 @@ def davidson_ragged(batch: RaggedBatch, x0s: list[torch.Tensor], ts: list[torch.
     for i in range(len(gens)):
         advance(i, first=True)
     st = stats if stats is not None else RaggedSolveStats()
+    prof = st.seconds
+
+    def mark(key, t0):
+        if prof is None:
+            return t0
+        torch.cuda.synchronize()
+        now = time.perf_counter()
+        prof[key] = prof.get(key, 0.0) + now - t0
+        return now
+
     while pending:
         st.rounds += 1
         resp: dict[int, object] = {}
+        t0 = mark("solver", time.perf_counter())
         applies = {i: a for i, (k, a) in pending.items() if k == "apply"}
         by_dtype: dict[torch.dtype, dict[int, torch.Tensor]] = {}
         for i, z in applies.items():
@@ -205,17 +263,31 @@ def davidson_ragged(batch: RaggedBatch, x0s: list[torch.Tensor], ts: list[torch.
             st.batched_applies += 1
             for i in group:
                 resp[i] = outs[i]
+        t0 = mark("apply", t0)
         eighs = {i: s for i, (k, s) in pending.items() if k == "eigh"}
         if eighs:
             keys = list(eighs)
             for i, wu in zip(keys, _eigh_batched([eighs[i] for i in keys])):
                 resp[i] = wu
             st.batched_eighs += 1
+        t0 = mark("eigh", t0)
+        hosts = {i: t for i, (k, t) in pending.items() if k == "host"}
+        if hosts:  # every pending host read in one device-to-host copy
+            keys = list(hosts)
+            flat = torch.cat([hosts[i].reshape(-1).to(torch.float64) for i in keys]).cpu()
+            off = 0
+            for i in keys:
+                t = hosts[i]
+                resp[i] = flat[off:off + t.numel()].reshape(t.shape).to(t.dtype)
+                off += t.numel()
+            st.host_reads += 1
+        t0 = mark("host", t0)
         if check_finite:
             for i, r in resp.items():
                 _check_finite(pending[i][0], i, pending[i][1], r)
         for i, r in resp.items():
             advance(i, r)
+        mark("solver", t0)
     return results
 
 

@@ -369,6 +369,207 @@ k_gemm_combine_lds(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu, 
       }
 }
 
+// Tile-shape variants of the two LDS-staged kernels: each wave computes WM x WN
+// MFMA blocks of 16 x 16 (2 x 2 waves per workgroup), so a workgroup covers
+// 32 WM rows x 32 WN real columns. (2, 2) is the kernels above; the autotuner
+// picks among (2, 2), (4, 2), (2, 4) and (4, 4) per workload.
+template <typename T, int WM, int WN>
+__global__ void __launch_bounds__(256)
+k_gemm_inner_tile(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long sx,
+                  const cpx<T>* __restrict__ Y, long ldy, long sy, int splits, int kchunk,
+                  double* __restrict__ part, int ntx, int nty, long total) {
+  using A4 = typename Mfma<T>::acc;
+  constexpr int LD = kLd<T>;
+  constexpr int TM = 32 * WM, TN = 16 * WN;  // X rows, Y rows (complex columns of C)
+  __shared__ cpx<T> As[TM * LD];
+  __shared__ cpx<T> Bs[TN * LD];
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const int bx = (int)(wid % ntx), by = (int)((wid / ntx) % nty), bz = (int)(wid / ((long)ntx * nty));
+  const int k = bz / splits, sp = bz - k * splits;
+  const int gb = sp * kchunk, ge = min(K, gb + kchunk);
+  const int row0 = by * TM, col0 = bx * TN;
+  const cpx<T>* Xk = X + (long)k * sx;
+  const cpx<T>* Yk = Y + (long)k * sy;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = w >> 1, wn = w & 1;
+  constexpr int NA = TM * kKC / 256, NB = TN * kKC / 256;
+  cpx<T> ra[NA], rb[NB];
+  auto gload = [&](int g0) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int e = tid + 256 * i, r = e >> 5, cc = e & 31;
+      const int row = row0 + r, g = g0 + cc;
+      ra[i] = (row < M && g < ge) ? Xk[(long)row * ldx + g] : cpx<T>{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int e = tid + 256 * i, r = e >> 5, cc = e & 31;
+      const int row = col0 + r, g = g0 + cc;
+      rb[i] = (row < N && g < ge) ? Yk[(long)row * ldy + g] : cpx<T>{0, 0};
+    }
+  };
+  A4 acc[WM][WN];
+#pragma unroll
+  for (int a = 0; a < WM; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) acc[a][b] = A4{0, 0, 0, 0};
+  if (gb < ge) gload(gb);
+  for (int g0 = gb; g0 < ge; g0 += kKC) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NA; ++i) { const int e = tid + 256 * i; As[(e >> 5) * LD + (e & 31)] = ra[i]; }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) { const int e = tid + 256 * i; Bs[(e >> 5) * LD + (e & 31)] = rb[i]; }
+    __syncthreads();
+    if (g0 + kKC < ge) gload(g0 + kKC);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      T av[WM][8], bv[WN][8];
+#pragma unroll
+      for (int bm = 0; bm < WM; ++bm) {
+        const cpx<T>* ap = As + (wm * 16 * WM + bm * 16 + li) * LD + lk * 8 + h * 4;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { const cpx<T> x = ap[v]; av[bm][2 * v] = x.x; av[bm][2 * v + 1] = x.y; }
+      }
+#pragma unroll
+      for (int bn = 0; bn < WN; ++bn) {
+        const int jj = wn * 16 * WN + bn * 16 + li;
+        const cpx<T>* bp = Bs + (jj >> 1) * LD + lk * 8 + h * 4;
+        const bool odd = jj & 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const cpx<T> y = bp[v];
+          bv[bn][2 * v] = odd ? y.y : y.x;
+          bv[bn][2 * v + 1] = odd ? -y.x : y.y;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int bm = 0; bm < WM; ++bm)
+#pragma unroll
+          for (int bn = 0; bn < WN; ++bn) acc[bm][bn] = Mfma<T>::op(av[bm][v], bv[bn][v], acc[bm][bn]);
+    }
+  }
+  const int ncol = 2 * N;
+  double* pk = part + ((long)k * splits + sp) * M * ncol;
+#pragma unroll
+  for (int bm = 0; bm < WM; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < WN; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + wm * 16 * WM + bm * 16 + Mfma<T>::row(lk, r);
+        const int col = 2 * col0 + wn * 16 * WN + bn * 16 + li;
+        if (row < M && col < ncol) pk[(long)row * ncol + col] = (double)acc[bm][bn][r];
+      }
+}
+
+template <typename T, int WM, int WN>
+__global__ void __launch_bounds__(256)
+k_gemm_combine_tile(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu, long su, int conj_u,
+                    const cpx<T>* __restrict__ Y, long ldy, long sy, cpx<T>* __restrict__ Z, long ldz, long sz,
+                    T beta, int ntx, int nty, long total) {
+  using A4 = typename Mfma<T>::acc;
+  constexpr int LD = kLd<T>;
+  constexpr int TA = 32 * WM, TG = 16 * WN;
+  __shared__ cpx<T> Ut[TA * LD];  // [a][j]
+  __shared__ cpx<T> Yt[TG * LD];  // [g][j]
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const int by = (int)(wid % nty), bx = (int)((wid / nty) % ntx), k = (int)(wid / ((long)ntx * nty));
+  const int a0 = by * TA, g0c = bx * TG;
+  const cpx<T>* Uk = U + (long)k * su;
+  const cpx<T>* Yk = Y + (long)k * sy;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = w >> 1, wn = w & 1;
+  const T usign = conj_u ? T(-1) : T(1);
+  constexpr int NU = TA * kKC / 256, NY = TG * kKC / 256;
+  cpx<T> ru[NU], ry[NY];
+  auto gload = [&](int j0) {
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {  // U rows j (32) x columns a (TA), a fastest
+      const int e = tid + 256 * i, jr = e / TA, ac = e % TA;
+      const int j = j0 + jr, a = a0 + ac;
+      ru[i] = (j < J && a < A) ? Uk[(long)j * ldu + a] : cpx<T>{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < NY; ++i) {  // Y rows j (32) x columns g (TG), g fastest
+      const int e = tid + 256 * i, jr = e / TG, gc = e % TG;
+      const int j = j0 + jr, g = g0c + gc;
+      ry[i] = (j < J && g < G) ? Yk[(long)j * ldy + g] : cpx<T>{0, 0};
+    }
+  };
+  A4 acc[WM][WN];
+#pragma unroll
+  for (int a = 0; a < WM; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) acc[a][b] = A4{0, 0, 0, 0};
+  if (J > 0) gload(0);
+  for (int j0 = 0; j0 < J; j0 += kKC) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NU; ++i) { const int e = tid + 256 * i; Ut[(e % TA) * LD + (e / TA)] = ru[i]; }
+#pragma unroll
+    for (int i = 0; i < NY; ++i) { const int e = tid + 256 * i; Yt[(e % TG) * LD + (e / TG)] = ry[i]; }
+    __syncthreads();
+    if (j0 + kKC < J) gload(j0 + kKC);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      T av[WM][8], bv[WN][8];
+#pragma unroll
+      for (int bm = 0; bm < WM; ++bm) {
+        const cpx<T>* up = Ut + (wm * 16 * WM + bm * 16 + li) * LD + lk * 8 + h * 4;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) { const cpx<T> u = up[v]; av[bm][2 * v] = u.x; av[bm][2 * v + 1] = usign * u.y; }
+      }
+#pragma unroll
+      for (int bn = 0; bn < WN; ++bn) {
+        const int col = wn * 16 * WN + bn * 16 + li;
+        const cpx<T>* yp = Yt + (col >> 1) * LD + lk * 8 + h * 4;
+        const bool im = col & 1;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const cpx<T> y = yp[v];
+          bv[bn][2 * v] = im ? y.y : y.x;
+          bv[bn][2 * v + 1] = im ? y.x : -y.y;
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+#pragma unroll
+        for (int bm = 0; bm < WM; ++bm)
+#pragma unroll
+          for (int bn = 0; bn < WN; ++bn) acc[bm][bn] = Mfma<T>::op(av[bm][v], bv[bn][v], acc[bm][bn]);
+    }
+  }
+  const int ncol = 2 * G;
+  T* Zk = reinterpret_cast<T*>(Z + (long)k * sz);
+#pragma unroll
+  for (int bm = 0; bm < WM; ++bm)
+#pragma unroll
+    for (int bn = 0; bn < WN; ++bn)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = a0 + wm * 16 * WM + bm * 16 + Mfma<T>::row(lk, r);
+        const int col = 2 * g0c + wn * 16 * WN + bn * 16 + li;
+        if (row < A && col < ncol) {
+          T* zp = Zk + (long)row * 2 * ldz + col;
+          const T v = acc[bm][bn][r];
+          *zp = (beta != T(0)) ? v + beta * (*zp) : v;
+        }
+      }
+}
+
+static int g_gemm_tile = 0;  // 0: (2, 2), 1: (4, 2), 2: (2, 4), 3: (4, 4) MFMA blocks per wave
+void set_gemm_tile(int v) { g_gemm_tile = (v >= 0 && v <= 3) ? v : 0; }
+int gemm_tile() { return g_gemm_tile; }
+
 static bool gemm_simple() {
   static const bool v = [] { const char* e = getenv("GW_GEMM_SIMPLE"); return e && e[0] == '1'; }();
   return v;
@@ -411,9 +612,21 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
   kchunk = ((kchunk + kKC - 1) / kKC) * kKC;  // multiple of both K granules (16, 32)
   const int ntx = (2 * N + kTile - 1) / kTile, nty = (M + kTile - 1) / kTile;
   const long total = (long)ntx * nty * nk * splits;
+  auto launch_tile = [&](auto kern, int wm, int wn) {
+    const int tx = (2 * N + 32 * wn - 1) / (32 * wn), ty = (M + 32 * wm - 1) / (32 * wm);
+    const long tt = (long)tx * ty * nk * splits;
+    hipLaunchKernelGGL(kern, dim3((unsigned)(8 * ((tt + 7) / 8))), dim3(256), 0, st, M, N, K, X, ldx, sx, Y, ldy,
+                       sy, splits, kchunk, (double*)partials, tx, ty, tt);
+  };
   if (gemm_simple())
     hipLaunchKernelGGL(k_gemm_inner<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X, ldx,
                        sx, Y, ldy, sy, splits, kchunk, (double*)partials, ntx, nty, total);
+  else if (g_gemm_tile == 1)
+    launch_tile(k_gemm_inner_tile<T, 4, 2>, 4, 2);
+  else if (g_gemm_tile == 2)
+    launch_tile(k_gemm_inner_tile<T, 2, 4>, 2, 4);
+  else if (g_gemm_tile == 3)
+    launch_tile(k_gemm_inner_tile<T, 4, 4>, 4, 4);
   else
     hipLaunchKernelGGL(k_gemm_inner_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X,
                        ldx, sx, Y, ldy, sy, splits, kchunk, (double*)partials, ntx, nty, total);
@@ -515,9 +728,21 @@ int gemm_combine(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su
   if (A <= 0 || G <= 0 || nk <= 0) return GW_OK;
   const int ntx = (2 * G + kTile - 1) / kTile, nty = (A + kTile - 1) / kTile;
   const long total = (long)ntx * nty * nk;
+  auto launch_tile = [&](auto kern, int wm, int wn) {
+    const int tx = (2 * G + 32 * wn - 1) / (32 * wn), ty = (A + 32 * wm - 1) / (32 * wm);
+    const long tt = (long)tx * ty * nk;
+    hipLaunchKernelGGL(kern, dim3((unsigned)(8 * ((tt + 7) / 8))), dim3(256), 0, st, A, G, J, U, ldu, su, conj_u, Y,
+                       ldy, sy, Z, ldz, sz, beta, tx, ty, tt);
+  };
   if (gemm_simple())
     hipLaunchKernelGGL(k_gemm_combine<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, A, G, J, U,
                        ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, ntx, nty, total);
+  else if (g_gemm_tile == 1)
+    launch_tile(k_gemm_combine_tile<T, 4, 2>, 4, 2);
+  else if (g_gemm_tile == 2)
+    launch_tile(k_gemm_combine_tile<T, 2, 4>, 2, 4);
+  else if (g_gemm_tile == 3)
+    launch_tile(k_gemm_combine_tile<T, 4, 4>, 4, 4);
   else
     hipLaunchKernelGGL(k_gemm_combine_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, A, G, J,
                        U, ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, ntx, nty, total);

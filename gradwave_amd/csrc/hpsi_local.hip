@@ -1055,12 +1055,18 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
     set_gemm_target_wgs(value);
     return GW_OK;
   }
+  if (key == GW_TUNE_GEMM_TILE) {
+    GW_REQUIRE(value >= 0 && value <= 3, GW_ERR_ARG, "bad tile variant %d", value);
+    set_gemm_tile(value);
+    return GW_OK;
+  }
   GW_REQUIRE(false, GW_ERR_ARG, "unknown tuning key %d", key);
 }
 
 extern "C" int gw_tune_get(int key, int arg) {
   if (key == GW_TUNE_RADIX) return (arg > 0 && arg < 1024) ? g_radix_pref[arg] : -1;
   if (key == GW_TUNE_GEMM_WGS) return gemm_target_wgs();
+  if (key == GW_TUNE_GEMM_TILE) return gemm_tile();
   return -1;
 }
 extern "C" int gw_version(void) { return 1; }

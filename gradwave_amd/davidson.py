@@ -94,21 +94,21 @@ def _hermitize(s: torch.Tensor) -> torch.Tensor:
     return (0.5 * (s + s.conj().transpose(-1, -2))).to(torch.complex128)
 
 
-def _cholqr2(v: torch.Tensor) -> torch.Tensor | None:
+def _cholqr2_dev(v: torch.Tensor):
     """Orthonormal rows spanning v (nk, n, npw) by two rounds of fp64 Gram ->
     Cholesky -> triangular inverse -> combination (davidson.py:195-222 in row form:
-    G = v v^H = L L^H, q = L^-1 v). None on breakdown."""
+    G = v v^H = L L^H, q = L^-1 v). Returns (q, breakdown flags on the device)."""
     n = v.shape[1]
     eye = torch.eye(n, dtype=torch.complex128, device=v.device)
+    infos = []
     for _ in range(2):
         g = LA.inner(v, v).to(torch.complex128).conj()  # G[i][j] = sum v_i conj(v_j)
         g = 0.5 * (g + g.mH)
         lo, info = torch.linalg.cholesky_ex(g)
-        if int(info.max()) != 0:
-            return None
+        infos.append(info.max())
         linv = torch.linalg.solve_triangular(lo, eye.expand_as(lo), upper=False)
         v = LA.combine(linv.transpose(-1, -2).to(v.dtype).contiguous(), v)
-    return v
+    return v, torch.stack(infos)
 
 
 def _project(v: torch.Tensor, against: torch.Tensor | None) -> torch.Tensor:
@@ -121,23 +121,32 @@ def _project(v: torch.Tensor, against: torch.Tensor | None) -> torch.Tensor:
     return v
 
 
-def _orthonormalize_b(v: torch.Tensor, mask: torch.Tensor,
-                      against: torch.Tensor | None = None) -> torch.Tensor:
+def _orthonormalize_steps(v: torch.Tensor, mask: torch.Tensor, against: torch.Tensor | None = None):
     """davidson.py:406-457: mask, project, deterministic jitter on degenerate rows,
-    orthonormalise (CholQR2 first, Householder QR on breakdown)."""
+    orthonormalise (CholQR2 first, Householder QR on breakdown). A step
+    generator: its two host reads are ("host", tensor) requests."""
     v = _project((v * mask[:, None, :]).contiguous(), against)
     rn = LA.row_norms(v)
     degenerate = rn < 1e-8
-    if bool(degenerate.any()):
+    # CholQR2 runs before the degenerate-row test is read, so both tests cost one host read
+    q, infos = _cholqr2_dev(v)
+    flags = yield ("host", torch.cat([degenerate.any().reshape(1).to(infos.dtype), infos]))
+    if bool(flags[0]):
         gen = torch.Generator(device="cpu").manual_seed(v.shape[1] + 7919)
         noise = torch.randn(*v.shape, 2, generator=gen, dtype=torch.float64)
         jit = torch.view_as_complex(noise).to(v.device).to(v.dtype)
         v = _project(((v + degenerate[..., None] * jit) * mask[:, None, :]).contiguous(), against)
-    q = _cholqr2(v)
-    if q is None:
+        q, infos = _cholqr2_dev(v)
+        flags = yield ("host", torch.cat([flags[:1].to(infos.device, infos.dtype), infos]))
+    if bool((flags[1:] != 0).any()):
         qq, _ = torch.linalg.qr(v.transpose(-1, -2), mode="reduced")
         q = qq.transpose(-1, -2).contiguous()
     return q
+
+
+def _orthonormalize_b(v: torch.Tensor, mask: torch.Tensor,
+                      against: torch.Tensor | None = None) -> torch.Tensor:
+    return drive(_orthonormalize_steps(v, mask, against), None)
 
 
 class _Basis:
@@ -205,13 +214,20 @@ def _apply_steps(z, low, counts):
 
 
 def drive(steps, h_apply):
-    """Run a Davidson step generator to completion, serving its apply requests
-    with ``h_apply`` and its subspace eigensolves with ``_eigh``."""
+    """Run a step generator to completion, serving its ("apply", block) requests
+    with ``h_apply``, its ("eigh", matrix) requests with ``_eigh`` and its
+    ("host", tensor) requests with a copy to the host."""
     try:
         req = next(steps)
         while True:
             kind, arg = req
-            req = steps.send(h_apply(arg) if kind == "apply" else _eigh(arg))
+            if kind == "apply":
+                resp = h_apply(arg)
+            elif kind == "eigh":
+                resp = _eigh(arg)
+            else:
+                resp = arg.cpu()
+            req = steps.send(resp)
     except StopIteration as stop:
         return stop.value
 
@@ -252,7 +268,7 @@ def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: f
     prev_rnmax = float("inf")
     basis = _Basis(nk, max(max_dim, nb), m, x0.dtype, device)
 
-    v0 = _orthonormalize_b(x0.contiguous(), mask)
+    v0 = yield from _orthonormalize_steps(x0.contiguous(), mask)
     hv0 = yield from _apply_steps(v0, use_fp32_exp, counts)
     basis.set(v0, hv0)
     eig = torch.zeros(nk, nb, dtype=rdtype, device=device)
@@ -281,10 +297,11 @@ def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: f
                     use_fp32_exp = False
         if history_out is not None:
             history_out.append((it, rn.detach().to("cpu").clone(), eig.detach().to("cpu").clone()))
-        if float(rn.max()) < tol:
+        gate = yield ("host", torch.stack([rn.max(), (rn > tol).sum(dim=1).max().to(rn.dtype)]))
+        if float(gate[0]) < tol:
             tm(None)
             return BatchedDavidsonResult(eig, x, it, rn.to(rdtype), counts["low"], counts["full"])
-        n_add = int((rn > tol).sum(dim=1).max())
+        n_add = int(gate[1])
         sel = torch.argsort(rn, dim=1, descending=True)[:, :n_add]
         d = LA.teter_expand(x, r, t, sel)
         tm("restart")
@@ -305,7 +322,7 @@ def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: f
             basis.set(x_orth, hx_orth)
         v, _ = basis.views()
         tm("orthonormalize")
-        d = _orthonormalize_b(d, mask, against=v)
+        d = yield from _orthonormalize_steps(d, mask, against=v)
         tm("h_apply")
         hd = yield from _apply_steps(d, use_fp32_exp, counts)
         tm("subspace_gemm")

@@ -186,8 +186,10 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
  * GW_TUNE_RADIX: for box length `arg`, the first factor `value` of the two-codelet
  *   split used by plans created afterwards (0 = the default most balanced split).
  * GW_TUNE_GEMM_WGS: the number of workgroups the split-K inner products aim at
- *   (`arg` ignored; 0 = default 2048). */
-enum { GW_TUNE_RADIX = 1, GW_TUNE_GEMM_WGS = 2 };
+ *   (`arg` ignored; 0 = default 2048).
+ * GW_TUNE_GEMM_TILE: MFMA blocks per wave of the complex GEMMs, 0 = 2x2, 1 = 4x2,
+ *   2 = 2x4, 3 = 4x4 (`arg` ignored). */
+enum { GW_TUNE_RADIX = 1, GW_TUNE_GEMM_WGS = 2, GW_TUNE_GEMM_TILE = 3 };
 int gw_tune_set(int key, int arg, int value);
 int gw_tune_get(int key, int arg);
 

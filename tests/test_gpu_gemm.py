@@ -122,3 +122,31 @@ def test_real_combine(dtype, nk, A, G, J, alpha, beta):
     LA.rcombine(u.to(dev()), y.to(dev()), out=z, alpha=alpha, beta=beta)
     scale = float(u.double().norm(dim=-2).max() * y.double().norm(dim=-2).max()) + float(z0.abs().max())
     assert float((z.cpu().double() - want).abs().max()) < RTOL[dtype] * scale
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_tile_variants_match(tile, dtype):
+    """The autotuner's MFMA tile shapes give the same contractions."""
+    from gradwave_amd import _lib as L
+
+    say(f"gemm tile variant {tile} {dtype}")
+    lib = L.lib()
+    try:
+        lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, tile)
+        for (nk, M, N, K) in [(1, 256, 160, 37073), (3, 17, 33, 1000), (2, 70, 5, 4097)]:
+            x, y = _rand(nk, M, K, seed=1, dtype=dtype), _rand(nk, N, K, seed=2, dtype=dtype)
+            want = torch.matmul(x.to(torch.complex128).conj(), y.to(torch.complex128).mT)
+            got = LA.inner(x.to(dev()), y.to(dev())).cpu().to(torch.complex128)
+            scale = float((x.abs() ** 2).sum(-1).max().sqrt() * (y.abs() ** 2).sum(-1).max().sqrt())
+            assert float((got - want).abs().max()) < TOL[dtype] * scale
+        for (nk, A, G, J) in [(1, 160, 37073, 256), (3, 33, 1000, 17), (2, 5, 4097, 70)]:
+            u, y = _rand(nk, J, A, seed=3, dtype=dtype), _rand(nk, J, G, seed=4, dtype=dtype)
+            z0 = _rand(nk, A, G, seed=5, dtype=dtype)
+            want = torch.matmul(u.to(torch.complex128).conj().mT, y.to(torch.complex128)) + z0.to(torch.complex128)
+            z = z0.to(dev())
+            LA.combine(u.to(dev()), y.to(dev()), out=z, beta=1.0, conj_u=True)
+            scale = float(u.abs().pow(2).sum(-2).max().sqrt() * y.abs().pow(2).sum(-2).max().sqrt()) + 1.0
+            assert float((z.cpu().to(torch.complex128) - want).abs().max()) < TOL[dtype] * scale
+    finally:
+        lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, 0)
