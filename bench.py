@@ -85,9 +85,9 @@ def main() -> None:
         from gradwave_amd import autotune
 
         t1 = time.perf_counter()
-        tile, wgs = autotune.tune_gemm([wl], nb=nb)
-        tuned = {"gemm_tile_variant": tile, "gemm_split_k_target_wgs": wgs, "seconds": time.perf_counter() - t1}
-        print(f"[bench] autotune GEMM: tile {tile}, split-K target {wgs} ({tuned['seconds']:.1f}s)",
+        tiles, wgs = autotune.tune_gemm([wl], nb=nb)
+        tuned = {"gemm_tile_variant": tiles, "gemm_split_k_target_wgs": wgs, "seconds": time.perf_counter() - t1}
+        print(f"[bench] autotune GEMM: tiles {tiles}, split-K target {wgs} ({tuned['seconds']:.1f}s)",
               file=sys.stderr, flush=True)
     if args.only is not None:
         profile_only(args, wl, x64)
@@ -300,7 +300,7 @@ def bench_ragged(args) -> None:
 
     t0 = time.perf_counter()
     radix = autotune.tune_radix(wls)
-    gemm_tile, gemm_wgs = autotune.tune_gemm(wls)
+    gemm_tile, gemm_wgs = autotune.tune_gemm(wls)  # ({"fp64": t, "fp32": t}, target)
     t_tune = time.perf_counter() - t0
     hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in wls]  # plans with the tuned splits
     batch = B.RaggedBatch(hams)

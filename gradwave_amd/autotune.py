@@ -45,7 +45,8 @@ def current() -> dict:
     radix = {n: lib.gw_tune_get(L.GW_TUNE_RADIX, n) for n in range(1, 1024)}
     return {"radix": {str(n): a for n, a in radix.items() if a > 0},
             "gemm_wgs": lib.gw_tune_get(L.GW_TUNE_GEMM_WGS, 0),
-            "gemm_tile": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, 0)}
+            "gemm_tile": {"fp64": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP64),
+                          "fp32": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32)}}
 
 
 def apply(choices: dict) -> None:
@@ -54,8 +55,9 @@ def apply(choices: dict) -> None:
         L.check(lib.gw_tune_set(L.GW_TUNE_RADIX, int(n), int(a)), "gw_tune_set")
     if "gemm_wgs" in choices:
         L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, int(choices["gemm_wgs"])), "gw_tune_set")
-    if "gemm_tile" in choices:
-        L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, int(choices["gemm_tile"])), "gw_tune_set")
+    for name, prec in (("fp64", L.GW_FP64), ("fp32", L.GW_FP32)):
+        if name in choices.get("gemm_tile", {}):
+            L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, int(choices["gemm_tile"][name])), "gw_tune_set")
 
 
 def load(path: str | None = None) -> dict | None:
@@ -122,33 +124,38 @@ def tune_radix(workloads, nb: int = 8, reps: int = 5) -> dict[int, int]:
 GEMM_TILES = (0, 1, 2, 3)  # MFMA blocks per wave: 2x2, 4x2, 2x4, 4x4
 
 
-def tune_gemm(workloads, nb: int = 32, reps: int = 5, dtype=torch.complex64) -> tuple[int, int]:
+def tune_gemm(workloads, nb: int = 32, reps: int = 5) -> tuple[dict, int]:
     """Time the projector products (nonlocal term) of ``workloads``: first each
-    MFMA tile shape at the current split-K target, then each split-K workgroup
-    target with the best tile; keep the fastest. Returns (tile, target)."""
+    MFMA tile shape per precision at the current split-K target, then each
+    split-K workgroup target (both precisions) with the best tiles; keep the
+    fastest. Returns ({"fp64": tile, "fp32": tile}, target)."""
     from gradwave_amd.hamiltonian import BatchedHamiltonian
 
     lib = L.lib()
     hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in workloads]
     g = torch.Generator().manual_seed(7)
-    blocks = [(torch.randn(w.bk.nk, nb, w.bk.npw_max, dtype=torch.complex128, generator=g).to(w.v_r.device)
-               * w.bk.mask[:, None, :]).to(dtype) for w in workloads]
+    b64 = [(torch.randn(w.bk.nk, nb, w.bk.npw_max, dtype=torch.complex128, generator=g).to(w.v_r.device)
+            * w.bk.mask[:, None, :]) for w in workloads]
+    blocks = {"fp64": b64, "fp32": [b.to(torch.complex64) for b in b64]}
 
-    def run():
-        return [h.apply_terms(x, L.GW_TERM_NONLOCAL) for h, x in zip(hams, blocks)]
+    def run(name):
+        return [h.apply_terms(x, L.GW_TERM_NONLOCAL) for h, x in zip(hams, blocks[name])]
 
-    best_tile, best_ms = 0, float("inf")
-    for tile in GEMM_TILES:
-        L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, tile), "gw_tune_set")
-        ms = _median_ms(run, reps)
-        if ms < best_ms:
-            best_tile, best_ms = tile, ms
-    L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, best_tile), "gw_tune_set")
+    tiles = {}
+    for name, prec in (("fp64", L.GW_FP64), ("fp32", L.GW_FP32)):
+        best_tile, best_ms = 0, float("inf")
+        for tile in GEMM_TILES:
+            L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, tile), "gw_tune_set")
+            ms = _median_ms(lambda: run(name), reps)
+            if ms < best_ms:
+                best_tile, best_ms = tile, ms
+        L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, best_tile), "gw_tune_set")
+        tiles[name] = best_tile
     best, best_ms = None, float("inf")
     for target in GEMM_TARGETS:
         L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, target), "gw_tune_set")
-        ms = _median_ms(run, reps)
+        ms = _median_ms(lambda: (run("fp64"), run("fp32")), reps)
         if ms < best_ms:
             best, best_ms = target, ms
     L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, best), "gw_tune_set")
-    return best_tile, best
+    return tiles, best

@@ -1057,7 +1057,8 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
   }
   if (key == GW_TUNE_GEMM_TILE) {
     GW_REQUIRE(value >= 0 && value <= 3, GW_ERR_ARG, "bad tile variant %d", value);
-    set_gemm_tile(value);
+    GW_REQUIRE(arg == GW_FP64 || arg == GW_FP32, GW_ERR_ARG, "bad precision %d", arg);
+    set_gemm_tile(arg, value);
     return GW_OK;
   }
   GW_REQUIRE(false, GW_ERR_ARG, "unknown tuning key %d", key);
@@ -1066,7 +1067,7 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
 extern "C" int gw_tune_get(int key, int arg) {
   if (key == GW_TUNE_RADIX) return (arg > 0 && arg < 1024) ? g_radix_pref[arg] : -1;
   if (key == GW_TUNE_GEMM_WGS) return gemm_target_wgs();
-  if (key == GW_TUNE_GEMM_TILE) return gemm_tile();
+  if (key == GW_TUNE_GEMM_TILE) return (arg == GW_FP64 || arg == GW_FP32) ? gemm_tile(arg) : -1;
   return -1;
 }
 extern "C" int gw_version(void) { return 1; }

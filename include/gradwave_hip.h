@@ -187,8 +187,8 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
  *   split used by plans created afterwards (0 = the default most balanced split).
  * GW_TUNE_GEMM_WGS: the number of workgroups the split-K inner products aim at
  *   (`arg` ignored; 0 = default 2048).
- * GW_TUNE_GEMM_TILE: MFMA blocks per wave of the complex GEMMs, 0 = 2x2, 1 = 4x2,
- *   2 = 2x4, 3 = 4x4 (`arg` ignored). */
+ * GW_TUNE_GEMM_TILE: MFMA blocks per wave of the complex GEMMs of precision `arg`
+ *   (GW_FP64 / GW_FP32), 0 = 2x2, 1 = 4x2, 2 = 2x4, 3 = 4x4. */
 enum { GW_TUNE_RADIX = 1, GW_TUNE_GEMM_WGS = 2, GW_TUNE_GEMM_TILE = 3 };
 int gw_tune_set(int key, int arg, int value);
 int gw_tune_get(int key, int arg);

@@ -127,8 +127,8 @@ def test_autotuner_choices_are_valid_and_exact(tmp_path, monkeypatch):
     say("autotune radix + split-K on 4 cells")
     wls = _instances()[:4]
     radix = autotune.tune_radix(wls, reps=2)
-    tile, wgs = autotune.tune_gemm(wls, reps=2)
-    assert wgs in autotune.GEMM_TARGETS and tile in autotune.GEMM_TILES
+    tiles, wgs = autotune.tune_gemm(wls, reps=2)
+    assert wgs in autotune.GEMM_TARGETS and all(t in autotune.GEMM_TILES for t in tiles.values())
     for n, a in radix.items():
         assert a in autotune.candidate_splits(n)
     hams = _hams(wls)
@@ -150,4 +150,5 @@ def test_autotuner_choices_are_valid_and_exact(tmp_path, monkeypatch):
     for n in radix:  # leave the default splits for later tests
         L.lib().gw_tune_set(L.GW_TUNE_RADIX, n, 0)
     L.lib().gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, 0)
-    L.lib().gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, 0)
+    L.lib().gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP64, 0)
+    L.lib().gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32, 0)

@@ -132,8 +132,9 @@ def test_tile_variants_match(tile, dtype):
 
     say(f"gemm tile variant {tile} {dtype}")
     lib = L.lib()
+    prec = L.precision_of(dtype)
     try:
-        lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, tile)
+        assert lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, tile) == 0
         for (nk, M, N, K) in [(1, 256, 160, 37073), (3, 17, 33, 1000), (2, 70, 5, 4097)]:
             x, y = _rand(nk, M, K, seed=1, dtype=dtype), _rand(nk, N, K, seed=2, dtype=dtype)
             want = torch.matmul(x.to(torch.complex128).conj(), y.to(torch.complex128).mT)
@@ -149,4 +150,4 @@ def test_tile_variants_match(tile, dtype):
             scale = float(u.abs().pow(2).sum(-2).max().sqrt() * y.abs().pow(2).sum(-2).max().sqrt()) + 1.0
             assert float((z.cpu().to(torch.complex128) - want).abs().max()) < TOL[dtype] * scale
     finally:
-        lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, 0, 0)
+        lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, 0)
