@@ -53,6 +53,8 @@ SIGNATURES = {
     "gw_row_norms": (_I, [_I, _I, _I, _I, _P, _L, _L, _P, _P]),
     "gw_hpsi_adjoint_v_workspace_bytes": (_Z, [_P, _I, _I]),
     "gw_hpsi_adjoint_v": (_I, [_P, _I, _P, _L, _L, _P, _L, _L, _I, _P, _P, _Z, _P]),
+    "gw_density_workspace_bytes": (_Z, [_P, _I, _I]),
+    "gw_density": (_I, [_P, _I, _P, _L, _L, _I, _P, _P, _P, _Z, _P]),
     "gw_proj_phase": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
     "gw_proj_phase_grad_workspace_bytes": (_Z, [_I, _I]),
     "gw_proj_phase_grad": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _Z, _P]),

@@ -881,6 +881,86 @@ static int adjv_wave_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, long ld
   return GW_OK;
 }
 
+// ---- band density: rho(r) += sum_{k,b} w_kb |psi_kb(r)|^2, psi = N ifftn(scatter c) ----
+// (core/batch.py:562-587 without the final 1/volume.) P1 fills the box, this
+// x-pass finishes the inverse transform of a group of bands and reduces
+// w |psi|^2 in FP64 into partials [group][x][y*n3 + z], summed by k_adjv_reduce.
+template <typename T, int N, int M, int S>
+__global__ void __launch_bounds__(256)
+k_xpass_density_wave(Geom g, int nbt, int bgroup, const cpx<T>* __restrict__ box, const double* __restrict__ wts,
+                     const cpx<T>* __restrict__ twx, int nchunk, long total, double* __restrict__ part) {
+  using W = WaveLine<T, N, M, S>;
+  constexpr int LW = W::LW;
+  __shared__ cpx<T> E[4 * W::E_PER_WAVE];
+  __shared__ cpx<T> tw[N];
+  const long wid = xcd_work_id(total);
+  if (wid < 0) return;
+  const int chunk = (int)(wid % nchunk), bgi = (int)(wid / nchunk);
+  for (int i = threadIdx.x; i < N; i += blockDim.x) tw[i] = twx[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = lane / LW, l = lane - t * LW;
+  const bool part_ = t < S;
+  const int n23 = g.n2 * g.n3;
+  const int j = (chunk * 4 + wave) * LW + l;
+  const bool jok = j < n23;
+  cpx<T>* Ew = E + wave * W::E_PER_WAVE;
+  auto slot = [&](int i0) { return i0 < g.xlo ? i0 : (i0 >= g.xhi ? g.xlo + i0 - g.xhi : -1); };
+  double acc[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) acc[s] = 0.0;
+  const int b0 = bgi * bgroup;
+  const int nbg = min(bgroup, nbt - b0);
+  for (int bg = 0; bg < nbg; ++bg) {
+    const long off = (long)(b0 + bg) * g.na * n23 + j;
+    const double wb = wts[b0 + bg];
+    cpx<T> x[M], y[S];
+#pragma unroll
+    for (int q = 0; q < M; ++q) {
+      const int sl = part_ ? slot(S * q + t) : -1;
+      x[q] = (jok && sl >= 0) ? box[off + (long)sl * n23] : czero<T>();
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s) y[s] = czero<T>();
+    W::inverse(x, y, t, l, part_, Ew, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) acc[s] += wb * ((double)y[s].x * y[s].x + (double)y[s].y * y[s].y);
+  }
+  if (t < M && jok) {
+    double* pp = part + (long)bgi * N * n23;
+#pragma unroll
+    for (int s = 0; s < S; ++s) pp[(long)(t + M * s) * n23 + j] = acc[s];
+  }
+}
+
+template <typename T, int N>
+static int density_wave_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, long ldc, long sc, const double* wts,
+                          cpx<T>* box, double* part, double* rho, hipStream_t st) {
+  using C = WaveCfg<N>;
+  using W = WaveLine<T, N, C::M, C::S>;
+  const long total = (long)P.na * nbt;
+  const int nw = plane_waves<T, N>(P);
+  GW_REQUIRE(nw > 0, GW_ERR_UNSUPPORTED, "x-plane sticks exceed LDS");
+  cpx<T>* nullout = nullptr;
+  if (nw == kPW)
+    launch_planes<T, N, kPW>(true, P, g, total, c, ldc, sc, box, nullptr, nullout, 0, 0, T(1), false, st);
+  else
+    launch_planes<T, N, 4>(true, P, g, total, c, ldc, sc, box, nullptr, nullout, 0, 0, T(1), false, st);
+  const int n23 = N * N;
+  const int lines_per_wg = 4 * W::LW;
+  const int nchunk = (n23 + lines_per_wg - 1) / lines_per_wg;
+  const int bgroup = kAdjBandGroup;
+  const int ngroups = (nbt + bgroup - 1) / bgroup;
+  const long xtotal = (long)nchunk * ngroups;
+  hipLaunchKernelGGL((k_xpass_density_wave<T, N, C::M, C::S>), dim3((unsigned)(8 * ((xtotal + 7) / 8))), dim3(256),
+                     0, st, g, nbt, bgroup, box, wts, tw_of<T>(&P, 0), nchunk, xtotal, part);
+  const long n = (long)N * n23;
+  hipLaunchKernelGGL(k_adjv_reduce, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, ngroups, 1.0, part,
+                     rho);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
 template <int N>
 static bool plane_w_fits(const Plan& P, int precision) {
   return (precision == GW_FP64 ? plane_waves<double, N>(P) : plane_waves<float, N>(P)) > 0;
@@ -1867,4 +1947,46 @@ extern "C" int gw_batch_apply(gw_batch* b, int precision, int terms, const void*
   const int rc = gw_batch_bind(b, precision, terms, c, ldc, sc, out, ldo, so, nb, ws, ws_bytes);
   if (rc) return rc;
   return gw_batch_apply_bound(b, stream);
+}
+
+extern "C" size_t gw_density_workspace_bytes(const gw_plan* h, int nb, int precision) {
+  if (!h || nb <= 0) return 0;
+  const Plan& P = h->p;
+  const size_t e = precision == GW_FP64 ? 16 : 8;
+  const size_t nbt = (size_t)P.nk * nb;
+  const size_t box = nbt * P.na * P.n[1] * P.n[2] * e;
+  const size_t ngroups = (nbt + kAdjBandGroup - 1) / kAdjBandGroup;
+  return align256(box) + align256(ngroups * (size_t)P.n[0] * P.n[1] * P.n[2] * sizeof(double));
+}
+
+template <typename T>
+static int density_t(gw_plan* h, const cpx<T>* c, long ldc, long sc, int nb, const double* wts, double* rho,
+                     unsigned char* ws, hipStream_t st) {
+  Plan& P = h->p;
+  const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
+  GW_REQUIRE(wave_path_ok(P, prec), GW_ERR_UNSUPPORTED,
+             "band density needs a cubic 24/36/90/120 box with the wave-line kernels");
+  const Geom g = geom_of(&P, nb);
+  const int nbt = P.nk * nb;
+  const size_t box = align256((size_t)nbt * P.na * P.n[1] * P.n[2] * sizeof(cpx<T>));
+  cpx<T>* bx = (cpx<T>*)ws;
+  double* part = (double*)(ws + box);
+  switch (P.n[0]) {
+    case 24: return density_wave_n<T, 24>(P, g, nbt, c, ldc, sc, wts, bx, part, rho, st);
+    case 36: return density_wave_n<T, 36>(P, g, nbt, c, ldc, sc, wts, bx, part, rho, st);
+    case 90: return density_wave_n<T, 90>(P, g, nbt, c, ldc, sc, wts, bx, part, rho, st);
+    case 120: return density_wave_n<T, 120>(P, g, nbt, c, ldc, sc, wts, bx, part, rho, st);
+    default: return GW_ERR_UNSUPPORTED;
+  }
+}
+
+extern "C" int gw_density(gw_plan* h, int precision, const void* c, long ldc, long sc, int nb, const double* weights,
+                          double* rho, void* ws, size_t ws_bytes, void* stream) {
+  GW_REQUIRE(h && c && weights && rho && nb > 0, GW_ERR_ARG, "bad argument");
+  const size_t need = gw_density_workspace_bytes(h, nb, precision);
+  GW_REQUIRE(ws && ws_bytes >= need, GW_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, need);
+  if (precision == GW_FP64)
+    return density_t<double>(h, (const double2*)c, ldc, sc, nb, weights, rho, (unsigned char*)ws,
+                             (hipStream_t)stream);
+  return density_t<float>(h, (const float2*)c, ldc, sc, nb, weights, rho, (unsigned char*)ws, (hipStream_t)stream);
 }

@@ -284,3 +284,49 @@ class BatchedHamiltonian:
                                        ws.data_ptr(), ws.numel(), L.stream_ptr(c.device)),
                 "gw_wfc_to_real")
         return psi
+
+
+_DENSITY_PLANS: dict[tuple, tuple] = {}
+
+
+def _density_plan(bk, shape) -> Plan:
+    flat = bk.flat_idx.detach().to("cpu", torch.int64)
+    key = (tuple(int(s) for s in shape), tuple(flat.shape))
+    hit = _DENSITY_PLANS.get(key)
+    if hit is not None and torch.equal(hit[0], flat):
+        return hit[1]
+    plan = Plan(shape, flat, bk.npw)
+    _DENSITY_PLANS[key] = (flat, plan)
+    return plan
+
+
+def density_b(coeffs: torch.Tensor, occ: torch.Tensor, kweights: torch.Tensor, bk, shape, volume: float) -> torch.Tensor:
+    """rho(r) on the dense grid [e/A^3] (core/batch.py:562-587):
+    sum_k w_k sum_b f_kb |psi_kb(r)|^2 / volume, psi = N ifftn(scatter c).
+
+    Cubic 24/36/90/120 boxes run the fused wave-line density kernel (one inverse
+    transform per band, FP64 accumulation); other boxes transform with
+    ``gw_wfc_to_real`` and reduce on the GPU."""
+    L.require_gpu(coeffs, "coeffs")
+    plan = _density_plan(bk, shape)
+    prec = L.precision_of(coeffs.dtype)
+    c = coeffs if coeffs.stride(2) == 1 else coeffs.contiguous()
+    nk, nb, _ = c.shape
+    w = (kweights.to(c.device, torch.float64)[:, None] * occ.to(c.device, torch.float64)).contiguous()
+    rho = torch.zeros(tuple(int(s) for s in shape), dtype=torch.float64, device=c.device)
+    info = (C.c_int * 11)()
+    L.check(L.lib().gw_plan_info(plan.handle, prec, info, 11), "gw_plan_info")
+    if info[2] == 2:
+        need = int(L.lib().gw_density_workspace_bytes(plan.handle, nb, prec))
+        ws = torch.empty(max(need, 256), dtype=torch.uint8, device=c.device)
+        L.check(L.lib().gw_density(plan.handle, prec, c.data_ptr(), c.stride(1), c.stride(0), nb, w.data_ptr(),
+                                   rho.data_ptr(), ws.data_ptr(), ws.numel(), L.stream_ptr(c.device)), "gw_density")
+    else:
+        need = int(L.lib().gw_wfc_to_real_workspace_bytes(plan.handle, nb, prec))
+        ws = torch.empty(max(need, 256), dtype=torch.uint8, device=c.device)
+        psi = torch.empty(nk, nb, *rho.shape, dtype=c.dtype, device=c.device)
+        cc = c.contiguous()
+        L.check(L.lib().gw_wfc_to_real(plan.handle, prec, cc.data_ptr(), psi.data_ptr(), nb, ws.data_ptr(),
+                                       ws.numel(), L.stream_ptr(c.device)), "gw_wfc_to_real")
+        rho += torch.einsum("kb,kbxyz->xyz", w, (psi.real.double() ** 2 + psi.imag.double() ** 2))
+    return rho / volume

@@ -161,6 +161,13 @@ size_t gw_hpsi_adjoint_v_workspace_bytes(const gw_plan* plan, int nb, int precis
 int gw_hpsi_adjoint_v(gw_plan* plan, int precision, const void* c, long ldc, long sc, const void* g,
                       long ldg, long sg, int nb, double* grad_v, void* workspace,
                       size_t workspace_bytes, void* stream);
+/* band density (core/batch.py:562-587 before its division by the volume):
+ * rho (n1, n2, n3) float64 natural order, ACCUMULATED: rho(r) += sum_{k,b} w[k][b] |psi_kb(r)|^2,
+ * psi = N ifftn(scatter c); weights (nk, nb) float64 on the device (kweight * occupation).
+ * Needs a cubic 24/36/90/120 box (gw_plan_info fused_planes == 2). */
+size_t gw_density_workspace_bytes(const gw_plan* plan, int nb, int precision);
+int gw_density(gw_plan* plan, int precision, const void* c, long ldc, long sc, int nb,
+               const double* weights, double* rho, void* workspace, size_t workspace_bytes, void* stream);
 /* projector phases (core/batch.py:229-235): p[k][q][g] = pf[k][q][g] exp(-i kpg[k][g] . tau[a(q)]),
  * 0 for g >= npw[k]. pf, p (nk, nproj, npw_max) complex128; kpg (nk, npw_max, 3) float64 A^-1;
  * positions (na, 3) A; atom_index (nproj,) int64; npw (nk,) int32 on the device. */

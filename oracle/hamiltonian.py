@@ -277,3 +277,16 @@ def toeplitz_local(bk: BatchedK, shape, v_eff_r: torch.Tensor, c: torch.Tensor) 
     M = vhat[idx]
     cm = c * bk.mask[:, None, :]
     return torch.einsum("kij,kbj->kbi", M, cm) * bk.mask[:, None, :]
+
+
+def density_b(coeffs: torch.Tensor, occ: torch.Tensor, kweights: torch.Tensor, bk: BatchedK, shape,
+              volume: float) -> torch.Tensor:
+    """core/batch.py:562-587 -- rho(r) = sum_k w_k sum_b f_kb |psi_kb(r)|^2 / volume, psi = N ifftn."""
+    nk, nb, m = coeffs.shape
+    n = shape[0] * shape[1] * shape[2]
+    box = torch.zeros(nk, nb, n, dtype=coeffs.dtype).scatter_add(2, bk.flat_idx[:, None, :].expand(nk, nb, m),
+                                                                   coeffs)
+    psi = torch.fft.ifftn(box.reshape(nk, nb, *shape), dim=(-3, -2, -1)) * n
+    w = kweights[:, None] * occ
+    rho = torch.einsum("kb,kbxyz->xyz", w.to(psi.real.dtype), psi.real ** 2 + psi.imag ** 2)
+    return rho / volume

@@ -228,3 +228,33 @@ def test_toeplitz_local_matches_oracle_and_fft(name, dtype):
     full = ht.apply(xd).cpu().to(torch.complex128)
     ref = ref_h(s, sel).apply(x)
     assert _rel(full, ref) < tol
+
+
+@pytest.mark.parametrize("name,nb", [("si2", 4), ("si8", 6), ("si64", 5), ("al1", 3)])
+def test_density_matches_oracle(name, nb):
+    """density_b (core/batch.py:562-587): fused wave-line density kernel on cubic
+    24/36/90 boxes, the transform-and-reduce path on other boxes (Al1, 18^3)."""
+    import numpy as np
+
+    from gradwave_amd.hamiltonian import density_b
+    from oracle import systems as S
+    from oracle.hamiltonian import density_b as ref_density
+
+    say(f"density {name} nb={nb}")
+    if name == "al1":
+        s = S.al1(20.0, np.array([[0.1, 0.2, -0.3], [0.0, 0.0, 0.0]]), nb)
+    else:
+        s = system(name)
+    sel = [0, 2] if name in ("si2", "si8") else None
+    cpu, gpu, _ = select_k(s, sel)
+    nk = cpu.npw.shape[0]
+    x = rand_block(nk, nb, cpu.mask.shape[1], cpu.mask, seed=6)
+    g = torch.Generator().manual_seed(2)
+    occ = torch.rand(nk, nb, dtype=torch.float64, generator=g)
+    kw = torch.rand(nk, dtype=torch.float64, generator=g)
+    want = ref_density(x, occ, kw, cpu, s.shape, s.grid.volume)
+    d = dev()
+    got = density_b(x.to(d), occ.to(d), kw.to(d), gpu, s.shape, s.grid.volume).cpu()
+    assert _rel(got, want) < 1e-12
+    got32 = density_b(x.to(d, torch.complex64), occ.to(d), kw.to(d), gpu, s.shape, s.grid.volume).cpu()
+    assert _rel(got32, want) < 2e-5

@@ -211,3 +211,20 @@ def test_cholqr2_spans_and_orthonormal():
     assert float((q.mH @ q - eye).abs().max()) < 1e-13
     # span: x = q (q^H x)
     assert float((q @ (q.mH @ x) - x).abs().max()) < 1e-12
+
+
+def test_density_integrates_to_weighted_norms(si2):
+    """Parseval: (volume / N) sum_r rho = sum_k w_k sum_b f_kb ||c_kb||^2 for the
+    oracle's density_b (core/batch.py:562-587)."""
+    from oracle.hamiltonian import density_b
+
+    g = torch.Generator().manual_seed(4)
+    bk = si2.bk
+    c = torch.randn(bk.nk, 3, bk.npw_max, dtype=torch.complex128, generator=g) * bk.mask[:, None, :]
+    occ = torch.rand(bk.nk, 3, dtype=torch.float64, generator=g)
+    kw = torch.as_tensor(si2.kweights)
+    rho = density_b(c, occ, kw, bk, si2.shape, si2.grid.volume)
+    n = si2.shape[0] * si2.shape[1] * si2.shape[2]
+    want = float((kw[:, None] * occ * (c.abs() ** 2).sum(-1)).sum())
+    assert abs(float(rho.sum()) * si2.grid.volume / n - want) < 1e-12 * want
+    assert float(rho.min()) >= 0.0
