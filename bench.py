@@ -300,7 +300,8 @@ def bench_ragged(args) -> None:
 
     t0 = time.perf_counter()
     radix = autotune.tune_radix(wls)
-    gemm_tile, gemm_wgs = autotune.tune_gemm(wls)  # ({"fp64": t, "fp32": t}, target)
+    # no rocBLAS back-projection here: the batched apply is captured in a HIP graph below
+    gemm_tile, gemm_wgs = autotune.tune_gemm(wls, allow_library=False)
     t_tune = time.perf_counter() - t0
     hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in wls]  # plans with the tuned splits
     batch = B.RaggedBatch(hams)

@@ -46,7 +46,9 @@ def current() -> dict:
     return {"radix": {str(n): a for n, a in radix.items() if a > 0},
             "gemm_wgs": lib.gw_tune_get(L.GW_TUNE_GEMM_WGS, 0),
             "gemm_tile": {"fp64": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP64),
-                          "fp32": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32)}}
+                          "fp32": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32)},
+            "combine_lib": {"fp64": lib.gw_tune_get(L.GW_TUNE_COMBINE_LIB, L.GW_FP64),
+                            "fp32": lib.gw_tune_get(L.GW_TUNE_COMBINE_LIB, L.GW_FP32)}}
 
 
 def apply(choices: dict) -> None:
@@ -58,6 +60,8 @@ def apply(choices: dict) -> None:
     for name, prec in (("fp64", L.GW_FP64), ("fp32", L.GW_FP32)):
         if name in choices.get("gemm_tile", {}):
             L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, int(choices["gemm_tile"][name])), "gw_tune_set")
+        if name in choices.get("combine_lib", {}):
+            L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, int(choices["combine_lib"][name])), "gw_tune_set")
 
 
 def load(path: str | None = None) -> dict | None:
@@ -124,11 +128,14 @@ def tune_radix(workloads, nb: int = 8, reps: int = 5) -> dict[int, int]:
 GEMM_TILES = (0, 1, 2, 3)  # MFMA blocks per wave: 2x2, 4x2, 2x4, 4x4
 
 
-def tune_gemm(workloads, nb: int = 32, reps: int = 5) -> tuple[dict, int]:
+def tune_gemm(workloads, nb: int = 32, reps: int = 5, allow_library: bool = True) -> tuple[dict, int]:
     """Time the projector products (nonlocal term) of ``workloads``: first each
-    MFMA tile shape per precision at the current split-K target, then each
-    split-K workgroup target (both precisions) with the best tiles; keep the
-    fastest. Returns ({"fp64": tile, "fp32": tile}, target)."""
+    MFMA tile shape per precision at the current split-K target (and, with
+    ``allow_library``, rocBLAS for the back-projection), then each split-K
+    workgroup target (both precisions) with the best choices; keep the fastest.
+    Returns ({"fp64": tile, "fp32": tile, "combine_lib": {...}}, target); tile
+    4 stands for the rocBLAS back-projection with the best tile for the inner
+    product."""
     from gradwave_amd.hamiltonian import BatchedHamiltonian
 
     lib = L.lib()
@@ -142,7 +149,9 @@ def tune_gemm(workloads, nb: int = 32, reps: int = 5) -> tuple[dict, int]:
         return [h.apply_terms(x, L.GW_TERM_NONLOCAL) for h, x in zip(hams, blocks[name])]
 
     tiles = {}
+    libs = {}
     for name, prec in (("fp64", L.GW_FP64), ("fp32", L.GW_FP32)):
+        L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 0), "gw_tune_set")
         best_tile, best_ms = 0, float("inf")
         for tile in GEMM_TILES:
             L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, tile), "gw_tune_set")
@@ -151,6 +160,13 @@ def tune_gemm(workloads, nb: int = 32, reps: int = 5) -> tuple[dict, int]:
                 best_tile, best_ms = tile, ms
         L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, best_tile), "gw_tune_set")
         tiles[name] = best_tile
+        libs[name] = 0
+        if allow_library:
+            L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 1), "gw_tune_set")
+            if _median_ms(lambda: run(name), reps) < best_ms:
+                libs[name] = 1
+            L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, libs[name]), "gw_tune_set")
+    tiles["combine_lib"] = libs
     best, best_ms = None, float("inf")
     for target in GEMM_TARGETS:
         L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, target), "gw_tune_set")

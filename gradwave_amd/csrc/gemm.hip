@@ -20,6 +20,9 @@
 // layout: A[i=l%16][k=l/16], B[k=l/16][j=l%16], D col l%16 and row Mfma<T>::row.
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
+
+#include <rocblas/rocblas.h>
 
 #include "gw_internal.h"
 
@@ -724,10 +727,55 @@ k_gemm_combine(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu, long
       }
 }
 
+// The combine is a plain complex GEMM, so it may also run on rocBLAS (the
+// autotuner picks per precision; default: the kernels above). Row-major
+// Z (A x G) = U'^T (A x J) Y (J x G) is column-major Z^T = Y^T U' with
+// Y^T read as stored (N) and U' = U^T or U^H (T / C).
+static int g_combine_lib[2] = {0, 0};
+void set_combine_lib(int precision, int v) { g_combine_lib[precision == GW_FP32] = v ? 1 : 0; }
+int combine_lib(int precision) { return g_combine_lib[precision == GW_FP32]; }
+
+static rocblas_handle blas_handle() {
+  static std::mutex mu;
+  static rocblas_handle handles[64] = {nullptr};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success) handles[dev] = nullptr;
+  return handles[dev];
+}
+
+template <typename T>
+static int combine_rocblas(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su, int conj_u,
+                           const cpx<T>* Y, long ldy, long sy, cpx<T>* Z, long ldz, long sz, T beta, hipStream_t st) {
+  rocblas_handle h = blas_handle();
+  GW_REQUIRE(h, GW_ERR_HIP, "rocblas handle unavailable");
+  GW_REQUIRE(rocblas_set_stream(h, st) == rocblas_status_success, GW_ERR_HIP, "rocblas_set_stream failed");
+  const rocblas_operation tb = conj_u ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
+  rocblas_status rs;
+  if constexpr (sizeof(T) == 8) {
+    const rocblas_double_complex one{1.0, 0.0}, b{(double)beta, 0.0};
+    rs = rocblas_zgemm_strided_batched(h, rocblas_operation_none, tb, G, A, J, &one,
+                                       (const rocblas_double_complex*)Y, (rocblas_int)ldy, sy,
+                                       (const rocblas_double_complex*)U, (rocblas_int)ldu, su, &b,
+                                       (rocblas_double_complex*)Z, (rocblas_int)ldz, sz, nk);
+  } else {
+    const rocblas_float_complex one{1.0f, 0.0f}, b{(float)beta, 0.0f};
+    rs = rocblas_cgemm_strided_batched(h, rocblas_operation_none, tb, G, A, J, &one,
+                                       (const rocblas_float_complex*)Y, (rocblas_int)ldy, sy,
+                                       (const rocblas_float_complex*)U, (rocblas_int)ldu, su, &b,
+                                       (rocblas_float_complex*)Z, (rocblas_int)ldz, sz, nk);
+  }
+  GW_REQUIRE(rs == rocblas_status_success, GW_ERR_HIP, "rocblas gemm failed (%d)", (int)rs);
+  return GW_OK;
+}
+
 template <typename T>
 int gemm_combine(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su, int conj_u, const cpx<T>* Y,
                  long ldy, long sy, cpx<T>* Z, long ldz, long sz, T beta, hipStream_t st) {
   if (A <= 0 || G <= 0 || nk <= 0) return GW_OK;
+  if (g_combine_lib[sizeof(T) == 4] && J > 0 && ldu < (1L << 31) && ldy < (1L << 31) && ldz < (1L << 31))
+    return combine_rocblas<T>(nk, A, G, J, U, ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, st);
   const int ntx = (2 * G + kTile - 1) / kTile, nty = (A + kTile - 1) / kTile;
   const long total = (long)ntx * nty * nk;
   auto launch_tile = [&](auto kern, int wm, int wn) {

@@ -130,6 +130,8 @@ void set_gemm_target_wgs(int v);
 int gemm_target_wgs();
 void set_gemm_tile(int precision, int v);
 int gemm_tile(int precision);
+void set_combine_lib(int precision, int v);
+int combine_lib(int precision);
 
 // Complex GEMM launchers (gemm.hip). All matrices complex interleaved, row-major.
 //   inner:   C[k](M x N) = alpha * X[k]^H-rows . Y[k]-rows  i.e. C[i][j] = sum_g conj(X[i][g]) Y[j][g]

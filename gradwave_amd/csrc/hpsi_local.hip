@@ -42,6 +42,10 @@ void set_error(const char* fmt, ...) {
 
 // Preferred first factor per box length, set by the autotuner (0: none).
 static int g_radix_pref[1024] = {0};
+// x-pass launch shape (wave-line path): waves per workgroup and bands per workgroup.
+static int g_xpass_waves = 4, g_xpass_bgroup = 8;
+static int xpass_waves() { return g_xpass_waves; }
+static int xpass_bgroup() { return g_xpass_bgroup; }
 
 bool split_length(int n, int* a, int* b) {
   if (n > 0 && n < 1024 && g_radix_pref[n] > 0) {
@@ -263,13 +267,13 @@ k_xpass(Geom g, int L, int nbt, cpx<T>* __restrict__ box, const T* __restrict__ 
 // lanes, the transpose between the two DFT passes stays inside the wave's own
 // LDS area, V(r) (natural x order) is held in registers across the band group,
 // and only the sphere's x-planes are loaded and stored.
-template <typename T, int N, int M, int S>
-__global__ void __launch_bounds__(256)
+template <typename T, int N, int M, int S, int NWX = 4>
+__global__ void __launch_bounds__(64 * NWX)
 k_xpass_wave(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __restrict__ vx,
              const cpx<T>* __restrict__ twx, int nchunk, long total) {
   using W = WaveLine<T, N, M, S>;
   constexpr int LW = W::LW;
-  __shared__ cpx<T> E[4 * W::E_PER_WAVE];
+  __shared__ cpx<T> E[NWX * W::E_PER_WAVE];
   __shared__ cpx<T> tw[N];
   // neighbouring line chunks share cache lines of every plane row: keep them on one XCD
   const long wid = xcd_work_id(total);
@@ -281,7 +285,7 @@ k_xpass_wave(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __r
   const int t = lane / LW, l = lane - t * LW;
   const bool part = t < S;
   const int n23 = g.n2 * g.n3;
-  const int j = (chunk * 4 + wave) * LW + l;
+  const int j = (chunk * NWX + wave) * LW + l;
   const bool jok = j < n23;
   cpx<T>* Ew = E + wave * W::E_PER_WAVE;
   // the sphere's x-planes are [0, xlo) U [xhi, N)
@@ -765,12 +769,21 @@ static int launch_wave_path_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, 
   tm.end(ST_PLANE_INV, ev);
   ev = tm.begin();
   const int n23 = N * N;
-  const int bgroup = 8;
-  const int lines_per_wg = 4 * W::LW;
+  const int bgroup = xpass_bgroup();
+  const int nwx = xpass_waves();
+  const int lines_per_wg = nwx * W::LW;
   const int nchunk = (n23 + lines_per_wg - 1) / lines_per_wg;
   const long xtotal = (long)nchunk * ((nbt + bgroup - 1) / bgroup);
-  hipLaunchKernelGGL((k_xpass_wave<T, N, C::M, C::S>), dim3((unsigned)(8 * ((xtotal + 7) / 8))), dim3(256), 0, st,
-                     g, nbt, bgroup, box, vn, tw_of<T>(&P, 0), nchunk, xtotal);
+  const dim3 xgrid((unsigned)(8 * ((xtotal + 7) / 8)));
+  if (nwx == 8)
+    hipLaunchKernelGGL((k_xpass_wave<T, N, C::M, C::S, 8>), xgrid, dim3(512), 0, st, g, nbt, bgroup, box, vn,
+                       tw_of<T>(&P, 0), nchunk, xtotal);
+  else if (nwx == 2)
+    hipLaunchKernelGGL((k_xpass_wave<T, N, C::M, C::S, 2>), xgrid, dim3(128), 0, st, g, nbt, bgroup, box, vn,
+                       tw_of<T>(&P, 0), nchunk, xtotal);
+  else
+    hipLaunchKernelGGL((k_xpass_wave<T, N, C::M, C::S, 4>), xgrid, dim3(256), 0, st, g, nbt, bgroup, box, vn,
+                       tw_of<T>(&P, 0), nchunk, xtotal);
   tm.end(ST_XPASS, ev);
   ev = tm.begin();
   if (nw == kPW) launch_planes<T, N, kPW>(false, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
@@ -1135,6 +1148,19 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
     set_gemm_target_wgs(value);
     return GW_OK;
   }
+  if (key == GW_TUNE_XPASS) {  // value = waves * 100 + bands, waves in {2, 4, 8}, bands in 1..64
+    const int w = value / 100, bgp = value % 100;
+    GW_REQUIRE((w == 2 || w == 4 || w == 8) && bgp >= 1 && bgp <= 64, GW_ERR_ARG, "bad x-pass shape %d", value);
+    g_xpass_waves = w;
+    g_xpass_bgroup = bgp;
+    return GW_OK;
+  }
+  if (key == GW_TUNE_COMBINE_LIB) {
+    GW_REQUIRE(arg == GW_FP64 || arg == GW_FP32, GW_ERR_ARG, "bad precision %d", arg);
+    GW_REQUIRE(value == 0 || value == 1, GW_ERR_ARG, "bad combine choice %d", value);
+    set_combine_lib(arg, value);
+    return GW_OK;
+  }
   if (key == GW_TUNE_GEMM_TILE) {
     GW_REQUIRE(value >= 0 && value <= 3, GW_ERR_ARG, "bad tile variant %d", value);
     GW_REQUIRE(arg == GW_FP64 || arg == GW_FP32, GW_ERR_ARG, "bad precision %d", arg);
@@ -1148,6 +1174,8 @@ extern "C" int gw_tune_get(int key, int arg) {
   if (key == GW_TUNE_RADIX) return (arg > 0 && arg < 1024) ? g_radix_pref[arg] : -1;
   if (key == GW_TUNE_GEMM_WGS) return gemm_target_wgs();
   if (key == GW_TUNE_GEMM_TILE) return (arg == GW_FP64 || arg == GW_FP32) ? gemm_tile(arg) : -1;
+  if (key == GW_TUNE_XPASS) return g_xpass_waves * 100 + g_xpass_bgroup;
+  if (key == GW_TUNE_COMBINE_LIB) return (arg == GW_FP64 || arg == GW_FP32) ? combine_lib(arg) : -1;
   return -1;
 }
 extern "C" int gw_version(void) { return 1; }

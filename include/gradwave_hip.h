@@ -195,8 +195,13 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
  * GW_TUNE_GEMM_WGS: the number of workgroups the split-K inner products aim at
  *   (`arg` ignored; 0 = default 2048).
  * GW_TUNE_GEMM_TILE: MFMA blocks per wave of the complex GEMMs of precision `arg`
- *   (GW_FP64 / GW_FP32), 0 = 2x2, 1 = 4x2, 2 = 2x4, 3 = 4x4. */
-enum { GW_TUNE_RADIX = 1, GW_TUNE_GEMM_WGS = 2, GW_TUNE_GEMM_TILE = 3 };
+ *   (GW_FP64 / GW_FP32), 0 = 2x2, 1 = 4x2, 2 = 2x4, 3 = 4x4.
+ * GW_TUNE_XPASS: launch shape of the wave-line x pass, value = waves per workgroup
+ *   (2, 4, 8) * 100 + bands per workgroup (1..64); default 408.
+ * GW_TUNE_COMBINE_LIB: for precision `arg`, 1 runs gw_zgemm_combine (and the
+ *   back-projection) on rocBLAS strided-batched GEMM, 0 on the kernels here (default). */
+enum { GW_TUNE_RADIX = 1, GW_TUNE_GEMM_WGS = 2, GW_TUNE_GEMM_TILE = 3, GW_TUNE_XPASS = 4,
+       GW_TUNE_COMBINE_LIB = 5 };
 int gw_tune_set(int key, int arg, int value);
 int gw_tune_get(int key, int arg);
 

@@ -127,8 +127,9 @@ def test_autotuner_choices_are_valid_and_exact(tmp_path, monkeypatch):
     say("autotune radix + split-K on 4 cells")
     wls = _instances()[:4]
     radix = autotune.tune_radix(wls, reps=2)
-    tiles, wgs = autotune.tune_gemm(wls, reps=2)
-    assert wgs in autotune.GEMM_TARGETS and all(t in autotune.GEMM_TILES for t in tiles.values())
+    tiles, wgs = autotune.tune_gemm(wls, reps=2, allow_library=False)
+    assert wgs in autotune.GEMM_TARGETS and tiles["fp64"] in autotune.GEMM_TILES
+    assert tiles["fp32"] in autotune.GEMM_TILES and tiles["combine_lib"] == {"fp64": 0, "fp32": 0}
     for n, a in radix.items():
         assert a in autotune.candidate_splits(n)
     hams = _hams(wls)

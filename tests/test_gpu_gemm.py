@@ -151,3 +151,28 @@ def test_tile_variants_match(tile, dtype):
             assert float((z.cpu().to(torch.complex128) - want).abs().max()) < TOL[dtype] * scale
     finally:
         lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, 0)
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_combine_on_rocblas_matches(dtype):
+    """The autotuner may move the combine (back-projection, Ritz) to rocBLAS."""
+    from gradwave_amd import _lib as L
+
+    say(f"combine on rocBLAS {dtype}")
+    lib = L.lib()
+    prec = L.precision_of(dtype)
+    try:
+        assert lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 1) == 0
+        for (nk, A, G, J) in [(1, 160, 37073, 256), (3, 33, 1000, 17), (2, 5, 4097, 70)]:
+            for conj_u, beta in ((False, 0.0), (True, 1.0)):
+                u, y = _rand(nk, J, A, seed=3, dtype=dtype), _rand(nk, J, G, seed=4, dtype=dtype)
+                z0 = _rand(nk, A, G, seed=5, dtype=dtype)
+                uu = u.to(torch.complex128)
+                want = torch.matmul((uu.conj() if conj_u else uu).mT, y.to(torch.complex128)) + beta * z0.to(
+                    torch.complex128)
+                z = z0.to(dev())
+                LA.combine(u.to(dev()), y.to(dev()), out=z, beta=beta, conj_u=conj_u)
+                scale = float(u.abs().pow(2).sum(-2).max().sqrt() * y.abs().pow(2).sum(-2).max().sqrt()) + 1.0
+                assert float((z.cpu().to(torch.complex128) - want).abs().max()) < TOL[dtype] * scale
+    finally:
+        lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 0)
