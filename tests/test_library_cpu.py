@@ -36,3 +36,33 @@ def test_missing_library_raises(monkeypatch, tmp_path):
     monkeypatch.setattr(L, "_lib", None)
     with pytest.raises(L.GWError, match="not built"):
         L.lib()
+
+
+@pytest.mark.skipif(not os.path.exists(L.LIB_PATH), reason="library not built")
+def test_argument_errors_are_reported_without_a_gpu():
+    """Entry points validate on the host before any launch and report through
+    gw_last_error (the reference raises ValueError/RuntimeError, e.g.
+    solvers/registry.py:87-94, grids.py:168-170)."""
+    import ctypes as C
+
+    lib = L.lib()
+
+    def err(rc):
+        assert rc != 0
+        return lib.gw_last_error().decode()
+
+    assert "unknown tuning key" in err(lib.gw_tune_set(99, 0, 0))
+    assert "does not split" in err(lib.gw_tune_set(L.GW_TUNE_RADIX, 24, 5))
+    assert "bad tile" in err(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32, 7))
+    assert "x-pass shape" in err(lib.gw_tune_set(L.GW_TUNE_XPASS, 0, 305))
+    assert lib.gw_tune_get(L.GW_TUNE_XPASS, 0) == 408
+    assert err(lib.gw_batch_create(C.byref(C.c_void_p()), None, 0))
+    assert err(lib.gw_toeplitz_build(0, 1, 0, 4, 4, 4, None, None, None, None, None))
+    assert err(lib.gw_gamma_pack(0, 0, 5, 5, None, 5, None, None, None, 5, None))
+    assert err(lib.gw_vloc_g(1, 8, None, None, None, None, 0.0, None, None))
+    assert err(lib.gw_proj_phase(1, 1, 1, None, None, None, None, None, None, None))
+    # a box length that no two codelets <= 16 can split (grids.py sizes are 7-smooth, 11 is not)
+    flat = (C.c_int64 * 1)(0)
+    npw = (C.c_int64 * 1)(1)
+    h = C.c_void_p()
+    assert "cannot be split" in err(lib.gw_plan_create(C.byref(h), 11, 11, 11, 1, 1, flat, npw))
