@@ -56,7 +56,7 @@ def parse():
 def main() -> None:
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
+        print(json.dumps({"error": "gradwave_amd runs on one GPU; --gpus > 1 is not supported "
                           "(k-point/instance sharding is a design note only)"}))
         sys.exit(2)
     import torch

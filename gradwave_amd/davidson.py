@@ -73,7 +73,7 @@ class BatchedDavidsonResult:
 def _fp32_expansion_active(x0: torch.Tensor) -> bool:
     """davidson.py:268-284 (the "auto" gate compares fp64/fp32 GEMM speed; on
     MI355X both MFMA peaks are within 2x, so "auto" stays off like the
-    reference's measured verdict for data-center fp64 hardware)."""
+    reference's measurement for data-center fp64 hardware)."""
     mode = os.environ.get("GRADWAVE_FP32_EXPANSION", "off").strip().lower()
     if mode not in ("auto", "on", "off"):
         raise ValueError(f"GRADWAVE_FP32_EXPANSION must be auto|on|off, got {mode!r}")

@@ -30,7 +30,7 @@ _TOEPLITZ_MODE = os.environ.get("GRADWAVE_TOEPLITZ", "auto").strip().lower()
 _TOEPLITZ_ON_GPU = os.environ.get("GW_TOEPLITZ_ON_GPU", "0") == "1"
 _TOEPLITZ_M_BUDGET_BYTES = int(os.environ.get("GW_TOEPLITZ_BUDGET", str(4 << 30)))
 _TOEP_TRIAL_MARGIN = 0.7
-_TOEP_VERDICT: dict[tuple, bool] = {}
+_TOEP_TRIAL: dict[tuple, bool] = {}
 
 
 def reset_happly_tally() -> None:
@@ -210,7 +210,7 @@ class BatchedHamiltonian:
         if not _TOEPLITZ_ON_GPU:
             return False
         key = (self.plan.nk, self.plan.npw_max, self.shape, c.dtype)
-        verdict = _TOEP_VERDICT.get(key)
+        verdict = _TOEP_TRIAL.get(key)
         if verdict is None:
             def timed(f):
                 f()
@@ -223,7 +223,7 @@ class BatchedHamiltonian:
             t_fft = timed(lambda: self.apply_terms(c, L.GW_TERM_LOCAL))
             t_toep = timed(lambda: self.local_toeplitz(c))
             verdict = t_toep < _TOEP_TRIAL_MARGIN * t_fft
-            _TOEP_VERDICT[key] = verdict
+            _TOEP_TRIAL[key] = verdict
             if not verdict:
                 self._toep_M.clear()
         return verdict

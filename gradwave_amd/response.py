@@ -13,7 +13,7 @@ into one compacted batch, and up to three true-residual rounds certify every
 column to ``tol``. The H applies are the HIP kernels; the occupied-space
 projections of the full block are MFMA contractions (``linalg.inner`` /
 ``combine``). Norm-conserving operators only (the S-metric branch of the
-reference is not part of this build).
+reference is not implemented here).
 """
 
 from __future__ import annotations

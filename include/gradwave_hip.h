@@ -14,7 +14,7 @@
  *
  * The reference (gradwave, pure Python/PyTorch) has no C ABI; each entry point
  * names the reference function whose tensor contract it implements. A
- * reference-side binding is the ctypes stub in INTEGRATION.md.
+ * reference-side binding is a ctypes stub that declares these signatures.
  */
 #ifndef GRADWAVE_HIP_H
 #define GRADWAVE_HIP_H
