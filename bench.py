@@ -159,6 +159,7 @@ def main() -> None:
                 "peak": MFMA_PEAK["fp32"], "unit": "TFLOP/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = profiled_traffic(args.workload, dom, "float")
+    roof["kernel_name"] = stage_kernel(dom, "float")
     roof["avg_ms"] = st32[dom]
     roof["algorithmic_per_launch"] = work
     roof["stage_ms_fp32"] = st32
@@ -504,6 +505,18 @@ STAGE_KERNEL = {"plane_inv": "k_plane_inv_w", "xpass": "k_xpass_wave", "plane_fw
 PROFILE_OF = {"si64_gamma": "hpsi_apply_si64_gamma_90cubed.txt"}
 
 
+def stage_kernel(stage: str, real: str) -> str:
+    """Kernel that runs a stage: the x pass is k_xpass_stage when the launch shape
+    of that precision stages the planes through LDS (mode digit of GW_TUNE_XPASS)."""
+    if stage == "xpass":
+        from gradwave_amd import _lib as L
+
+        prec = L.GW_FP32 if real == "float" else L.GW_FP64
+        if L.lib().gw_tune_get(L.GW_TUNE_XPASS, prec) // 10000 == 1:
+            return "k_xpass_stage"
+    return STAGE_KERNEL[stage]
+
+
 def profiled_traffic(workload: str, stage: str, real: str):
     """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE) of the stage's
     kernel, read from the rocprofv3 summary committed under profiles/, or None."""
@@ -513,7 +526,7 @@ def profiled_traffic(workload: str, stage: str, real: str):
     path = os.path.join(HERE, "profiles", name)
     if not os.path.exists(path):
         return None
-    key = f"gw::{STAGE_KERNEL[stage]}<{real}"
+    key = f"gw::{stage_kernel(stage, real)}<{real}"
     for line in open(path):
         if line.startswith(key):
             f = line.split()

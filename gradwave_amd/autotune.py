@@ -11,6 +11,9 @@ the GEMMs (no reference counterpart: the reference uses torch.fft and BLAS).
   ``gw_rgemm_inner`` split the long axis until the launch has about
   ``GW_TUNE_GEMM_WGS`` workgroups; ``tune_gemm`` times the projector products
   of real operators for each tile shape and target.
+* x pass of the wave-line boxes (24, 36, 90, 120 cubed): lines straight from
+  HBM or planes staged through LDS, waves and bands per workgroup
+  (``GW_TUNE_XPASS``, per precision); ``tune_xpass`` times the local term.
 
 Choices persist as JSON (``GW_TUNE_FILE``, default
 ``~/.cache/gradwave_amd/tune.json``) and are loaded with the library.
@@ -48,7 +51,9 @@ def current() -> dict:
             "gemm_tile": {"fp64": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP64),
                           "fp32": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32)},
             "combine_lib": {"fp64": lib.gw_tune_get(L.GW_TUNE_COMBINE_LIB, L.GW_FP64),
-                            "fp32": lib.gw_tune_get(L.GW_TUNE_COMBINE_LIB, L.GW_FP32)}}
+                            "fp32": lib.gw_tune_get(L.GW_TUNE_COMBINE_LIB, L.GW_FP32)},
+            "xpass": {"fp64": lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP64),
+                      "fp32": lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP32)}}
 
 
 def apply(choices: dict) -> None:
@@ -62,6 +67,8 @@ def apply(choices: dict) -> None:
             L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, int(choices["gemm_tile"][name])), "gw_tune_set")
         if name in choices.get("combine_lib", {}):
             L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, int(choices["combine_lib"][name])), "gw_tune_set")
+        if name in choices.get("xpass", {}):
+            L.check(lib.gw_tune_set(L.GW_TUNE_XPASS, prec, int(choices["xpass"][name])), "gw_tune_set")
 
 
 def load(path: str | None = None) -> dict | None:
@@ -175,3 +182,36 @@ def tune_gemm(workloads, nb: int = 32, reps: int = 5, allow_library: bool = True
             best, best_ms = target, ms
     L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, best), "gw_tune_set")
     return tiles, best
+
+
+# x-pass shapes: mode * 10000 + waves * 100 + bands (mode 1: planes staged through LDS)
+XPASS_SHAPES = (408, 416, 208, 10408, 10416, 10432, 10208)
+WAVE_LENGTHS = (24, 36, 90, 120)
+
+
+def tune_xpass(workloads, nb: int = 32, reps: int = 5) -> dict:
+    """For each precision, time the local term of the ``workloads`` whose box is a
+    wave-line cube under each x-pass shape and keep the fastest. Returns
+    {"fp64": shape, "fp32": shape} (the current shapes if no workload qualifies)."""
+    from gradwave_amd.hamiltonian import BatchedHamiltonian
+
+    lib = L.lib()
+    users = [w for w in workloads if len(set(w.shape)) == 1 and w.shape[0] in WAVE_LENGTHS]
+    chosen = {"fp64": lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP64),
+              "fp32": lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP32)}
+    if not users:
+        return chosen
+    hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in users]
+    g = torch.Generator().manual_seed(5)
+    b64 = [(torch.randn(w.bk.nk, nb, w.bk.npw_max, dtype=torch.complex128, generator=g).to(w.v_r.device)
+            * w.bk.mask[:, None, :]) for w in users]
+    for name, prec, blocks in (("fp64", L.GW_FP64, b64), ("fp32", L.GW_FP32, [b.to(torch.complex64) for b in b64])):
+        best, best_ms = chosen[name], float("inf")
+        for shape in XPASS_SHAPES:
+            L.check(lib.gw_tune_set(L.GW_TUNE_XPASS, prec, shape), "gw_tune_set")
+            ms = _median_ms(lambda: [h.apply_terms(x, L.GW_TERM_LOCAL) for h, x in zip(hams, blocks)], reps)
+            if ms < best_ms:
+                best, best_ms = shape, ms
+        L.check(lib.gw_tune_set(L.GW_TUNE_XPASS, prec, best), "gw_tune_set")
+        chosen[name] = best
+    return chosen

@@ -26,20 +26,20 @@ template <> struct cpx_t<double> { using type = double2; };
 template <> struct cpx_t<float> { using type = float2; };
 template <typename T> using cpx = typename cpx_t<T>::type;
 
-template <typename T> __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+template <typename T> __host__ __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
   cpx<T> r;
   r.x = a.x * b.x - a.y * b.y;
   r.y = a.x * b.y + a.y * b.x;
   return r;
 }
-template <typename T> __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) {
+template <typename T> __host__ __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) {
   cpx<T> r; r.x = a.x + b.x; r.y = a.y + b.y; return r;
 }
-template <typename T> __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) {
+template <typename T> __host__ __device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) {
   cpx<T> r; r.x = a.x - b.x; r.y = a.y - b.y; return r;
 }
 // multiply by the conjugate-or-not twiddle w (w = e^{+2 pi i t/n} from the table)
-template <int SIGN, typename T> __device__ __forceinline__ cpx<T> twmul(cpx<T> a, cpx<T> w) {
+template <int SIGN, typename T> __host__ __device__ __forceinline__ cpx<T> twmul(cpx<T> a, cpx<T> w) {
   cpx<T> r;
   if constexpr (SIGN > 0) {
     r.x = a.x * w.x - a.y * w.y;
@@ -99,9 +99,25 @@ constexpr int smallest_factor(int n) {
   return n;
 }
 
+constexpr int gcd_c(int a, int b) { return b == 0 ? a : gcd_c(b, a % b); }
+
+// Largest P < n with n = P*Q, gcd(P, Q) = 1, Q > 1; 0 for prime powers.
+constexpr int coprime_split(int n) {
+  for (int p = n - 1; p > 1; --p)
+    if (n % p == 0 && gcd_c(p, n / p) == 1) return p;
+  return 0;
+}
+
+// x with (a * x) % m == 1, for gcd(a, m) = 1
+constexpr int inv_mod(int a, int m) {
+  for (int x = 1; x < m; ++x)
+    if ((a * x) % m == 1) return x;
+  return m == 1 ? 0 : -1;
+}
+
 // In-place DFT of N complex values in registers, natural order in and out:
 // X[k] = sum_j a[j] exp(SIGN * 2 pi i j k / N).
-template <int N, int SIGN, typename T> __device__ __forceinline__ void dft(cpx<T>* a) {
+template <int N, int SIGN, typename T> __host__ __device__ __forceinline__ void dft(cpx<T>* a) {
   if constexpr (N == 1) {
     return;
   } else if constexpr (N == 2) {
@@ -155,6 +171,32 @@ template <int N, int SIGN, typename T> __device__ __forceinline__ void dft(cpx<T
     }
 #pragma unroll
     for (int k = 0; k < N; ++k) a[k] = out[k];
+  } else if constexpr (coprime_split(N) > 0) {
+    // Good-Thomas prime-factor form, no twiddles: input j = (Q*j1 + P*j2) mod N,
+    // output k = CRT(k1 mod P, k2 mod Q) = (k1*Q*(Q^-1 mod P) + k2*P*(P^-1 mod Q)) mod N.
+    constexpr int P = coprime_split(N);
+    constexpr int Q = N / P;
+    constexpr int CP = Q * inv_mod(Q % P, P);
+    constexpr int CQ = P * inv_mod(P % Q, Q);
+    cpx<T> b[N];
+#pragma unroll
+    for (int j2 = 0; j2 < Q; ++j2) {
+      cpx<T> t[P];
+#pragma unroll
+      for (int j1 = 0; j1 < P; ++j1) t[j1] = a[(Q * j1 + P * j2) % N];
+      dft<P, SIGN, T>(t);
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) b[Q * k1 + j2] = t[k1];
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+      cpx<T> t[Q];
+#pragma unroll
+      for (int j2 = 0; j2 < Q; ++j2) t[j2] = b[Q * k1 + j2];
+      dft<Q, SIGN, T>(t);
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) a[(CP * k1 + CQ * k2) % N] = t[k2];
+    }
   } else {
     constexpr int P = smallest_factor(N);
     constexpr int Q = N / P;

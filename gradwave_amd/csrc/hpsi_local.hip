@@ -42,10 +42,15 @@ void set_error(const char* fmt, ...) {
 
 // Preferred first factor per box length, set by the autotuner (0: none).
 static int g_radix_pref[1024] = {0};
-// x-pass launch shape (wave-line path): waves per workgroup and bands per workgroup.
-static int g_xpass_waves = 4, g_xpass_bgroup = 8;
-static int xpass_waves() { return g_xpass_waves; }
-static int xpass_bgroup() { return g_xpass_bgroup; }
+// x-pass launch shape of the wave-line path, per precision (0 = FP64, 1 = FP32):
+// mode (0 = lines straight from HBM, 1 = planes staged through LDS), waves per
+// workgroup and bands per workgroup.
+struct XpassShape { int mode, waves, bgroup; };
+// Defaults from timing Si64 (90^3) and Si216 (120^3) Gamma applies: FP32 gains from
+// staging (-16% / -12%), FP64 (twice the bytes per lane, half the tiles per CU) does not.
+static XpassShape g_xpass[2] = {{0, 4, 8}, {1, 4, 16}};
+static int xp_index(int precision) { return precision == GW_FP64 ? 0 : 1; }
+static int xpass_encode(const XpassShape& x) { return x.mode * 10000 + x.waves * 100 + x.bgroup; }
 
 bool split_length(int n, int* a, int* b) {
   if (n > 0 && n < 1024 && g_radix_pref[n] > 0) {
@@ -157,6 +162,30 @@ static Geom geom_of(const Plan* p, int nb) {
 }
 
 template <typename T> __device__ __forceinline__ cpx<T> czero() { cpx<T> z; z.x = 0; z.y = 0; return z; }
+
+// Range-checked buffer access (raw buffer resource, stride 0): a lane whose byte
+// offset lies past `bytes` reads zero and its store is discarded by the memory
+// unit, so masked loads and stores need no branch and every lane issues the
+// same instruction stream (the wait counts before dependent uses stay exact).
+// The base must be wave-uniform; bytes < 2^31.
+constexpr int kBufOff = 0x7fffff00;  // offset of a masked lane
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+using u32x2 = unsigned int __attribute__((ext_vector_type(2)));
+using u32x4 = unsigned int __attribute__((ext_vector_type(4)));
+template <typename T> __device__ __forceinline__ cpx<T> buf_load(__amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (sizeof(cpx<T>) == 8)
+    return __builtin_bit_cast(cpx<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  else
+    return __builtin_bit_cast(cpx<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+template <typename T> __device__ __forceinline__ void buf_store(cpx<T> v, __amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (sizeof(cpx<T>) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, off, 0, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, 0);
+}
 
 // Linear workgroup id -> work id so that consecutive work ids share an XCD
 // (dispatch places workgroup L on XCD L % 8). Returns -1 for the padding ids.
@@ -292,36 +321,164 @@ k_xpass_wave(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __r
   auto slot = [&](int i0) { return i0 < g.xlo ? i0 : (i0 >= g.xhi ? g.xlo + i0 - g.xhi : -1); };
   T vr[S];
 #pragma unroll
-  for (int s = 0; s < S; ++s) vr[s] = (t < M && jok) ? vx[(long)(t + M * s) * n23 + j] : T(0);
-  auto load = [&](int bi, cpx<T> (&v)[M]) {
-    const cpx<T>* bb = box + (long)bi * g.na * n23 + j;
+  for (int s = 0; s < S; ++s) vr[s] = (t < M && jok) ? vx[(long)W::rpos(t, s) * n23 + j] : T(0);
+  // byte offsets of this lane's elements inside one band's planes (kBufOff: none)
+  int off[M];
 #pragma unroll
-    for (int q = 0; q < M; ++q) {
-      const int sl = part ? slot(S * q + t) : -1;
-      v[q] = (jok && sl >= 0) ? bb[(long)sl * n23] : czero<T>();
-    }
+  for (int q = 0; q < M; ++q) {
+    const int sl = part ? slot(W::gpos(t, q)) : -1;
+    off[q] = (jok && sl >= 0) ? (sl * n23 + j) * (int)sizeof(cpx<T>) : kBufOff;
+  }
+  const unsigned band_bytes = (unsigned)((long)g.na * n23 * sizeof(cpx<T>));
+  auto load = [&](int bi, cpx<T> (&v)[M]) {
+    const auto r = buf_rsrc(box + (long)bi * g.na * n23, band_bytes);
+#pragma unroll
+    for (int q = 0; q < M; ++q) v[q] = buf_load<T>(r, off[q]);
   };
   const int b0 = bgi * bgroup;
   const int nbg = min(bgroup, nbt - b0);
+  if (nbg <= 0) return;
   cpx<T> x[M], xn[M], y[S];
-  if (nbg > 0) load(b0, xn);
+  // The first band's loads complete before the loop, so no load is pending at
+  // the loop head; inside, the next band's loads (an empty range after the last
+  // band: zeros, no traffic) overlap this band's transforms, and the copy at the
+  // end waits for them only, not for this band's stores.
+  load(b0, x);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
   for (int bg = 0; bg < nbg; ++bg) {
     const int bi = b0 + bg;
+    {
+      const bool more = bg + 1 < nbg;
+      const auto rn = buf_rsrc(box + (long)(more ? bi + 1 : bi) * g.na * n23, more ? band_bytes : 0u);
 #pragma unroll
-    for (int q = 0; q < M; ++q) x[q] = xn[q];
-    if (bg + 1 < nbg) load(bi + 1, xn);  // next band's loads overlap this band's transforms
-#pragma unroll
-    for (int s = 0; s < S; ++s) y[s] = czero<T>();
+      for (int q = 0; q < M; ++q) xn[q] = buf_load<T>(rn, off[q]);
+    }
     W::inverse(x, y, t, l, part, Ew, tw);
 #pragma unroll
     for (int s = 0; s < S; ++s) { y[s].x *= vr[s]; y[s].y *= vr[s]; }
     W::forward(y, x, t, l, part, Ew, tw);
-    cpx<T>* bb = box + (long)bi * g.na * n23 + j;
+    const auto r = buf_rsrc(box + (long)bi * g.na * n23, band_bytes);
 #pragma unroll
-    for (int q = 0; q < M; ++q) {
-      const int sl = part ? slot(S * q + t) : -1;
-      if (jok && sl >= 0) bb[(long)sl * n23] = x[q];
+    for (int q = 0; q < M; ++q) buf_store<T>(x[q], r, off[q]);
+#pragma unroll
+    for (int q = 0; q < M; ++q) x[q] = xn[q];
+  }
+}
+
+// P2 with the band's sphere planes staged through LDS. A workgroup of NWX waves
+// owns a chunk of LC = NWX*LW yz-lines; one band's na planes of that chunk form
+// a tile of na rows x LC contiguous elements, which arrives with row-contiguous
+// range-checked loads (each wave instruction covers LC-element row segments
+// instead of S planes x LW elements) and leaves the same way. Each wave reads its
+// LW lines out of the tile (a zero row stands in for the planes outside the
+// sphere), transforms them in registers as in k_xpass_wave and writes them back
+// in place (a discard row takes the outside planes). Two tiles alternate, so the
+// next band's loads are in flight during this band's transforms.
+template <typename T, int N, int M, int S, int NWX>
+__global__ void __launch_bounds__(64 * NWX)
+k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __restrict__ vx,
+              const cpx<T>* __restrict__ twx, int nchunk, long total) {
+  using W = WaveLine<T, N, M, S>;
+  constexpr int LW = W::LW;
+  constexpr int LC = NWX * LW;                  // lines per workgroup
+  constexpr int NT = 64 * NWX;
+  constexpr int KS = (N * LC + NT - 1) / NT;    // staged elements per thread (na <= N rows)
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* E = reinterpret_cast<cpx<T>*>(smem_raw);
+  cpx<T>* tw = E + NWX * W::E_PER_WAVE;
+  cpx<T>* tiles = tw + N;                        // 2 x (na + 2) rows of LC
+  const long wid = xcd_work_id(total);
+  if (wid < 0) return;
+  const int chunk = (int)(wid % nchunk), bgi = (int)(wid / nchunk);
+  const int b0 = bgi * bgroup;
+  const int nbg = min(bgroup, nbt - b0);
+  if (nbg <= 0) return;
+  const int na = g.na, n23 = g.n2 * g.n3;
+  const int tstride = (na + 2) * LC;             // one tile: rows 0..na-1, zero row na, discard row na+1
+  if constexpr (!W::PFA) {
+    for (int i = threadIdx.x; i < N; i += NT) tw[i] = twx[i];
+  }
+  for (int i = threadIdx.x; i < LC; i += NT) {
+    tiles[na * LC + i] = czero<T>();
+    tiles[tstride + na * LC + i] = czero<T>();
+  }
+  const int j0 = chunk * LC;
+  // staging map: thread element k is tile entry e = threadIdx.x + k*NT = a*LC + jj
+  // <-> box[band][a][j0 + jj]; entries past the tile or the last line are masked
+  const int ne = na * LC;
+  int soff[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int e = threadIdx.x + k * NT;
+    const int a = e / LC, jj = e - a * LC;
+    soff[k] = (e < ne && j0 + jj < n23) ? (a * n23 + j0 + jj) * (int)sizeof(cpx<T>) : kBufOff;
+  }
+  const unsigned band_bytes = (unsigned)((long)na * n23 * sizeof(cpx<T>));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int t = lane / LW, l = lane - t * LW;
+  const bool part = t < S;
+  const int col = wave * LW + l;
+  const bool jok = j0 + col < n23;
+  cpx<T>* Ew = E + wave * W::E_PER_WAVE;
+  auto slot = [&](int i0) { return i0 < g.xlo ? i0 : (i0 >= g.xhi ? g.xlo + i0 - g.xhi : -1); };
+  // tile entries of this lane's elements: read from the zero row, write to the
+  // discard row for planes outside the sphere (and idle lanes)
+  int rd[M], wr[M];
+#pragma unroll
+  for (int q = 0; q < M; ++q) {
+    const int sl = part ? slot(W::gpos(t, q)) : -1;
+    rd[q] = (sl >= 0 ? sl : na) * LC + min(col, LC - 1);
+    wr[q] = (sl >= 0 ? sl : na + 1) * LC + min(col, LC - 1);
+  }
+  T vr[S];
+#pragma unroll
+  for (int s = 0; s < S; ++s) vr[s] = (t < M && jok) ? vx[(long)W::rpos(t, s) * n23 + j0 + col] : T(0);
+  cpx<T> st[KS];
+  {
+    const auto r = buf_rsrc(box + (long)b0 * na * n23, band_bytes);
+#pragma unroll
+    for (int k = 0; k < KS; ++k) st[k] = buf_load<T>(r, soff[k]);
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ne) tiles[e] = st[k];
     }
+  }
+  __syncthreads();
+  cpx<T> x[M], y[S];
+  for (int bg = 0; bg < nbg; ++bg) {
+    const int bi = b0 + bg;
+    cpx<T>* cur = tiles + (bg & 1) * tstride;
+    cpx<T>* nxt = tiles + ((bg + 1) & 1) * tstride;
+    {  // next band's tile entries (an empty range after the last band: zeros, no traffic)
+      const bool more = bg + 1 < nbg;
+      const auto r = buf_rsrc(box + (long)(more ? bi + 1 : bi) * na * n23, more ? band_bytes : 0u);
+#pragma unroll
+      for (int k = 0; k < KS; ++k) st[k] = buf_load<T>(r, soff[k]);
+    }
+#pragma unroll
+    for (int q = 0; q < M; ++q) x[q] = cur[rd[q]];
+    W::inverse(x, y, t, l, part, Ew, tw);
+#pragma unroll
+    for (int s = 0; s < S; ++s) { y[s].x *= vr[s]; y[s].y *= vr[s]; }
+    W::forward(y, x, t, l, part, Ew, tw);
+#pragma unroll
+    for (int q = 0; q < M; ++q) cur[wr[q]] = x[q];
+    __syncthreads();
+    {
+      const auto r = buf_rsrc(box + (long)bi * na * n23, band_bytes);
+#pragma unroll
+      for (int k = 0; k < KS; ++k) {
+        const int e = threadIdx.x + k * NT;
+        buf_store<T>(cur[min(e, ne - 1)], r, soff[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KS; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < ne) nxt[e] = st[k];
+    }
+    __syncthreads();
   }
 }
 
@@ -596,13 +753,11 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
     const int row = r0 + l;
     const bool ok = part && row < ns;
 #pragma unroll
-    for (int q = 0; q < M; ++q) x[q] = ok ? rows[row * N + S * q + t] : czero<T>();
-#pragma unroll
-    for (int u = 0; u < S; ++u) y[u] = czero<T>();
+    for (int q = 0; q < M; ++q) x[q] = ok ? rows[row * N + W::gpos(t, q)] : czero<T>();
     W::inverse(x, y, t, l, part, Ew, tw);
     if (t < M && row < ns) {
 #pragma unroll
-      for (int u = 0; u < S; ++u) rows[row * N + t + M * u] = y[u];
+      for (int u = 0; u < S; ++u) rows[row * N + W::rpos(t, u)] = y[u];
     }
   }
   __syncthreads();
@@ -613,15 +768,13 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
     const bool ok = part && col < N;
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-      const int st = ok ? map[S * q + t] : -1;
+      const int st = ok ? map[W::gpos(t, q)] : -1;
       x[q] = st >= 0 ? rows[st * N + col] : czero<T>();
     }
-#pragma unroll
-    for (int u = 0; u < S; ++u) y[u] = czero<T>();
     W::inverse(x, y, t, l, part, Ew, tw);
     if (t < M && col < N) {
 #pragma unroll
-      for (int u = 0; u < S; ++u) dst[(long)(t + M * u) * N + col] = y[u];
+      for (int u = 0; u < S; ++u) dst[(long)W::rpos(t, u) * N + col] = y[u];
     }
   }
 }
@@ -666,12 +819,12 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
     const int col = c0 + l;
     const bool ok = col < N;
 #pragma unroll
-    for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? srcp[(long)(t + M * u) * N + col] : czero<T>();
+    for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? srcp[(long)W::rpos(t, u) * N + col] : czero<T>();
     W::forward(y, x, t, l, part, Ew, tw);
     if (ok && part) {
 #pragma unroll
       for (int q = 0; q < M; ++q) {
-        const int st = map[S * q + t];
+        const int st = map[W::gpos(t, q)];
         if (st >= 0) rows[st * N + col] = x[q];
       }
     }
@@ -681,11 +834,11 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
     const int row = r0 + l;
     const bool ok = row < ns;
 #pragma unroll
-    for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? rows[row * N + t + M * u] : czero<T>();
+    for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? rows[row * N + W::rpos(t, u)] : czero<T>();
     W::forward(y, x, t, l, part, Ew, tw);
     if (ok && part) {
 #pragma unroll
-      for (int q = 0; q < M; ++q) rows[row * N + S * q + t] = x[q];
+      for (int q = 0; q < M; ++q) rows[row * N + W::gpos(t, q)] = x[q];
     }
   }
   __syncthreads();
@@ -763,19 +916,35 @@ static int launch_wave_path_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, 
   const long total = (long)P.na * nbt;
   const int nw = plane_waves<T, N>(P);
   GW_REQUIRE(nw > 0, GW_ERR_UNSUPPORTED, "x-plane sticks exceed LDS");
+  // the x-pass addresses one band's planes through a buffer resource
+  GW_REQUIRE((long)P.na * N * N * (long)sizeof(cpx<T>) <= (long)kBufOff, GW_ERR_UNSUPPORTED,
+             "one band's x-planes exceed 2 GiB");
   hipEvent_t ev = tm.begin();
   if (nw == kPW) launch_planes<T, N, kPW>(true, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
   else launch_planes<T, N, 4>(true, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
   tm.end(ST_PLANE_INV, ev);
   ev = tm.begin();
   const int n23 = N * N;
-  const int bgroup = xpass_bgroup();
-  const int nwx = xpass_waves();
+  const XpassShape xs = g_xpass[xp_index(sizeof(T) == 8 ? GW_FP64 : GW_FP32)];
+  const int bgroup = xs.bgroup;
+  const int nwx = xs.waves;
   const int lines_per_wg = nwx * W::LW;
   const int nchunk = (n23 + lines_per_wg - 1) / lines_per_wg;
   const long xtotal = (long)nchunk * ((nbt + bgroup - 1) / bgroup);
   const dim3 xgrid((unsigned)(8 * ((xtotal + 7) / 8)));
-  if (nwx == 8)
+  const size_t stage_lds =
+      ((size_t)nwx * W::E_PER_WAVE + N + 2 * (size_t)(P.na + 2) * lines_per_wg) * sizeof(cpx<T>);
+  if (xs.mode == 1 && stage_lds <= (size_t)std::min(P.lds_max, kMaxLds)) {
+    if (nwx == 8)
+      hipLaunchKernelGGL((k_xpass_stage<T, N, C::M, C::S, 8>), xgrid, dim3(512), stage_lds, st, g, nbt, bgroup,
+                         box, vn, tw_of<T>(&P, 0), nchunk, xtotal);
+    else if (nwx == 2)
+      hipLaunchKernelGGL((k_xpass_stage<T, N, C::M, C::S, 2>), xgrid, dim3(128), stage_lds, st, g, nbt, bgroup,
+                         box, vn, tw_of<T>(&P, 0), nchunk, xtotal);
+    else
+      hipLaunchKernelGGL((k_xpass_stage<T, N, C::M, C::S, 4>), xgrid, dim3(256), stage_lds, st, g, nbt, bgroup,
+                         box, vn, tw_of<T>(&P, 0), nchunk, xtotal);
+  } else if (nwx == 8)
     hipLaunchKernelGGL((k_xpass_wave<T, N, C::M, C::S, 8>), xgrid, dim3(512), 0, st, g, nbt, bgroup, box, vn,
                        tw_of<T>(&P, 0), nchunk, xtotal);
   else if (nwx == 2)
@@ -830,19 +999,15 @@ k_xpass_adjv_wave(Geom g, int nbt, int bgroup, const cpx<T>* __restrict__ boxc, 
     cpx<T> x[M], yc[S], yg[S];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-      const int sl = part_ ? slot(S * q + t) : -1;
+      const int sl = part_ ? slot(W::gpos(t, q)) : -1;
       x[q] = (jok && sl >= 0) ? boxc[off + (long)sl * n23] : czero<T>();
     }
-#pragma unroll
-    for (int s = 0; s < S; ++s) yc[s] = czero<T>();
     W::inverse(x, yc, t, l, part_, Ew, tw);
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-      const int sl = part_ ? slot(S * q + t) : -1;
+      const int sl = part_ ? slot(W::gpos(t, q)) : -1;
       x[q] = (jok && sl >= 0) ? boxg[off + (long)sl * n23] : czero<T>();
     }
-#pragma unroll
-    for (int s = 0; s < S; ++s) yg[s] = czero<T>();
     W::inverse(x, yg, t, l, part_, Ew, tw);
 #pragma unroll
     for (int s = 0; s < S; ++s) acc[s] += (double)yg[s].x * yc[s].x + (double)yg[s].y * yc[s].y;
@@ -850,7 +1015,7 @@ k_xpass_adjv_wave(Geom g, int nbt, int bgroup, const cpx<T>* __restrict__ boxc, 
   if (t < M && jok) {
     double* pp = part + (long)bgi * N * n23;
 #pragma unroll
-    for (int s = 0; s < S; ++s) pp[(long)(t + M * s) * n23 + j] = acc[s];
+    for (int s = 0; s < S; ++s) pp[(long)W::rpos(t, s) * n23 + j] = acc[s];
   }
 }
 
@@ -930,11 +1095,9 @@ k_xpass_density_wave(Geom g, int nbt, int bgroup, const cpx<T>* __restrict__ box
     cpx<T> x[M], y[S];
 #pragma unroll
     for (int q = 0; q < M; ++q) {
-      const int sl = part_ ? slot(S * q + t) : -1;
+      const int sl = part_ ? slot(W::gpos(t, q)) : -1;
       x[q] = (jok && sl >= 0) ? box[off + (long)sl * n23] : czero<T>();
     }
-#pragma unroll
-    for (int s = 0; s < S; ++s) y[s] = czero<T>();
     W::inverse(x, y, t, l, part_, Ew, tw);
 #pragma unroll
     for (int s = 0; s < S; ++s) acc[s] += wb * ((double)y[s].x * y[s].x + (double)y[s].y * y[s].y);
@@ -942,7 +1105,7 @@ k_xpass_density_wave(Geom g, int nbt, int bgroup, const cpx<T>* __restrict__ box
   if (t < M && jok) {
     double* pp = part + (long)bgi * N * n23;
 #pragma unroll
-    for (int s = 0; s < S; ++s) pp[(long)(t + M * s) * n23 + j] = acc[s];
+    for (int s = 0; s < S; ++s) pp[(long)W::rpos(t, s) * n23 + j] = acc[s];
   }
 }
 
@@ -1148,11 +1311,13 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
     set_gemm_target_wgs(value);
     return GW_OK;
   }
-  if (key == GW_TUNE_XPASS) {  // value = waves * 100 + bands, waves in {2, 4, 8}, bands in 1..64
-    const int w = value / 100, bgp = value % 100;
-    GW_REQUIRE((w == 2 || w == 4 || w == 8) && bgp >= 1 && bgp <= 64, GW_ERR_ARG, "bad x-pass shape %d", value);
-    g_xpass_waves = w;
-    g_xpass_bgroup = bgp;
+  if (key == GW_TUNE_XPASS) {  // value = mode * 10000 + waves * 100 + bands
+    const int mode = value / 10000, w = (value / 100) % 100, bgp = value % 100;
+    GW_REQUIRE((mode == 0 || mode == 1) && (w == 2 || w == 4 || w == 8) && bgp >= 1 && bgp <= 64, GW_ERR_ARG,
+               "bad x-pass shape %d", value);
+    const XpassShape xs{mode, w, bgp};
+    if (arg == GW_FP64 || arg == GW_FP32) g_xpass[xp_index(arg)] = xs;
+    else g_xpass[0] = g_xpass[1] = xs;
     return GW_OK;
   }
   if (key == GW_TUNE_COMBINE_LIB) {
@@ -1174,7 +1339,7 @@ extern "C" int gw_tune_get(int key, int arg) {
   if (key == GW_TUNE_RADIX) return (arg > 0 && arg < 1024) ? g_radix_pref[arg] : -1;
   if (key == GW_TUNE_GEMM_WGS) return gemm_target_wgs();
   if (key == GW_TUNE_GEMM_TILE) return (arg == GW_FP64 || arg == GW_FP32) ? gemm_tile(arg) : -1;
-  if (key == GW_TUNE_XPASS) return g_xpass_waves * 100 + g_xpass_bgroup;
+  if (key == GW_TUNE_XPASS) return xpass_encode(g_xpass[xp_index(arg == GW_FP64 ? GW_FP64 : GW_FP32)]);
   if (key == GW_TUNE_COMBINE_LIB) return (arg == GW_FP64 || arg == GW_FP32) ? combine_lib(arg) : -1;
   return -1;
 }

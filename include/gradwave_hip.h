@@ -196,8 +196,12 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
  *   (`arg` ignored; 0 = default 2048).
  * GW_TUNE_GEMM_TILE: MFMA blocks per wave of the complex GEMMs of precision `arg`
  *   (GW_FP64 / GW_FP32), 0 = 2x2, 1 = 4x2, 2 = 2x4, 3 = 4x4.
- * GW_TUNE_XPASS: launch shape of the wave-line x pass, value = waves per workgroup
- *   (2, 4, 8) * 100 + bands per workgroup (1..64); default 408.
+ * GW_TUNE_XPASS: launch shape of the wave-line x pass for precision `arg` (GW_FP64 /
+ *   GW_FP32; any other `arg` sets both), value = mode * 10000 + waves per workgroup
+ *   (2, 4, 8) * 100 + bands per workgroup (1..64); mode 0 reads the x-lines straight
+ *   from the box, mode 1 stages each band's planes through LDS with row-contiguous
+ *   loads and stores (used when the tiles fit in LDS); default 408 for FP64,
+ *   10416 for FP32.
  * GW_TUNE_COMBINE_LIB: for precision `arg`, 1 runs gw_zgemm_combine (and the
  *   back-projection) on rocBLAS strided-batched GEMM, 0 on the kernels here (default). */
 enum { GW_TUNE_RADIX = 1, GW_TUNE_GEMM_WGS = 2, GW_TUNE_GEMM_TILE = 3, GW_TUNE_XPASS = 4,

@@ -153,3 +153,40 @@ def test_autotuner_choices_are_valid_and_exact(tmp_path, monkeypatch):
     L.lib().gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, 0)
     L.lib().gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP64, 0)
     L.lib().gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32, 0)
+
+
+def test_xpass_tuner_round_trip(tmp_path, monkeypatch):
+    """tune_xpass picks shapes from its list for a 90^3 and a 36^3 operator, the
+    choice survives save/load, and the applies stay exact."""
+    from tests.gpu_cases import select_k, system
+    from gradwave_amd import _lib as L
+    from gradwave_amd import autotune
+    from gradwave_amd.hamiltonian import BatchedHamiltonian
+
+    say("autotune x-pass shapes")
+    lib = L.lib()
+
+    class _W:  # the fields tune_xpass reads
+        def __init__(self, s):
+            _, self.bk, p = select_k(s, [0])
+            self.bk.nk, self.bk.npw_max = self.bk.mask.shape
+            self.shape, self.v_r, self.p = s.shape, s.v_r.to(dev()), p.to(dev())
+
+    wls = [_W(system("si64")), _W(system("si8"))]
+    c = (torch.randn(1, 6, wls[0].bk.npw_max, dtype=torch.complex128, generator=torch.Generator().manual_seed(3))
+         .to(dev()) * wls[0].bk.mask[:, None, :])
+    h = BatchedHamiltonian(wls[0].bk, wls[0].shape, wls[0].v_r, wls[0].p)
+    before = h.apply(c)
+    try:
+        got = autotune.tune_xpass(wls, nb=8, reps=2)
+        assert got["fp64"] in autotune.XPASS_SHAPES and got["fp32"] in autotune.XPASS_SHAPES
+        assert lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP32) == got["fp32"]
+        assert torch.equal(h.apply(c), before)
+        monkeypatch.setenv("GW_TUNE_FILE", str(tmp_path / "tune.json"))
+        autotune.save()
+        lib.gw_tune_set(L.GW_TUNE_XPASS, 0, 208)
+        assert autotune.load()["xpass"] == got
+        assert lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP64) == got["fp64"]
+    finally:
+        lib.gw_tune_set(L.GW_TUNE_XPASS, L.GW_FP64, 408)
+        lib.gw_tune_set(L.GW_TUNE_XPASS, L.GW_FP32, 10416)

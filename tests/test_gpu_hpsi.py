@@ -179,6 +179,36 @@ def test_wave_line_path_selected_and_exact(name, n):
     assert _rel(h.apply(c.to(torch.complex64).to(dev())).cpu().to(torch.complex128), want) < 2e-5
 
 
+@pytest.mark.parametrize("name,n", [("si2", 24), ("si8", 36), ("si64", 90), ("si8_120", 120)])
+def test_xpass_launch_shapes_agree(name, n):
+    """Every x-pass launch shape (lines straight from the box or planes staged
+    through LDS; 2, 4 or 8 waves; 4, 8 or 16 bands per workgroup, so band groups
+    end part-full) gives bit-identical results in both precisions, and the
+    oracle's to 1e-12 in FP64."""
+    say(f"x-pass shapes {name} n={n}")
+    s = system(name)
+    sel = list(range(min(2, s.bk.nk)))
+    cpu, h = _gpu_h(s, sel)
+    ref = ref_h(s, sel)
+    c = rand_block(len(sel), 10, cpu.mask.shape[1], cpu.mask, seed=12)
+    want = ref.apply(c)
+    lib = L.lib()
+    try:
+        for dt, prec in ((torch.complex128, L.GW_FP64), (torch.complex64, L.GW_FP32)):
+            cd = c.to(dt).to(dev())
+            L.check(lib.gw_tune_set(L.GW_TUNE_XPASS, prec, 408), "gw_tune_set")
+            base = h.apply(cd).cpu()
+            for shape in (10408, 10204, 10816, 10416, 204, 816):
+                L.check(lib.gw_tune_set(L.GW_TUNE_XPASS, prec, shape), "gw_tune_set")
+                assert lib.gw_tune_get(L.GW_TUNE_XPASS, prec) == shape
+                assert torch.equal(h.apply(cd).cpu(), base), (dt, shape)
+            if dt == torch.complex128:
+                assert _rel(base, want) < 1e-12
+    finally:
+        L.check(lib.gw_tune_set(L.GW_TUNE_XPASS, L.GW_FP64, 408), "gw_tune_set")
+        L.check(lib.gw_tune_set(L.GW_TUNE_XPASS, L.GW_FP32, 10416), "gw_tune_set")
+
+
 def test_si216_gamma_workload_against_oracle():
     """The 216-atom Gamma workload (80,797 PW, 120^3 box, 864 projectors) built by
     the product's own setup, applied on 4 bands, against the oracle apply on the

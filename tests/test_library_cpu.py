@@ -55,7 +55,8 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert "does not split" in err(lib.gw_tune_set(L.GW_TUNE_RADIX, 24, 5))
     assert "bad tile" in err(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32, 7))
     assert "x-pass shape" in err(lib.gw_tune_set(L.GW_TUNE_XPASS, 0, 305))
-    assert lib.gw_tune_get(L.GW_TUNE_XPASS, 0) == 408
+    assert lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP64) == 408
+    assert lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP32) == 10416
     assert err(lib.gw_batch_create(C.byref(C.c_void_p()), None, 0))
     assert err(lib.gw_toeplitz_build(0, 1, 0, 4, 4, 4, None, None, None, None, None))
     assert err(lib.gw_gamma_pack(0, 0, 5, 5, None, 5, None, None, None, 5, None))
