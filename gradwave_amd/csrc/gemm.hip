@@ -66,7 +66,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_gemm_inner(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long sx,
              const cpx<T>* __restrict__ Y, long ldy, long sy, int splits, int kchunk,
-             double* __restrict__ part, int ntx, int nty, long total) {
+             T* __restrict__ part, int ntx, int nty, long total) {
   using A4 = typename Mfma<T>::acc;
   // work id order: (k, split) slowest, then row tile, then column tile, so the
   // tiles of one K-chunk run together on one XCD and share its L2
@@ -135,7 +135,7 @@ k_gemm_inner(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long s
         for (int bn = 0; bn < 2; ++bn) acc[bm][bn] = Mfma<T>::op(av[bm][v], bv[bn][v], acc[bm][bn]);
   }
   const int ncol = 2 * N;
-  double* pk = part + ((long)k * splits + sp) * M * ncol;
+  T* pk = part + ((long)k * splits + sp) * M * ncol;
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
@@ -144,29 +144,45 @@ k_gemm_inner(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long s
       for (int r = 0; r < 4; ++r) {
         const int row = i0 + bm * 16 + Mfma<T>::row(lk, r);
         const int col = jj0 + bn * 16 + li;
-        if (row < M && col < ncol) pk[(long)row * ncol + col] = (double)acc[bm][bn][r];
+        if (row < M && col < ncol) pk[(long)row * ncol + col] = acc[bm][bn][r];
       }
 }
 
+// Sums the split-K partials of each real output element in FP64. The 4 waves of a
+// workgroup take every 4th split of 64 consecutive elements (coalesced rows, 4
+// independent loads in flight per lane) and combine through LDS.
 template <typename T>
-__global__ void k_gemm_inner_reduce(int M, int N, int splits, const double* __restrict__ part,
-                                    cpx<T>* __restrict__ C, long ldc, long sc) {
-  const int k = blockIdx.y;
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;  // complex index
-  const long tot = (long)M * N;
-  if (e >= tot) return;
-  const int i = (int)(e / N), j = (int)(e - (long)i * N);
-  const double* pk = part + (long)k * splits * M * 2 * N;
-  double re = 0, im = 0;
-  for (int s = 0; s < splits; ++s) {
-    const double* q = pk + (long)s * M * 2 * N + (long)i * 2 * N + 2 * j;
-    re += q[0];
-    im += q[1];
+__device__ __forceinline__ double reduce_splits(const T* __restrict__ pk, long stride, long e, bool ok, int splits) {
+  __shared__ double red[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  if (ok) {
+    int s = w;
+    for (; s + 12 < splits; s += 16) {
+      a0 += (double)pk[(long)s * stride + e];
+      a1 += (double)pk[(long)(s + 4) * stride + e];
+      a2 += (double)pk[(long)(s + 8) * stride + e];
+      a3 += (double)pk[(long)(s + 12) * stride + e];
+    }
+    for (; s < splits; s += 4) a0 += (double)pk[(long)s * stride + e];
   }
-  cpx<T> out;
-  out.x = (T)re;
-  out.y = (T)im;
-  C[(long)k * sc + (long)i * ldc + j] = out;
+  red[w][lane] = (a0 + a1) + (a2 + a3);
+  __syncthreads();
+  return (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_inner_reduce(int M, int N, int splits, const T* __restrict__ part, cpx<T>* __restrict__ C, long ldc,
+                    long sc) {
+  const int k = blockIdx.y;
+  const long nreal = (long)M * 2 * N;
+  const long e = (long)blockIdx.x * 64 + (threadIdx.x & 63);  // real element i*2N + 2j + c
+  const double v = reduce_splits<T>(part + (long)k * splits * nreal, nreal, e, e < nreal, splits);
+  if (threadIdx.x < 64 && e < nreal) {
+    const long i = e / (2 * N), r = e - i * 2 * N;
+    reinterpret_cast<T*>(C + (long)k * sc + i * ldc + (r >> 1))[r & 1] = (T)v;
+  }
 }
 
 
@@ -186,7 +202,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_gemm_inner_lds(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long sx,
                  const cpx<T>* __restrict__ Y, long ldy, long sy, int splits, int kchunk,
-                 double* __restrict__ part, int ntx, int nty, long total) {
+                 T* __restrict__ part, int ntx, int nty, long total) {
   using A4 = typename Mfma<T>::acc;
   constexpr int LD = kLd<T>;
   __shared__ cpx<T> As[64 * LD];
@@ -263,7 +279,7 @@ k_gemm_inner_lds(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, lo
     }
   }
   const int ncol = 2 * N;
-  double* pk = part + ((long)k * splits + sp) * M * ncol;
+  T* pk = part + ((long)k * splits + sp) * M * ncol;
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
@@ -272,7 +288,7 @@ k_gemm_inner_lds(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, lo
       for (int r = 0; r < 4; ++r) {
         const int row = row0 + wm * 32 + bm * 16 + Mfma<T>::row(lk, r);
         const int col = 2 * col0 + wn * 32 + bn * 16 + li;
-        if (row < M && col < ncol) pk[(long)row * ncol + col] = (double)acc[bm][bn][r];
+        if (row < M && col < ncol) pk[(long)row * ncol + col] = acc[bm][bn][r];
       }
 }
 
@@ -380,7 +396,7 @@ template <typename T, int WM, int WN>
 __global__ void __launch_bounds__(256)
 k_gemm_inner_tile(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long sx,
                   const cpx<T>* __restrict__ Y, long ldy, long sy, int splits, int kchunk,
-                  double* __restrict__ part, int ntx, int nty, long total) {
+                  T* __restrict__ part, int ntx, int nty, long total) {
   using A4 = typename Mfma<T>::acc;
   constexpr int LD = kLd<T>;
   constexpr int TM = 32 * WM, TN = 16 * WN;  // X rows, Y rows (complex columns of C)
@@ -458,7 +474,7 @@ k_gemm_inner_tile(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, l
     }
   }
   const int ncol = 2 * N;
-  double* pk = part + ((long)k * splits + sp) * M * ncol;
+  T* pk = part + ((long)k * splits + sp) * M * ncol;
 #pragma unroll
   for (int bm = 0; bm < WM; ++bm)
 #pragma unroll
@@ -467,7 +483,7 @@ k_gemm_inner_tile(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, l
       for (int r = 0; r < 4; ++r) {
         const int row = row0 + wm * 16 * WM + bm * 16 + Mfma<T>::row(lk, r);
         const int col = 2 * col0 + wn * 16 * WN + bn * 16 + li;
-        if (row < M && col < ncol) pk[(long)row * ncol + col] = (double)acc[bm][bn][r];
+        if (row < M && col < ncol) pk[(long)row * ncol + col] = acc[bm][bn][r];
       }
 }
 
@@ -596,7 +612,7 @@ template <typename T>
 size_t gemm_inner_partials_bytes(int nk, int M, int N, int K) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   const int s = inner_splits(nk, M, N, K);
-  return (size_t)nk * s * M * 2 * N * sizeof(double);
+  return (size_t)nk * s * M * 2 * N * sizeof(T);
 }
 template size_t gemm_inner_partials_bytes<double>(int, int, int, int);
 template size_t gemm_inner_partials_bytes<float>(int, int, int, int);
@@ -611,7 +627,7 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
     return GW_OK;
   }
   const int splits = inner_splits(nk, M, N, K);
-  const size_t need = (size_t)nk * splits * M * 2 * N * sizeof(double);
+  const size_t need = (size_t)nk * splits * M * 2 * N * sizeof(T);
   GW_REQUIRE(partials && part_bytes >= need, GW_ERR_WORKSPACE, "gemm_inner workspace %zu < %zu", part_bytes, need);
   int kchunk = (K + splits - 1) / splits;
   kchunk = ((kchunk + kKC - 1) / kKC) * kKC;  // multiple of both K granules (16, 32)
@@ -621,11 +637,11 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
     const int tx = (2 * N + 32 * wn - 1) / (32 * wn), ty = (M + 32 * wm - 1) / (32 * wm);
     const long tt = (long)tx * ty * nk * splits;
     hipLaunchKernelGGL(kern, dim3((unsigned)(8 * ((tt + 7) / 8))), dim3(256), 0, st, M, N, K, X, ldx, sx, Y, ldy,
-                       sy, splits, kchunk, (double*)partials, tx, ty, tt);
+                       sy, splits, kchunk, (T*)partials, tx, ty, tt);
   };
   if (gemm_simple())
     hipLaunchKernelGGL(k_gemm_inner<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X, ldx,
-                       sx, Y, ldy, sy, splits, kchunk, (double*)partials, ntx, nty, total);
+                       sx, Y, ldy, sy, splits, kchunk, (T*)partials, ntx, nty, total);
   else if (tile_of<T>() == 1)
     launch_tile(k_gemm_inner_tile<T, 4, 2>, 4, 2);
   else if (tile_of<T>() == 2)
@@ -634,10 +650,10 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
     launch_tile(k_gemm_inner_tile<T, 4, 4>, 4, 4);
   else
     hipLaunchKernelGGL(k_gemm_inner_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X,
-                       ldx, sx, Y, ldy, sy, splits, kchunk, (double*)partials, ntx, nty, total);
-  const long tot = (long)M * N;
-  hipLaunchKernelGGL(k_gemm_inner_reduce<T>, dim3((tot + 255) / 256, nk), dim3(256), 0, st, M, N, splits,
-                     (const double*)partials, C, ldc, sc);
+                       ldx, sx, Y, ldy, sy, splits, kchunk, (T*)partials, ntx, nty, total);
+  const long nreal = (long)M * 2 * N;
+  hipLaunchKernelGGL(k_gemm_inner_reduce<T>, dim3((unsigned)((nreal + 63) / 64), nk), dim3(256), 0, st, M, N,
+                     splits, (const T*)partials, C, ldc, sc);
   GW_CHECK_HIP(hipGetLastError());
   return GW_OK;
 }
@@ -852,7 +868,7 @@ namespace {
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_rgemm_inner_lds(int M, int N, int K, const T* __restrict__ X, long ldx, long sx, const T* __restrict__ Y,
-                  long ldy, long sy, int splits, int kchunk, double* __restrict__ part, int ntx, int nty,
+                  long ldy, long sy, int splits, int kchunk, T* __restrict__ part, int ntx, int nty,
                   long total) {
   using A4 = typename Mfma<T>::acc;
   constexpr int LD = kLdR<T>;
@@ -917,7 +933,7 @@ k_rgemm_inner_lds(int M, int N, int K, const T* __restrict__ X, long ldx, long s
 #pragma unroll
         for (int bn = 0; bn < 2; ++bn) acc[bm][bn] = Mfma<T>::op(av[bm][v], bv[bn][v], acc[bm][bn]);
   }
-  double* pk = part + ((long)k * splits + sp) * M * N;
+  T* pk = part + ((long)k * splits + sp) * M * N;
 #pragma unroll
   for (int bm = 0; bm < 2; ++bm)
 #pragma unroll
@@ -926,22 +942,21 @@ k_rgemm_inner_lds(int M, int N, int K, const T* __restrict__ X, long ldx, long s
       for (int r = 0; r < 4; ++r) {
         const int row = row0 + wm * 32 + bm * 16 + Mfma<T>::row(lk, r);
         const int col = col0 + wn * 32 + bn * 16 + li;
-        if (row < M && col < N) pk[(long)row * N + col] = (double)acc[bm][bn][r];
+        if (row < M && col < N) pk[(long)row * N + col] = acc[bm][bn][r];
       }
 }
 
 template <typename T>
-__global__ void k_rgemm_inner_reduce(int M, int N, int splits, const double* __restrict__ part, T* __restrict__ C,
-                                     long ldc, long sc) {
+__global__ void __launch_bounds__(256)
+k_rgemm_inner_reduce(int M, int N, int splits, const T* __restrict__ part, T* __restrict__ C, long ldc, long sc) {
   const int k = blockIdx.y;
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long tot = (long)M * N;
-  if (e >= tot) return;
-  const int i = (int)(e / N), j = (int)(e - (long)i * N);
-  const double* pk = part + (long)k * splits * tot + e;
-  double s = 0;
-  for (int q = 0; q < splits; ++q) s += pk[(long)q * tot];
-  C[(long)k * sc + (long)i * ldc + j] = (T)s;
+  const long e = (long)blockIdx.x * 64 + (threadIdx.x & 63);
+  const double v = reduce_splits<T>(part + (long)k * splits * tot, tot, e, e < tot, splits);
+  if (threadIdx.x < 64 && e < tot) {
+    const long i = e / N, j = e - i * N;
+    C[(long)k * sc + i * ldc + j] = (T)v;
+  }
 }
 
 template <typename T>
@@ -1040,7 +1055,7 @@ int rgemm_inner(int nk, int M, int N, int K, const T* X, long ldx, long sx, cons
   if (M <= 0 || N <= 0 || nk <= 0) return GW_OK;
   GW_REQUIRE(K > 0, GW_ERR_ARG, "K must be positive");
   const int splits = rinner_splits(nk, M, N, K);
-  const size_t need = (size_t)nk * splits * M * N * sizeof(double);
+  const size_t need = (size_t)nk * splits * M * N * sizeof(T);
   GW_REQUIRE(partials && part_bytes >= need, GW_ERR_WORKSPACE, "rgemm_inner workspace %zu < %zu", part_bytes,
              need);
   GW_REQUIRE(nk <= 65535, GW_ERR_UNSUPPORTED, "nk too large");
@@ -1049,10 +1064,10 @@ int rgemm_inner(int nk, int M, int N, int K, const T* X, long ldx, long sx, cons
   const int ntx = (N + 63) / 64, nty = (M + 63) / 64;
   const long total = (long)ntx * nty * nk * splits;
   hipLaunchKernelGGL(k_rgemm_inner_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X,
-                     ldx, sx, Y, ldy, sy, splits, kchunk, (double*)partials, ntx, nty, total);
+                     ldx, sx, Y, ldy, sy, splits, kchunk, (T*)partials, ntx, nty, total);
   const long tot = (long)M * N;
-  hipLaunchKernelGGL(k_rgemm_inner_reduce<T>, dim3((unsigned)((tot + 255) / 256), nk), dim3(256), 0, st, M, N,
-                     splits, (const double*)partials, C, ldc, sc);
+  hipLaunchKernelGGL(k_rgemm_inner_reduce<T>, dim3((unsigned)((tot + 63) / 64), nk), dim3(256), 0, st, M, N,
+                     splits, (const T*)partials, C, ldc, sc);
   GW_CHECK_HIP(hipGetLastError());
   return GW_OK;
 }
@@ -1072,9 +1087,8 @@ int rgemm_combine(int nk, int A, int G, int J, const T* U, long ldu, long su, co
 }  // namespace
 
 extern "C" size_t gw_rgemm_inner_workspace_bytes(int precision, int nk, int M, int N, int K) {
-  (void)precision;
   if (M <= 0 || N <= 0 || K <= 0 || nk <= 0) return 0;
-  return (size_t)nk * rinner_splits(nk, M, N, K) * M * N * sizeof(double);
+  return (size_t)nk * rinner_splits(nk, M, N, K) * M * N * (precision == GW_FP64 ? sizeof(double) : sizeof(float));
 }
 
 extern "C" int gw_rgemm_inner(int precision, int nk, int M, int N, int K, const void* X, long ldx, long sx,
