@@ -6,8 +6,8 @@ the GEMMs (no reference counterpart: the reference uses torch.fft and BLAS).
   ``gw_tune_set(GW_TUNE_RADIX, n, a)`` says otherwise. ``tune_radix`` times
   the generic-size local-term kernels of real operators for each candidate a
   and keeps the fastest.
-* GEMM tiles and split-K: the complex MFMA kernels run 2x2, 4x2, 2x4 or 4x4
-  16x16 blocks per wave (``GW_TUNE_GEMM_TILE``), and ``gw_zgemm_inner`` /
+* GEMM tiles and split-K: the complex MFMA kernels run 2x2, 4x2, 2x4, 4x4 or
+  band-sized (2x5 / 5x2, 4x5 / 5x4) 16x16 blocks per wave (``GW_TUNE_GEMM_TILE``), and ``gw_zgemm_inner`` /
   ``gw_rgemm_inner`` split the long axis until the launch has about
   ``GW_TUNE_GEMM_WGS`` workgroups; ``tune_gemm`` times the projector products
   of real operators for each tile shape and target.
@@ -132,7 +132,7 @@ def tune_radix(workloads, nb: int = 8, reps: int = 5) -> dict[int, int]:
     return chosen
 
 
-GEMM_TILES = (0, 1, 2, 3)  # MFMA blocks per wave: 2x2, 4x2, 2x4, 4x4
+GEMM_TILES = (0, 1, 2, 3, 4, 5)  # MFMA blocks per wave: 2x2, 4x2, 2x4, 4x4, band-sized 2x5/5x2, 4x5/5x4
 
 
 def tune_gemm(workloads, nb: int = 32, reps: int = 5, allow_library: bool = True) -> tuple[dict, int]:

@@ -587,7 +587,7 @@ k_gemm_combine_tile(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu,
 
 // per precision (GW_FP64, GW_FP32): 0: (2, 2), 1: (4, 2), 2: (2, 4), 3: (4, 4) MFMA blocks per wave
 static int g_gemm_tile[2] = {0, 0};
-void set_gemm_tile(int precision, int v) { g_gemm_tile[precision == GW_FP32] = (v >= 0 && v <= 3) ? v : 0; }
+void set_gemm_tile(int precision, int v) { g_gemm_tile[precision == GW_FP32] = (v >= 0 && v <= 5) ? v : 0; }
 int gemm_tile(int precision) { return g_gemm_tile[precision == GW_FP32]; }
 template <typename T> static int tile_of() { return g_gemm_tile[sizeof(T) == 4]; }
 
@@ -648,6 +648,10 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
     launch_tile(k_gemm_inner_tile<T, 2, 4>, 2, 4);
   else if (tile_of<T>() == 3)
     launch_tile(k_gemm_inner_tile<T, 4, 4>, 4, 4);
+  else if (tile_of<T>() == 4)
+    launch_tile(k_gemm_inner_tile<T, 2, 5>, 2, 5);
+  else if (tile_of<T>() == 5)
+    launch_tile(k_gemm_inner_tile<T, 4, 5>, 4, 5);
   else
     hipLaunchKernelGGL(k_gemm_inner_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X,
                        ldx, sx, Y, ldy, sy, splits, kchunk, (T*)partials, ntx, nty, total);
@@ -809,6 +813,10 @@ int gemm_combine(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su
     launch_tile(k_gemm_combine_tile<T, 2, 4>, 2, 4);
   else if (tile_of<T>() == 3)
     launch_tile(k_gemm_combine_tile<T, 4, 4>, 4, 4);
+  else if (tile_of<T>() == 4)
+    launch_tile(k_gemm_combine_tile<T, 5, 2>, 5, 2);
+  else if (tile_of<T>() == 5)
+    launch_tile(k_gemm_combine_tile<T, 5, 4>, 5, 4);
   else
     hipLaunchKernelGGL(k_gemm_combine_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, A, G, J,
                        U, ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, ntx, nty, total);

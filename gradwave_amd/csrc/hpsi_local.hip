@@ -1327,7 +1327,7 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
     return GW_OK;
   }
   if (key == GW_TUNE_GEMM_TILE) {
-    GW_REQUIRE(value >= 0 && value <= 3, GW_ERR_ARG, "bad tile variant %d", value);
+    GW_REQUIRE(value >= 0 && value <= 5, GW_ERR_ARG, "bad tile variant %d", value);
     GW_REQUIRE(arg == GW_FP64 || arg == GW_FP32, GW_ERR_ARG, "bad precision %d", arg);
     set_gemm_tile(arg, value);
     return GW_OK;

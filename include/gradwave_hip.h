@@ -195,7 +195,10 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
  * GW_TUNE_GEMM_WGS: the number of workgroups the split-K inner products aim at
  *   (`arg` ignored; 0 = default 2048).
  * GW_TUNE_GEMM_TILE: MFMA blocks per wave of the complex GEMMs of precision `arg`
- *   (GW_FP64 / GW_FP32), 0 = 2x2, 1 = 4x2, 2 = 2x4, 3 = 4x4.
+ *   (GW_FP64 / GW_FP32), 0 = 2x2, 1 = 4x2, 2 = 2x4, 3 = 4x4; 4 and 5 cover 160
+ *   bands (80 complex columns of the inner product, 160 rows of the
+ *   back-projection) per workgroup: 4 = inner 2x5 / combine 5x2, 5 = inner 4x5 /
+ *   combine 5x4.
  * GW_TUNE_XPASS: launch shape of the wave-line x pass for precision `arg` (GW_FP64 /
  *   GW_FP32; any other `arg` sets both), value = mode * 10000 + waves per workgroup
  *   (2, 4, 8) * 100 + bands per workgroup (1..64); mode 0 reads the x-lines straight

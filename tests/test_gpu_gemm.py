@@ -124,7 +124,7 @@ def test_real_combine(dtype, nk, A, G, J, alpha, beta):
     assert float((z.cpu().double() - want).abs().max()) < RTOL[dtype] * scale
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_tile_variants_match(tile, dtype):
     """The autotuner's MFMA tile shapes give the same contractions."""
