@@ -406,6 +406,7 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
   // staging map: thread element k is tile entry e = threadIdx.x + k*NT = a*LC + jj
   // <-> box[band][a][j0 + jj]; entries past the tile or the last line are masked
   const int ne = na * LC;
+  const int ks = (ne + NT - 1) / NT;  // staged elements per thread actually needed (uniform)
   int soff[KS];
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
@@ -437,13 +438,17 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
   {
     const auto r = buf_rsrc(box + (long)b0 * na * n23, band_bytes);
 #pragma unroll
-    for (int k = 0; k < KS; ++k) st[k] = buf_load<T>(r, soff[k]);
+    for (int k = 0; k < KS; ++k)
+      if (k < ks) st[k] = buf_load<T>(r, soff[k]);
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       const int e = threadIdx.x + k * NT;
-      if (e < ne) tiles[e] = st[k];
+      if (k < ks && e < ne) tiles[e] = st[k];
     }
   }
+  // V(r) and the first tile have landed on every path: no load is pending at the
+  // loop head, so the waits inside the loop concern the next band's tile only
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
   __syncthreads();
   cpx<T> x[M], y[S];
   for (int bg = 0; bg < nbg; ++bg) {
@@ -454,7 +459,8 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
       const bool more = bg + 1 < nbg;
       const auto r = buf_rsrc(box + (long)(more ? bi + 1 : bi) * na * n23, more ? band_bytes : 0u);
 #pragma unroll
-      for (int k = 0; k < KS; ++k) st[k] = buf_load<T>(r, soff[k]);
+      for (int k = 0; k < KS; ++k)
+        if (k < ks) st[k] = buf_load<T>(r, soff[k]);
     }
 #pragma unroll
     for (int q = 0; q < M; ++q) x[q] = cur[rd[q]];
@@ -470,13 +476,13 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
 #pragma unroll
       for (int k = 0; k < KS; ++k) {
         const int e = threadIdx.x + k * NT;
-        buf_store<T>(cur[min(e, ne - 1)], r, soff[k]);
+        if (k < ks) buf_store<T>(cur[min(e, ne - 1)], r, soff[k]);
       }
     }
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       const int e = threadIdx.x + k * NT;
-      if (e < ne) nxt[e] = st[k];
+      if (k < ks && e < ne) nxt[e] = st[k];
     }
     __syncthreads();
   }
