@@ -742,13 +742,28 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
   const int* src = g.pt_src + (long)k * g.npw_max;
   const int* yz = g.pt_yz + (long)k * g.npw_max;
   const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  // The plane's points, one per thread and pass over all threads; the first two
+  // passes are loaded before the LDS set-up so their latency overlaps it.
+  constexpr int NT = 64 * NW;
+  const int p0 = pp[s0], p1 = pp[s1];
+  const int qa = p0 + threadIdx.x, qb = qa + NT;
+  int va = -1, vb = -1;
+  cpx<T> ca = czero<T>(), cbv = czero<T>();
+  if (qa < p1) { va = yz[qa]; ca = cb[src[qa]]; }
+  if (qb < p1) { vb = yz[qb]; cbv = cb[src[qb]]; }
   for (int i = threadIdx.x; i < N; i += blockDim.x) { tw[i] = twg[i]; map[i] = -1; }
   for (int e = threadIdx.x; e < ns * N; e += blockDim.x) rows[e] = czero<T>();
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) map[i1s[s]] = s - s0;
-  for (int s = s0 + wave; s < s1; s += NW)
-    for (int q = pp[s] + lane; q < pp[s + 1]; q += 64) rows[(s - s0) * N + yz[q] % N] = cb[src[q]];
+  __syncthreads();
+  // point (y, z) of the plane -> stick row map[y], column z
+  if (va >= 0) rows[map[va / N] * N + va % N] = ca;
+  if (vb >= 0) rows[map[vb / N] * N + vb % N] = cbv;
+  for (int q = qb + NT; q < p1; q += NT) {
+    const int v = yz[q];
+    rows[map[v / N] * N + v % N] = cb[src[q]];
+  }
   __syncthreads();
   const int t = lane / LW, l = lane - t * LW;
   const bool part = t < S;
@@ -811,6 +826,26 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
   if (ns == 0) return;
   const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
   const int* i1s = g.st_i1 + (long)k * g.nst_max;
+  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  const int* src = g.pt_src + (long)k * g.npw_max;
+  const int* yz = g.pt_yz + (long)k * g.npw_max;
+  const T* tk = tk_all + (long)k * g.npw_max;
+  // The plane's points, one per thread and pass over all threads: the indices and
+  // the kinetic term t*c of the first two passes are loaded now, so their latency
+  // overlaps the transforms.
+  constexpr int NT = 64 * NW;
+  const int p0 = pp[s0], p1 = pp[s1];
+  const int qa = p0 + threadIdx.x, qb = qa + NT;
+  int sa = -1, sb = -1, va = 0, vb = 0;
+  cpx<T> ka = czero<T>(), kb = czero<T>();
+  if (qa < p1) {
+    sa = src[qa]; va = yz[qa];
+    if constexpr (KIN) { const cpx<T> cv = cb[sa]; const T tt = tk[sa]; ka.x = tt * cv.x; ka.y = tt * cv.y; }
+  }
+  if (qb < p1) {
+    sb = src[qb]; vb = yz[qb];
+    if constexpr (KIN) { const cpx<T> cv = cb[sb]; const T tt = tk[sb]; kb.x = tt * cv.x; kb.y = tt * cv.y; }
+  }
   for (int i = threadIdx.x; i < N; i += blockDim.x) { tw[i] = twg[i]; map[i] = -1; }
   __syncthreads();
   for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) map[i1s[s]] = s - s0;
@@ -819,13 +854,20 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
   const int t = lane / LW, l = lane - t * LW;
   const bool part = t < S;
   cpx<T>* Ew = E + wave * W::E_PER_WAVE;
-  cpx<T> x[M], y[S];
+  cpx<T> x[M], y[S], yn[S];
   const cpx<T>* srcp = box + ((long)bi * g.na + a) * N * N;
+  auto load_cols = [&](int c0, cpx<T> (&v)[S]) {
+    const int col = c0 + l;
+#pragma unroll
+    for (int u = 0; u < S; ++u) v[u] = (col < N && t < M) ? srcp[(long)W::rpos(t, u) * N + col] : czero<T>();
+  };
+  if (wave * LW < N) load_cols(wave * LW, yn);
   for (int c0 = wave * LW; c0 < N; c0 += NW * LW) {
     const int col = c0 + l;
     const bool ok = col < N;
 #pragma unroll
-    for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? srcp[(long)W::rpos(t, u) * N + col] : czero<T>();
+    for (int u = 0; u < S; ++u) y[u] = yn[u];
+    if (c0 + NW * LW < N) load_cols(c0 + NW * LW, yn);  // next columns' loads overlap this transform
     W::forward(y, x, t, l, part, Ew, tw);
     if (ok && part) {
 #pragma unroll
@@ -848,23 +890,21 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
     }
   }
   __syncthreads();
-  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
-  const int* src = g.pt_src + (long)k * g.npw_max;
-  const int* yz = g.pt_yz + (long)k * g.npw_max;
-  const T* tk = tk_all + (long)k * g.npw_max;
-  for (int s = s0 + wave; s < s1; s += NW)
-    for (int q = pp[s] + lane; q < pp[s + 1]; q += 64) {
-      const int si = src[q];
-      cpx<T> v = rows[(s - s0) * N + yz[q] % N];
-      v.x *= scale;
-      v.y *= scale;
-      if constexpr (KIN) {
-        const cpx<T> cv = cb[si];
-        v.x += tk[si] * cv.x;
-        v.y += tk[si] * cv.y;
-      }
-      ob[si] = v;
-    }
+  // gather: point (y, z) of the plane sits in stick row map[y], column z
+  auto put = [&](int si, int v, cpx<T> kin) {
+    cpx<T> r = rows[map[v / N] * N + v % N];
+    r.x = r.x * scale + kin.x;
+    r.y = r.y * scale + kin.y;
+    ob[si] = r;
+  };
+  if (sa >= 0) put(sa, va, ka);
+  if (sb >= 0) put(sb, vb, kb);
+  for (int q = qb + NT; q < p1; q += NT) {
+    const int si = src[q];
+    cpx<T> kin = czero<T>();
+    if constexpr (KIN) { const cpx<T> cv = cb[si]; const T tt = tk[si]; kin.x = tt * cv.x; kin.y = tt * cv.y; }
+    put(si, yz[q], kin);
+  }
 }
 
 // wave-line configurations: N = M*S with M <= S <= 16
