@@ -7,10 +7,11 @@ the GEMMs (no reference counterpart: the reference uses torch.fft and BLAS).
   the generic-size local-term kernels of real operators for each candidate a
   and keeps the fastest.
 * GEMM tiles and split-K: the complex MFMA kernels run 2x2, 4x2, 2x4, 4x4 or
-  band-sized (2x5 / 5x2, 4x5 / 5x4) 16x16 blocks per wave (``GW_TUNE_GEMM_TILE``), and ``gw_zgemm_inner`` /
-  ``gw_rgemm_inner`` split the long axis until the launch has about
-  ``GW_TUNE_GEMM_WGS`` workgroups; ``tune_gemm`` times the projector products
-  of real operators for each tile shape and target.
+  band-sized (2x5 / 5x2, 4x5 / 5x4) 16x16 blocks per wave (``GW_TUNE_GEMM_TILE``,
+  per precision and per contraction), and ``gw_zgemm_inner`` / ``gw_rgemm_inner``
+  split the long axis until the launch has about ``GW_TUNE_GEMM_WGS``
+  workgroups; ``tune_gemm`` times the projector products of real operators for
+  each tile shape and target.
 * x pass of the wave-line boxes (24, 36, 90, 120 cubed): lines straight from
   HBM or planes staged through LDS, waves and bands per workgroup
   (``GW_TUNE_XPASS``, per precision); ``tune_xpass`` times the local term.
@@ -48,8 +49,10 @@ def current() -> dict:
     radix = {n: lib.gw_tune_get(L.GW_TUNE_RADIX, n) for n in range(1, 1024)}
     return {"radix": {str(n): a for n, a in radix.items() if a > 0},
             "gemm_wgs": lib.gw_tune_get(L.GW_TUNE_GEMM_WGS, 0),
-            "gemm_tile": {"fp64": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP64),
-                          "fp32": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32)},
+            "gemm_tile": {"fp64": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP64 + 256),
+                          "fp32": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32 + 256)},
+            "gemm_tile_combine": {"fp64": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP64 + 512),
+                                  "fp32": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32 + 512)},
             "combine_lib": {"fp64": lib.gw_tune_get(L.GW_TUNE_COMBINE_LIB, L.GW_FP64),
                             "fp32": lib.gw_tune_get(L.GW_TUNE_COMBINE_LIB, L.GW_FP32)},
             "xpass": {"fp64": lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP64),
@@ -64,7 +67,10 @@ def apply(choices: dict) -> None:
         L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, int(choices["gemm_wgs"])), "gw_tune_set")
     for name, prec in (("fp64", L.GW_FP64), ("fp32", L.GW_FP32)):
         if name in choices.get("gemm_tile", {}):
-            L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, int(choices["gemm_tile"][name])), "gw_tune_set")
+            L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec + 256, int(choices["gemm_tile"][name])), "gw_tune_set")
+        if name in choices.get("gemm_tile_combine", {}):
+            L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec + 512, int(choices["gemm_tile_combine"][name])),
+                    "gw_tune_set")
         if name in choices.get("combine_lib", {}):
             L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, int(choices["combine_lib"][name])), "gw_tune_set")
         if name in choices.get("xpass", {}):
@@ -136,52 +142,75 @@ GEMM_TILES = (0, 1, 2, 3, 4, 5)  # MFMA blocks per wave: 2x2, 4x2, 2x4, 4x4, ban
 
 
 def tune_gemm(workloads, nb: int = 32, reps: int = 5, allow_library: bool = True) -> tuple[dict, int]:
-    """Time the projector products (nonlocal term) of ``workloads``: first each
-    MFMA tile shape per precision at the current split-K target (and, with
-    ``allow_library``, rocBLAS for the back-projection), then each split-K
-    workgroup target (both precisions) with the best choices; keep the fastest.
-    Returns ({"fp64": tile, "fp32": tile, "combine_lib": {...}}, target); tile
-    4 stands for the rocBLAS back-projection with the best tile for the inner
-    product."""
+    """Tune the projector products (nonlocal term) of ``workloads`` on the plan's
+    per-stage HIP-event times: per precision, the inner-product tile on the becp
+    stage and the back-projection tile on the backproj stage, each on its own
+    (with ``allow_library``, rocBLAS is also tried for the back-projection); then
+    the split-K workgroup target on the becp stage of both precisions. Returns
+    ({"fp64": inner tile, "fp32": inner tile, "combine": {"fp64": tile, "fp32": tile},
+    "combine_lib": {"fp64": 0|1, "fp32": 0|1}}, target)."""
     from gradwave_amd.hamiltonian import BatchedHamiltonian
 
     lib = L.lib()
     hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in workloads]
+    for h in hams:
+        h.set_flags(L.GW_PLAN_TIMING)
     g = torch.Generator().manual_seed(7)
     b64 = [(torch.randn(w.bk.nk, nb, w.bk.npw_max, dtype=torch.complex128, generator=g).to(w.v_r.device)
             * w.bk.mask[:, None, :]) for w in workloads]
     blocks = {"fp64": b64, "fp32": [b.to(torch.complex64) for b in b64]}
 
-    def run(name):
-        return [h.apply_terms(x, L.GW_TERM_NONLOCAL) for h, x in zip(hams, blocks[name])]
+    def stage_ms(name):
+        def run():
+            for h, x in zip(hams, blocks[name]):
+                h.apply_terms(x, L.GW_TERM_NONLOCAL)
+        run()
+        torch.cuda.synchronize()
+        for h in hams:
+            h.stage_times(reset=True)
+        for _ in range(reps):
+            run()
+        torch.cuda.synchronize()
+        tot = {"becp": 0.0, "backproj": 0.0}
+        for h in hams:
+            st = h.stage_times(reset=True)
+            for k in tot:
+                tot[k] += st[k][0] / max(st[k][1], 1)
+        return tot
 
-    tiles = {}
-    libs = {}
-    for name, prec in (("fp64", L.GW_FP64), ("fp32", L.GW_FP32)):
-        L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 0), "gw_tune_set")
-        best_tile, best_ms = 0, float("inf")
-        for tile in GEMM_TILES:
-            L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, tile), "gw_tune_set")
-            ms = _median_ms(lambda: run(name), reps)
+    tiles, combine, libs = {}, {}, {}
+    try:
+        for name, prec in (("fp64", L.GW_FP64), ("fp32", L.GW_FP32)):
+            L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 0), "gw_tune_set")
+            best = {"becp": (0, float("inf")), "backproj": (0, float("inf"))}
+            for tile in GEMM_TILES:
+                L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, tile), "gw_tune_set")
+                t = stage_ms(name)
+                for k in best:
+                    if t[k] < best[k][1]:
+                        best[k] = (tile, t[k])
+            tiles[name], combine[name] = best["becp"][0], best["backproj"][0]
+            L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec + 256, tiles[name]), "gw_tune_set")
+            L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec + 512, combine[name]), "gw_tune_set")
+            libs[name] = 0
+            if allow_library:
+                L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 1), "gw_tune_set")
+                if stage_ms(name)["backproj"] < best["backproj"][1]:
+                    libs[name] = 1
+                L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, libs[name]), "gw_tune_set")
+        best_target, best_ms = None, float("inf")
+        for target in GEMM_TARGETS:
+            L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, target), "gw_tune_set")
+            ms = stage_ms("fp64")["becp"] + stage_ms("fp32")["becp"]
             if ms < best_ms:
-                best_tile, best_ms = tile, ms
-        L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, best_tile), "gw_tune_set")
-        tiles[name] = best_tile
-        libs[name] = 0
-        if allow_library:
-            L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 1), "gw_tune_set")
-            if _median_ms(lambda: run(name), reps) < best_ms:
-                libs[name] = 1
-            L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, libs[name]), "gw_tune_set")
+                best_target, best_ms = target, ms
+        L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, best_target), "gw_tune_set")
+    finally:
+        for h in hams:
+            h.set_flags(0)
+    tiles["combine"] = combine
     tiles["combine_lib"] = libs
-    best, best_ms = None, float("inf")
-    for target in GEMM_TARGETS:
-        L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, target), "gw_tune_set")
-        ms = _median_ms(lambda: (run("fp64"), run("fp32")), reps)
-        if ms < best_ms:
-            best, best_ms = target, ms
-    L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, best), "gw_tune_set")
-    return tiles, best
+    return tiles, best_target
 
 
 # x-pass shapes: mode * 10000 + waves * 100 + bands (mode 1: planes staged through LDS)

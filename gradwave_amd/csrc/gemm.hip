@@ -586,10 +586,15 @@ k_gemm_combine_tile(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu,
 }
 
 // per precision (GW_FP64, GW_FP32): 0: (2, 2), 1: (4, 2), 2: (2, 4), 3: (4, 4) MFMA blocks per wave
-static int g_gemm_tile[2] = {0, 0};
-void set_gemm_tile(int precision, int v) { g_gemm_tile[precision == GW_FP32] = (v >= 0 && v <= 5) ? v : 0; }
-int gemm_tile(int precision) { return g_gemm_tile[precision == GW_FP32]; }
-template <typename T> static int tile_of() { return g_gemm_tile[sizeof(T) == 4]; }
+// tile variant per [precision (0 FP64, 1 FP32)][contraction (0 inner, 1 combine)]
+static int g_gemm_tile[2][2] = {{0, 0}, {0, 0}};
+void set_gemm_tile(int precision, int op, int v) {
+  const int t = (v >= 0 && v <= 5) ? v : 0;
+  if (op != 1) g_gemm_tile[precision == GW_FP32][0] = t;
+  if (op != 0) g_gemm_tile[precision == GW_FP32][1] = t;
+}
+int gemm_tile(int precision, int op) { return g_gemm_tile[precision == GW_FP32][op == 1]; }
+template <typename T> static int tile_of(int op) { return g_gemm_tile[sizeof(T) == 4][op]; }
 
 static bool gemm_simple() {
   static const bool v = [] { const char* e = getenv("GW_GEMM_SIMPLE"); return e && e[0] == '1'; }();
@@ -642,15 +647,15 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
   if (gemm_simple())
     hipLaunchKernelGGL(k_gemm_inner<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X, ldx,
                        sx, Y, ldy, sy, splits, kchunk, (T*)partials, ntx, nty, total);
-  else if (tile_of<T>() == 1)
+  else if (tile_of<T>(0) == 1)
     launch_tile(k_gemm_inner_tile<T, 4, 2>, 4, 2);
-  else if (tile_of<T>() == 2)
+  else if (tile_of<T>(0) == 2)
     launch_tile(k_gemm_inner_tile<T, 2, 4>, 2, 4);
-  else if (tile_of<T>() == 3)
+  else if (tile_of<T>(0) == 3)
     launch_tile(k_gemm_inner_tile<T, 4, 4>, 4, 4);
-  else if (tile_of<T>() == 4)
+  else if (tile_of<T>(0) == 4)
     launch_tile(k_gemm_inner_tile<T, 2, 5>, 2, 5);
-  else if (tile_of<T>() == 5)
+  else if (tile_of<T>(0) == 5)
     launch_tile(k_gemm_inner_tile<T, 4, 5>, 4, 5);
   else
     hipLaunchKernelGGL(k_gemm_inner_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, M, N, K, X,
@@ -807,15 +812,15 @@ int gemm_combine(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su
   if (gemm_simple())
     hipLaunchKernelGGL(k_gemm_combine<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, A, G, J, U,
                        ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, ntx, nty, total);
-  else if (tile_of<T>() == 1)
+  else if (tile_of<T>(1) == 1)
     launch_tile(k_gemm_combine_tile<T, 4, 2>, 4, 2);
-  else if (tile_of<T>() == 2)
+  else if (tile_of<T>(1) == 2)
     launch_tile(k_gemm_combine_tile<T, 2, 4>, 2, 4);
-  else if (tile_of<T>() == 3)
+  else if (tile_of<T>(1) == 3)
     launch_tile(k_gemm_combine_tile<T, 4, 4>, 4, 4);
-  else if (tile_of<T>() == 4)
+  else if (tile_of<T>(1) == 4)
     launch_tile(k_gemm_combine_tile<T, 5, 2>, 5, 2);
-  else if (tile_of<T>() == 5)
+  else if (tile_of<T>(1) == 5)
     launch_tile(k_gemm_combine_tile<T, 5, 4>, 5, 4);
   else
     hipLaunchKernelGGL(k_gemm_combine_lds<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, A, G, J,

@@ -128,8 +128,8 @@ ApplyLayout apply_layout(const Plan* p, int nb, int precision);
 bool plane_fits_lds(const Plan* p, int precision);
 void set_gemm_target_wgs(int v);
 int gemm_target_wgs();
-void set_gemm_tile(int precision, int v);
-int gemm_tile(int precision);
+void set_gemm_tile(int precision, int op, int v);  // op 0 inner, 1 combine, other: both
+int gemm_tile(int precision, int op);
 void set_combine_lib(int precision, int v);
 int combine_lib(int precision);
 

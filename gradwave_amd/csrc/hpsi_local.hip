@@ -1372,10 +1372,12 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
     set_combine_lib(arg, value);
     return GW_OK;
   }
-  if (key == GW_TUNE_GEMM_TILE) {
+  if (key == GW_TUNE_GEMM_TILE) {  // arg = precision + 256 * (0 both, 1 inner, 2 combine)
+    const int prec = arg & 255, op = arg >> 8;
     GW_REQUIRE(value >= 0 && value <= 5, GW_ERR_ARG, "bad tile variant %d", value);
-    GW_REQUIRE(arg == GW_FP64 || arg == GW_FP32, GW_ERR_ARG, "bad precision %d", arg);
-    set_gemm_tile(arg, value);
+    GW_REQUIRE(prec == GW_FP64 || prec == GW_FP32, GW_ERR_ARG, "bad precision %d", prec);
+    GW_REQUIRE(op >= 0 && op <= 2, GW_ERR_ARG, "bad contraction selector %d", op);
+    set_gemm_tile(prec, op == 0 ? -1 : op - 1, value);
     return GW_OK;
   }
   GW_REQUIRE(false, GW_ERR_ARG, "unknown tuning key %d", key);
@@ -1384,7 +1386,10 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
 extern "C" int gw_tune_get(int key, int arg) {
   if (key == GW_TUNE_RADIX) return (arg > 0 && arg < 1024) ? g_radix_pref[arg] : -1;
   if (key == GW_TUNE_GEMM_WGS) return gemm_target_wgs();
-  if (key == GW_TUNE_GEMM_TILE) return (arg == GW_FP64 || arg == GW_FP32) ? gemm_tile(arg) : -1;
+  if (key == GW_TUNE_GEMM_TILE) {
+    const int prec = arg & 255, op = arg >> 8;
+    return (prec == GW_FP64 || prec == GW_FP32) && op >= 0 && op <= 2 ? gemm_tile(prec, op == 2 ? 1 : 0) : -1;
+  }
   if (key == GW_TUNE_XPASS) return xpass_encode(g_xpass[xp_index(arg == GW_FP64 ? GW_FP64 : GW_FP32)]);
   if (key == GW_TUNE_COMBINE_LIB) return (arg == GW_FP64 || arg == GW_FP32) ? combine_lib(arg) : -1;
   return -1;

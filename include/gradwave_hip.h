@@ -194,8 +194,10 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
  *   split used by plans created afterwards (0 = the default most balanced split).
  * GW_TUNE_GEMM_WGS: the number of workgroups the split-K inner products aim at
  *   (`arg` ignored; 0 = default 2048).
- * GW_TUNE_GEMM_TILE: MFMA blocks per wave of the complex GEMMs of precision `arg`
- *   (GW_FP64 / GW_FP32), 0 = 2x2, 1 = 4x2, 2 = 2x4, 3 = 4x4; 4 and 5 cover 160
+ * GW_TUNE_GEMM_TILE: MFMA blocks per wave of the complex GEMMs; `arg` = precision
+ *   (GW_FP64 / GW_FP32) + 256 * (0: both contractions, 1: inner only, 2: combine
+ *   only; gw_tune_get with 0 reports the inner one); value 0 = 2x2, 1 = 4x2,
+ *   2 = 2x4, 3 = 4x4; 4 and 5 cover 160
  *   bands (80 complex columns of the inner product, 160 rows of the
  *   back-projection) per workgroup: 4 = inner 2x5 / combine 5x2, 5 = inner 4x5 /
  *   combine 5x4.

@@ -130,6 +130,9 @@ def test_autotuner_choices_are_valid_and_exact(tmp_path, monkeypatch):
     tiles, wgs = autotune.tune_gemm(wls, reps=2, allow_library=False)
     assert wgs in autotune.GEMM_TARGETS and tiles["fp64"] in autotune.GEMM_TILES
     assert tiles["fp32"] in autotune.GEMM_TILES and tiles["combine_lib"] == {"fp64": 0, "fp32": 0}
+    assert set(tiles["combine"].values()) <= set(autotune.GEMM_TILES)
+    assert L.lib().gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32 + 512) == tiles["combine"]["fp32"]
+    assert L.lib().gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP64 + 256) == tiles["fp64"]
     for n, a in radix.items():
         assert a in autotune.candidate_splits(n)
     hams = _hams(wls)

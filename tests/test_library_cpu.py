@@ -54,6 +54,11 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert "unknown tuning key" in err(lib.gw_tune_set(99, 0, 0))
     assert "does not split" in err(lib.gw_tune_set(L.GW_TUNE_RADIX, 24, 5))
     assert "bad tile" in err(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32, 7))
+    assert "contraction selector" in err(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32 + 768, 0))
+    assert lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32 + 512, 4) == 0  # back-projection only
+    assert lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32 + 512) == 4
+    assert lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32) == 0
+    assert lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32, 0) == 0
     assert "x-pass shape" in err(lib.gw_tune_set(L.GW_TUNE_XPASS, 0, 305))
     assert lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP64) == 408
     assert lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP32) == 10416
