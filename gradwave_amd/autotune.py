@@ -5,7 +5,7 @@ the GEMMs (no reference counterpart: the reference uses torch.fft and BLAS).
   (a, b in {1..10, 12, 14, 15, 16}); plans take the most balanced split unless
   ``gw_tune_set(GW_TUNE_RADIX, n, a)`` says otherwise. ``tune_radix`` times
   the generic-size local-term kernels of real operators for each candidate a
-  and keeps the fastest.
+  (splits into factors <= 10 when there are any) and keeps the fastest.
 * GEMM tiles and split-K: the complex MFMA kernels run 2x2, 4x2, 2x4, 4x4 or
   band-sized (2x5 / 5x2, 4x5 / 5x4) 16x16 blocks per wave (``GW_TUNE_GEMM_TILE``,
   per precision and per contraction), and ``gw_zgemm_inner`` / ``gw_rgemm_inner``
@@ -41,7 +41,12 @@ def tune_file() -> str:
 
 
 def candidate_splits(n: int) -> list[int]:
-    return [a for a in range(1, 17) if n % a == 0 and a in CODELETS and n // a in CODELETS]
+    """First factors a of the two-codelet splits n = a * b; when splits with both
+    factors <= 10 exist only those are offered (the kernels built for codelets up
+    to 10 need far fewer registers than the ones carrying 12-16)."""
+    all_ = [a for a in range(1, 17) if n % a == 0 and a in CODELETS and n // a in CODELETS]
+    small = [a for a in all_ if a <= 10 and n // a <= 10]
+    return small or all_
 
 
 def current() -> dict:

@@ -258,11 +258,14 @@ __device__ __forceinline__ void line_pass(cpx<T>* __restrict__ lds, const cpx<T>
   }
 }
 
-template <int SIGN, bool TWIDDLE, bool LINES_FAST, typename T>
+// Runtime-length pass: the switch instantiates every codelet up to RMAX, and the
+// largest one sets the register budget of the whole kernel, so kernels whose
+// plans only split into factors <= 10 are built with RMAX = 10.
+template <int SIGN, bool TWIDDLE, bool LINES_FAST, typename T, int RMAX = 16>
 __device__ __forceinline__ void line_pass_rt(int np, cpx<T>* lds, const cpx<T>* tw, int nlines,
                                              int lstride, int es, int ntask, int A, int B,
                                              const int* line_map) {
-#define GW_LP(NN) case NN: line_pass<NN, SIGN, TWIDDLE, LINES_FAST, T>(lds, tw, nlines, lstride, es, ntask, A, B, line_map); break;
+#define GW_LP(NN) case NN: if constexpr (NN <= RMAX) line_pass<NN, SIGN, TWIDDLE, LINES_FAST, T>(lds, tw, nlines, lstride, es, ntask, A, B, line_map); break;
   switch (np) {
     GW_LP(1) GW_LP(2) GW_LP(3) GW_LP(4) GW_LP(5) GW_LP(6) GW_LP(7) GW_LP(8) GW_LP(9)
     GW_LP(10) GW_LP(12) GW_LP(14) GW_LP(15) GW_LP(16)
@@ -274,26 +277,26 @@ __device__ __forceinline__ void line_pass_rt(int np, cpx<T>* lds, const cpx<T>* 
 // Axis plan: n = n1f * n2f. Inverse (G->r, natural -> permuted), then the
 // matching forward (r->G, permuted -> natural). Barriers are the caller's
 // between transforms; one barrier sits between the two passes here.
-template <typename T, bool LINES_FAST>
+template <typename T, bool LINES_FAST, int RMAX = 16>
 __device__ __forceinline__ void line_fft_inverse(cpx<T>* lds, const cpx<T>* tw, int n1f, int n2f,
                                                  int nlines, int lstride, int es,
                                                  const int* line_map) {
   // pass 1: task j2 < n2f: DFT_n1f over a[n2f*q + j2], twiddle W^{+j2*k1}
-  line_pass_rt<+1, true, LINES_FAST, T>(n1f, lds, tw, nlines, lstride, es, n2f, n2f, 1, line_map);
+  line_pass_rt<+1, true, LINES_FAST, T, RMAX>(n1f, lds, tw, nlines, lstride, es, n2f, n2f, 1, line_map);
   __syncthreads();
   // pass 2: task k1 < n1f: DFT_n2f over a[n2f*k1 + q]
-  line_pass_rt<+1, false, LINES_FAST, T>(n2f, lds, tw, nlines, lstride, es, n1f, 1, n2f, line_map);
+  line_pass_rt<+1, false, LINES_FAST, T, RMAX>(n2f, lds, tw, nlines, lstride, es, n1f, 1, n2f, line_map);
 }
 
-template <typename T, bool LINES_FAST>
+template <typename T, bool LINES_FAST, int RMAX = 16>
 __device__ __forceinline__ void line_fft_forward(cpx<T>* lds, const cpx<T>* tw, int n1f, int n2f,
                                                  int nlines, int lstride, int es,
                                                  const int* line_map) {
   // pass 1: task a < n1f: DFT_n2f over a[n2f*a + q], twiddle W^{-a*k2}
-  line_pass_rt<-1, true, LINES_FAST, T>(n2f, lds, tw, nlines, lstride, es, n1f, 1, n2f, line_map);
+  line_pass_rt<-1, true, LINES_FAST, T, RMAX>(n2f, lds, tw, nlines, lstride, es, n1f, 1, n2f, line_map);
   __syncthreads();
   // pass 2: task k2 < n2f: DFT_n1f over a[n2f*q + k2]
-  line_pass_rt<-1, false, LINES_FAST, T>(n1f, lds, tw, nlines, lstride, es, n2f, n2f, 1, line_map);
+  line_pass_rt<-1, false, LINES_FAST, T, RMAX>(n1f, lds, tw, nlines, lstride, es, n2f, n2f, 1, line_map);
 }
 
 }  // namespace gw

@@ -79,6 +79,14 @@ bool split_length(int n, int* a, int* b) {
   return true;
 }
 
+// Largest codelet the plan's axis splits use: kernels built for radices <= 10
+// need far fewer registers than the full set (up to 16).
+static int plan_rmax(const Plan* p) {
+  int m = 1;
+  for (int ax = 0; ax < 3; ++ax) m = std::max(m, std::max(p->f[ax][0], p->f[ax][1]));
+  return m <= 10 ? 10 : 16;
+}
+
 constexpr int kMaxLds = 160 * 1024;
 constexpr int kPlaneThreads = 512;
 constexpr int kXThreads = 256;
@@ -199,7 +207,7 @@ __device__ __forceinline__ long xcd_work_id(long total) {
 
 // P1 fused: scatter -> inverse z (sticks) -> inverse y (all columns) -> store plane,
 // for x-plane a of band row bi (= k * nb + b); `tile` holds n2 * n3 elements.
-template <typename T>
+template <typename T, int RMAX = 16>
 __device__ void plane_inv_body(const Geom& g, int a, int bi, const cpx<T>* __restrict__ c, long ldc, long sc,
                                cpx<T>* __restrict__ box, const cpx<T>* __restrict__ twy,
                                const cpx<T>* __restrict__ twz, cpx<T>* tile) {
@@ -217,27 +225,27 @@ __device__ void plane_inv_body(const Geom& g, int a, int bi, const cpx<T>* __res
   for (int q = p0 + threadIdx.x; q < p1; q += blockDim.x) tile[yz[q]] = cb[src[q]];
   __syncthreads();
   if (s1 > s0) {
-    line_fft_inverse<T, false>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1,
+    line_fft_inverse<T, false, RMAX>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1,
                                g.st_i1 + (long)k * g.nst_max + s0);
     __syncthreads();
-    line_fft_inverse<T, true>(tile, twy, g.fy0, g.fy1, g.n3, 1, g.n3, nullptr);
+    line_fft_inverse<T, true, RMAX>(tile, twy, g.fy0, g.fy1, g.n3, 1, g.n3, nullptr);
     __syncthreads();
   }
   cpx<T>* dst = box + ((long)bi * g.na + a) * n23;
   for (int e = threadIdx.x; e < n23; e += blockDim.x) dst[e] = tile[e];
 }
 
-template <typename T>
+template <typename T, int RMAX = 16>
 __global__ void __launch_bounds__(kPlaneThreads)
 k_plane_inv(Geom g, const cpx<T>* __restrict__ c, long ldc, long sc, cpx<T>* __restrict__ box,
             const cpx<T>* __restrict__ twy, const cpx<T>* __restrict__ twz) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  plane_inv_body<T>(g, blockIdx.x, blockIdx.y, c, ldc, sc, box, twy, twz, reinterpret_cast<cpx<T>*>(smem_raw));
+  plane_inv_body<T, RMAX>(g, blockIdx.x, blockIdx.y, c, ldc, sc, box, twy, twz, reinterpret_cast<cpx<T>*>(smem_raw));
 }
 
 // P2: inverse x-FFT, x V(r), forward x-FFT over chunk `chunk` of L yz-lines for
 // band group `bgrp` (kXBandGroup bands); `tile` holds n1 * L elements.
-template <typename T>
+template <typename T, int RMAX = 16>
 __device__ void xpass_body(const Geom& g, int L, int nbt, int chunk, int bgrp, cpx<T>* __restrict__ box,
                            const T* __restrict__ vp, const cpx<T>* __restrict__ twx, cpx<T>* tile) {
   const int n23 = g.n2 * g.n3;
@@ -266,7 +274,7 @@ __device__ void xpass_body(const Geom& g, int L, int nbt, int chunk, int bgrp, c
       if (l < nl) tile[g.planes[a] * L + l] = bb[(long)a * n23 + l];
     }
     __syncthreads();
-    line_fft_inverse<T, true>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
+    line_fft_inverse<T, true, RMAX>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kXVpt; ++i) {
@@ -274,7 +282,7 @@ __device__ void xpass_body(const Geom& g, int L, int nbt, int chunk, int bgrp, c
       if (e < ne) { tile[e].x *= vr[i]; tile[e].y *= vr[i]; }
     }
     __syncthreads();
-    line_fft_forward<T, true>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
+    line_fft_forward<T, true, RMAX>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
     __syncthreads();
     for (int e = threadIdx.x; e < g.na * L; e += blockDim.x) {
       const int a = e / L, l = e - a * L;
@@ -284,12 +292,12 @@ __device__ void xpass_body(const Geom& g, int L, int nbt, int chunk, int bgrp, c
   }
 }
 
-template <typename T>
+template <typename T, int RMAX = 16>
 __global__ void __launch_bounds__(kXThreads)
 k_xpass(Geom g, int L, int nbt, cpx<T>* __restrict__ box, const T* __restrict__ vp,
         const cpx<T>* __restrict__ twx) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  xpass_body<T>(g, L, nbt, blockIdx.x, blockIdx.y, box, vp, twx, reinterpret_cast<cpx<T>*>(smem_raw));
+  xpass_body<T, RMAX>(g, L, nbt, blockIdx.x, blockIdx.y, box, vp, twx, reinterpret_cast<cpx<T>*>(smem_raw));
 }
 
 // P2 with wave-cooperative lines (gw_regfft.h): each x-line is spread over S
@@ -489,7 +497,7 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
 }
 
 // P3 fused: load plane -> forward y (all columns) -> forward z (sticks) -> gather (+ t c).
-template <typename T, bool KIN>
+template <typename T, bool KIN, int RMAX = 16>
 __device__ void plane_fwd_body(const Geom& g, int a, int bi, const cpx<T>* __restrict__ box,
                                const cpx<T>* __restrict__ c, long ldc, long sc, const T* __restrict__ t,
                                cpx<T>* __restrict__ out, long ldo, long so, T scale, const cpx<T>* __restrict__ twy,
@@ -508,9 +516,9 @@ __device__ void plane_fwd_body(const Geom& g, int a, int bi, const cpx<T>* __res
   const cpx<T>* srcb = box + ((long)bi * g.na + a) * n23;
   for (int e = threadIdx.x; e < n23; e += blockDim.x) tile[e] = srcb[e];
   __syncthreads();
-  line_fft_forward<T, true>(tile, twy, g.fy0, g.fy1, g.n3, 1, g.n3, nullptr);
+  line_fft_forward<T, true, RMAX>(tile, twy, g.fy0, g.fy1, g.n3, 1, g.n3, nullptr);
   __syncthreads();
-  line_fft_forward<T, false>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1,
+  line_fft_forward<T, false, RMAX>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1,
                              g.st_i1 + (long)k * g.nst_max + s0);
   __syncthreads();
   const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
@@ -530,14 +538,14 @@ __device__ void plane_fwd_body(const Geom& g, int a, int bi, const cpx<T>* __res
   }
 }
 
-template <typename T, bool KIN>
+template <typename T, bool KIN, int RMAX = 16>
 __global__ void __launch_bounds__(kPlaneThreads)
 k_plane_fwd(Geom g, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ c, long ldc,
             long sc, const T* __restrict__ t, cpx<T>* __restrict__ out, long ldo, long so,
             T scale, const cpx<T>* __restrict__ twy, const cpx<T>* __restrict__ twz) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  plane_fwd_body<T, KIN>(g, blockIdx.x, blockIdx.y, box, c, ldc, sc, t, out, ldo, so, scale, twy, twz,
-                         reinterpret_cast<cpx<T>*>(smem_raw));
+  plane_fwd_body<T, KIN, RMAX>(g, blockIdx.x, blockIdx.y, box, c, ldc, sc, t, out, ldo, so, scale, twy, twz,
+                               reinterpret_cast<cpx<T>*>(smem_raw));
 }
 
 // ---- unfused path (plane tile larger than LDS) ----
@@ -1569,6 +1577,18 @@ extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int
   (void)hipFuncSetAttribute((const void*)k_plane_fwd<float, false>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_xpass<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_xpass<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv<double, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv<float, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd<double, true, 10>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd<double, false, 10>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd<float, true, 10>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd<float, false, 10>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_xpass<double, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_xpass<float, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_x_to_real<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_x_to_real<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   set_wave_lds_attrs(want);
@@ -1716,9 +1736,14 @@ static int apply_t(gw_plan* h, int terms, const cpx<T>* c, long ldc, long sc, cp
     } else {
       const size_t plane_lds = (size_t)P.n[1] * P.n[2] * sizeof(cpx<T>);
       ev = tm.begin();
+      const bool r10 = plan_rmax(&P) <= 10;
       if (L.fused) {
-        hipLaunchKernelGGL(k_plane_inv<T>, dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, c, ldc, sc,
-                           box, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
+        if (r10)
+          hipLaunchKernelGGL((k_plane_inv<T, 10>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, c, ldc,
+                             sc, box, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
+        else
+          hipLaunchKernelGGL(k_plane_inv<T>, dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, c, ldc, sc,
+                             box, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
       } else {
         cpx<T>* sticks = (cpx<T>*)(ws + L.stick_off);
         hipLaunchKernelGGL(k_sticks_inv<T>, dim3((P.nst_max + kStickChunk - 1) / kStickChunk, nbt), dim3(256),
@@ -1729,15 +1754,27 @@ static int apply_t(gw_plan* h, int terms, const cpx<T>* c, long ldc, long sc, cp
       tm.end(ST_PLANE_INV, ev);
       const int XL = x_lines(&P);
       ev = tm.begin();
+      if (r10)
+        hipLaunchKernelGGL((k_xpass<T, 10>),
+                           dim3((P.n[1] * P.n[2] + XL - 1) / XL, (nbt + kXBandGroup - 1) / kXBandGroup),
+                           dim3(kXThreads), (size_t)P.n[0] * XL * sizeof(cpx<T>), st, g, XL, nbt, box, vp,
+                           tw_of<T>(&P, 0));
+      else
         hipLaunchKernelGGL(k_xpass<T>, dim3((P.n[1] * P.n[2] + XL - 1) / XL, (nbt + kXBandGroup - 1) / kXBandGroup),
                            dim3(kXThreads), (size_t)P.n[0] * XL * sizeof(cpx<T>), st, g, XL, nbt, box, vp,
                            tw_of<T>(&P, 0));
       tm.end(ST_XPASS, ev);
       ev = tm.begin();
       if (L.fused) {
-        if (kin)
+        if (kin && r10)
+          hipLaunchKernelGGL((k_plane_fwd<T, true, 10>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g,
+                             box, c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
+        else if (kin)
           hipLaunchKernelGGL((k_plane_fwd<T, true>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, box,
                              c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
+        else if (r10)
+          hipLaunchKernelGGL((k_plane_fwd<T, false, 10>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g,
+                             box, c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
         else
           hipLaunchKernelGGL((k_plane_fwd<T, false>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, box,
                              c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
@@ -1964,7 +2001,7 @@ __device__ __forceinline__ int rag_find(const RagInst<T>* R, int n, long w, bool
   return i;
 }
 
-template <typename T>
+template <typename T, int RMAX = 16>
 __global__ void __launch_bounds__(kPlaneThreads) k_plane_inv_rag(const RagInst<T>* __restrict__ R, int n, long total) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const long w = xcd_work_id(total);
@@ -1972,10 +2009,10 @@ __global__ void __launch_bounds__(kPlaneThreads) k_plane_inv_rag(const RagInst<T
   const RagInst<T>& d = R[rag_find(R, n, w, false)];
   const long local = w - d.w13;
   const int a = (int)(local % d.g.na), bi = (int)(local / d.g.na);
-  plane_inv_body<T>(d.g, a, bi, d.c, d.ldc, d.sc, d.box, d.twy, d.twz, reinterpret_cast<cpx<T>*>(smem_raw));
+  plane_inv_body<T, RMAX>(d.g, a, bi, d.c, d.ldc, d.sc, d.box, d.twy, d.twz, reinterpret_cast<cpx<T>*>(smem_raw));
 }
 
-template <typename T>
+template <typename T, int RMAX = 16>
 __global__ void __launch_bounds__(kXThreads) k_xpass_rag(const RagInst<T>* __restrict__ R, int n, long total) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const long w = xcd_work_id(total);
@@ -1983,10 +2020,10 @@ __global__ void __launch_bounds__(kXThreads) k_xpass_rag(const RagInst<T>* __res
   const RagInst<T>& d = R[rag_find(R, n, w, true)];
   const long local = w - d.w2;
   const int chunk = (int)(local % d.nchunk), bgrp = (int)(local / d.nchunk);
-  xpass_body<T>(d.g, d.XL, d.nbt, chunk, bgrp, d.box, d.vp, d.twx, reinterpret_cast<cpx<T>*>(smem_raw));
+  xpass_body<T, RMAX>(d.g, d.XL, d.nbt, chunk, bgrp, d.box, d.vp, d.twx, reinterpret_cast<cpx<T>*>(smem_raw));
 }
 
-template <typename T>
+template <typename T, int RMAX = 16>
 __global__ void __launch_bounds__(kPlaneThreads) k_plane_fwd_rag(const RagInst<T>* __restrict__ R, int n, long total) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const long w = xcd_work_id(total);
@@ -1996,10 +2033,11 @@ __global__ void __launch_bounds__(kPlaneThreads) k_plane_fwd_rag(const RagInst<T
   const int a = (int)(local % d.g.na), bi = (int)(local / d.g.na);
   cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
   if (d.kin)
-    plane_fwd_body<T, true>(d.g, a, bi, d.box, d.c, d.ldc, d.sc, d.t, d.out, d.ldo, d.so, d.scale, d.twy, d.twz, tile);
+    plane_fwd_body<T, true, RMAX>(d.g, a, bi, d.box, d.c, d.ldc, d.sc, d.t, d.out, d.ldo, d.so, d.scale, d.twy, d.twz,
+                                  tile);
   else
-    plane_fwd_body<T, false>(d.g, a, bi, d.box, d.c, d.ldc, d.sc, d.t, d.out, d.ldo, d.so, d.scale, d.twy, d.twz,
-                             tile);
+    plane_fwd_body<T, false, RMAX>(d.g, a, bi, d.box, d.c, d.ldc, d.sc, d.t, d.out, d.ldo, d.so, d.scale, d.twy,
+                                   d.twz, tile);
 }
 
 }  // namespace gw
@@ -2009,7 +2047,7 @@ struct gw_batch {
   void* table = nullptr;  // device copy of the descriptors
   size_t table_cap = 0;
   // bound call
-  int prec = -1, terms = 0;
+  int prec = -1, terms = 0, rmax = 16;
   long tot13 = 0, tot2 = 0;
   size_t lds13 = 0, lds2 = 0;
   struct Bound {
@@ -2050,6 +2088,12 @@ extern "C" int gw_batch_create(gw_batch** out, gw_plan* const* plans, int n) {
   (void)hipFuncSetAttribute((const void*)k_plane_fwd_rag<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_xpass_rag<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_xpass_rag<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_rag<double, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_rag<float, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_rag<double, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_rag<float, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_xpass_rag<double, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_xpass_rag<float, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   *out = b;
   return GW_OK;
 }
@@ -2123,6 +2167,8 @@ static int batch_bind_t(gw_batch* b, int terms, const void* const* c, const long
   GW_CHECK_HIP(hipMemcpy(b->table, R.data(), (size_t)n * sizeof(RagInst<T>), hipMemcpyHostToDevice));
   b->prec = prec;
   b->terms = terms;
+  b->rmax = 1;
+  for (gw_plan* pl : b->plans) b->rmax = std::max(b->rmax, plan_rmax(&pl->p));
   b->tot13 = w13;
   b->tot2 = w2;
   b->lds13 = lds13;
@@ -2135,12 +2181,16 @@ static int batch_apply_t(gw_batch* b, hipStream_t st) {
   const int n = (int)b->plans.size();
   const RagInst<T>* R = (const RagInst<T>*)b->table;
   if (b->tot13 == 0) return GW_OK;
-  hipLaunchKernelGGL(k_plane_inv_rag<T>, dim3((unsigned)(8 * ((b->tot13 + 7) / 8))), dim3(kPlaneThreads), b->lds13,
-                     st, R, n, b->tot13);
-  hipLaunchKernelGGL(k_xpass_rag<T>, dim3((unsigned)(8 * ((b->tot2 + 7) / 8))), dim3(kXThreads), b->lds2, st, R, n,
-                     b->tot2);
-  hipLaunchKernelGGL(k_plane_fwd_rag<T>, dim3((unsigned)(8 * ((b->tot13 + 7) / 8))), dim3(kPlaneThreads), b->lds13,
-                     st, R, n, b->tot13);
+  const dim3 g13((unsigned)(8 * ((b->tot13 + 7) / 8))), g2((unsigned)(8 * ((b->tot2 + 7) / 8)));
+  if (b->rmax <= 10) {
+    hipLaunchKernelGGL((k_plane_inv_rag<T, 10>), g13, dim3(kPlaneThreads), b->lds13, st, R, n, b->tot13);
+    hipLaunchKernelGGL((k_xpass_rag<T, 10>), g2, dim3(kXThreads), b->lds2, st, R, n, b->tot2);
+    hipLaunchKernelGGL((k_plane_fwd_rag<T, 10>), g13, dim3(kPlaneThreads), b->lds13, st, R, n, b->tot13);
+  } else {
+    hipLaunchKernelGGL(k_plane_inv_rag<T>, g13, dim3(kPlaneThreads), b->lds13, st, R, n, b->tot13);
+    hipLaunchKernelGGL(k_xpass_rag<T>, g2, dim3(kXThreads), b->lds2, st, R, n, b->tot2);
+    hipLaunchKernelGGL(k_plane_fwd_rag<T>, g13, dim3(kPlaneThreads), b->lds13, st, R, n, b->tot13);
+  }
   GW_CHECK_HIP(hipGetLastError());
   if (!(b->terms & GW_TERM_NONLOCAL)) return GW_OK;
   const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
