@@ -88,6 +88,7 @@ static int plan_rmax(const Plan* p) {
 }
 
 constexpr int kMaxLds = 160 * 1024;
+constexpr int kXTilePad = 0;  // extra elements per row of the staged x-pass tiles (1, 2, 4 measured slower)
 constexpr int kPlaneThreads = 512;
 constexpr int kXThreads = 256;
 constexpr int kXVpt = 16;      // V values cached per thread in the x pass
@@ -391,6 +392,7 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
   constexpr int LC = NWX * LW;                  // lines per workgroup
   constexpr int NT = 64 * NWX;
   constexpr int KS = (N * LC + NT - 1) / NT;    // staged elements per thread (na <= N rows)
+  constexpr int LDT = LC + kXTilePad;           // tile row stride
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* E = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* tw = E + NWX * W::E_PER_WAVE;
@@ -402,25 +404,26 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
   const int nbg = min(bgroup, nbt - b0);
   if (nbg <= 0) return;
   const int na = g.na, n23 = g.n2 * g.n3;
-  const int tstride = (na + 2) * LC;             // one tile: rows 0..na-1, zero row na, discard row na+1
+  const int tstride = (na + 2) * LDT;            // one tile: rows 0..na-1, zero row na, discard row na+1
   if constexpr (!W::PFA) {
     for (int i = threadIdx.x; i < N; i += NT) tw[i] = twx[i];
   }
   for (int i = threadIdx.x; i < LC; i += NT) {
-    tiles[na * LC + i] = czero<T>();
-    tiles[tstride + na * LC + i] = czero<T>();
+    tiles[na * LDT + i] = czero<T>();
+    tiles[tstride + na * LDT + i] = czero<T>();
   }
   const int j0 = chunk * LC;
   // staging map: thread element k is tile entry e = threadIdx.x + k*NT = a*LC + jj
   // <-> box[band][a][j0 + jj]; entries past the tile or the last line are masked
   const int ne = na * LC;
   const int ks = (ne + NT - 1) / NT;  // staged elements per thread actually needed (uniform)
-  int soff[KS];
+  int soff[KS], tix[KS];
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
     const int e = threadIdx.x + k * NT;
     const int a = e / LC, jj = e - a * LC;
     soff[k] = (e < ne && j0 + jj < n23) ? (a * n23 + j0 + jj) * (int)sizeof(cpx<T>) : kBufOff;
+    tix[k] = e < ne ? a * LDT + jj : 0;
   }
   const unsigned band_bytes = (unsigned)((long)na * n23 * sizeof(cpx<T>));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -436,8 +439,8 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
 #pragma unroll
   for (int q = 0; q < M; ++q) {
     const int sl = part ? slot(W::gpos(t, q)) : -1;
-    rd[q] = (sl >= 0 ? sl : na) * LC + min(col, LC - 1);
-    wr[q] = (sl >= 0 ? sl : na + 1) * LC + min(col, LC - 1);
+    rd[q] = (sl >= 0 ? sl : na) * LDT + min(col, LC - 1);
+    wr[q] = (sl >= 0 ? sl : na + 1) * LDT + min(col, LC - 1);
   }
   T vr[S];
 #pragma unroll
@@ -451,7 +454,7 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       const int e = threadIdx.x + k * NT;
-      if (k < ks && e < ne) tiles[e] = st[k];
+      if (k < ks && e < ne) tiles[tix[k]] = st[k];
     }
   }
   // V(r) and the first tile have landed on every path: no load is pending at the
@@ -482,15 +485,13 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
     {
       const auto r = buf_rsrc(box + (long)bi * na * n23, band_bytes);
 #pragma unroll
-      for (int k = 0; k < KS; ++k) {
-        const int e = threadIdx.x + k * NT;
-        if (k < ks) buf_store<T>(cur[min(e, ne - 1)], r, soff[k]);
-      }
+      for (int k = 0; k < KS; ++k)
+        if (k < ks) buf_store<T>(cur[tix[k]], r, soff[k]);
     }
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       const int e = threadIdx.x + k * NT;
-      if (k < ks && e < ne) nxt[e] = st[k];
+      if (k < ks && e < ne) nxt[tix[k]] = st[k];
     }
     __syncthreads();
   }
@@ -987,7 +988,7 @@ static int launch_wave_path_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, 
   const long xtotal = (long)nchunk * ((nbt + bgroup - 1) / bgroup);
   const dim3 xgrid((unsigned)(8 * ((xtotal + 7) / 8)));
   const size_t stage_lds =
-      ((size_t)nwx * W::E_PER_WAVE + N + 2 * (size_t)(P.na + 2) * lines_per_wg) * sizeof(cpx<T>);
+      ((size_t)nwx * W::E_PER_WAVE + N + 2 * (size_t)(P.na + 2) * (lines_per_wg + kXTilePad)) * sizeof(cpx<T>);
   if (xs.mode == 1 && stage_lds <= (size_t)std::min(P.lds_max, kMaxLds)) {
     if (nwx == 8)
       hipLaunchKernelGGL((k_xpass_stage<T, N, C::M, C::S, 8>), xgrid, dim3(512), stage_lds, st, g, nbt, bgroup,
