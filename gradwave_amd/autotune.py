@@ -60,6 +60,8 @@ def current() -> dict:
                                   "fp32": lib.gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32 + 512)},
             "combine_lib": {"fp64": lib.gw_tune_get(L.GW_TUNE_COMBINE_LIB, L.GW_FP64),
                             "fp32": lib.gw_tune_get(L.GW_TUNE_COMBINE_LIB, L.GW_FP32)},
+            "inner_lib": {"fp64": lib.gw_tune_get(L.GW_TUNE_INNER_LIB, L.GW_FP64),
+                          "fp32": lib.gw_tune_get(L.GW_TUNE_INNER_LIB, L.GW_FP32)},
             "xpass": {"fp64": lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP64),
                       "fp32": lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP32)}}
 
@@ -78,6 +80,8 @@ def apply(choices: dict) -> None:
                     "gw_tune_set")
         if name in choices.get("combine_lib", {}):
             L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, int(choices["combine_lib"][name])), "gw_tune_set")
+        if name in choices.get("inner_lib", {}):
+            L.check(lib.gw_tune_set(L.GW_TUNE_INNER_LIB, prec, int(choices["inner_lib"][name])), "gw_tune_set")
         if name in choices.get("xpass", {}):
             L.check(lib.gw_tune_set(L.GW_TUNE_XPASS, prec, int(choices["xpass"][name])), "gw_tune_set")
 
@@ -150,10 +154,10 @@ def tune_gemm(workloads, nb: int = 32, reps: int = 5, allow_library: bool = True
     """Tune the projector products (nonlocal term) of ``workloads`` on the plan's
     per-stage HIP-event times: per precision, the inner-product tile on the becp
     stage and the back-projection tile on the backproj stage, each on its own
-    (with ``allow_library``, rocBLAS is also tried for the back-projection); then
+    (with ``allow_library``, rocBLAS is also tried for both contractions); then
     the split-K workgroup target on the becp stage of both precisions. Returns
     ({"fp64": inner tile, "fp32": inner tile, "combine": {"fp64": tile, "fp32": tile},
-    "combine_lib": {"fp64": 0|1, "fp32": 0|1}}, target)."""
+    "combine_lib": {"fp64": 0|1, "fp32": 0|1}, "inner_lib": {...}}, target)."""
     from gradwave_amd.hamiltonian import BatchedHamiltonian
 
     lib = L.lib()
@@ -183,10 +187,11 @@ def tune_gemm(workloads, nb: int = 32, reps: int = 5, allow_library: bool = True
                 tot[k] += st[k][0] / max(st[k][1], 1)
         return tot
 
-    tiles, combine, libs = {}, {}, {}
+    tiles, combine, libs, ilibs = {}, {}, {}, {}
     try:
         for name, prec in (("fp64", L.GW_FP64), ("fp32", L.GW_FP32)):
             L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 0), "gw_tune_set")
+            L.check(lib.gw_tune_set(L.GW_TUNE_INNER_LIB, prec, 0), "gw_tune_set")
             best = {"becp": (0, float("inf")), "backproj": (0, float("inf"))}
             for tile in GEMM_TILES:
                 L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, tile), "gw_tune_set")
@@ -197,12 +202,15 @@ def tune_gemm(workloads, nb: int = 32, reps: int = 5, allow_library: bool = True
             tiles[name], combine[name] = best["becp"][0], best["backproj"][0]
             L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec + 256, tiles[name]), "gw_tune_set")
             L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec + 512, combine[name]), "gw_tune_set")
-            libs[name] = 0
+            libs[name] = ilibs[name] = 0
             if allow_library:
                 L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 1), "gw_tune_set")
-                if stage_ms(name)["backproj"] < best["backproj"][1]:
-                    libs[name] = 1
+                L.check(lib.gw_tune_set(L.GW_TUNE_INNER_LIB, prec, 1), "gw_tune_set")
+                t = stage_ms(name)
+                libs[name] = int(t["backproj"] < best["backproj"][1])
+                ilibs[name] = int(t["becp"] < best["becp"][1])
                 L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, libs[name]), "gw_tune_set")
+                L.check(lib.gw_tune_set(L.GW_TUNE_INNER_LIB, prec, ilibs[name]), "gw_tune_set")
         best_target, best_ms = None, float("inf")
         for target in GEMM_TARGETS:
             L.check(lib.gw_tune_set(L.GW_TUNE_GEMM_WGS, 0, target), "gw_tune_set")
@@ -215,6 +223,7 @@ def tune_gemm(workloads, nb: int = 32, reps: int = 5, allow_library: bool = True
             h.set_flags(0)
     tiles["combine"] = combine
     tiles["combine_lib"] = libs
+    tiles["inner_lib"] = ilibs
     return tiles, best_target
 
 

@@ -174,14 +174,14 @@ __device__ __forceinline__ double reduce_splits(const T* __restrict__ pk, long s
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_gemm_inner_reduce(int M, int N, int splits, const T* __restrict__ part, cpx<T>* __restrict__ C, long ldc,
-                    long sc) {
+                    long sc, int conj) {
   const int k = blockIdx.y;
   const long nreal = (long)M * 2 * N;
   const long e = (long)blockIdx.x * 64 + (threadIdx.x & 63);  // real element i*2N + 2j + c
   const double v = reduce_splits<T>(part + (long)k * splits * nreal, nreal, e, e < nreal, splits);
   if (threadIdx.x < 64 && e < nreal) {
     const long i = e / (2 * N), r = e - i * 2 * N;
-    reinterpret_cast<T*>(C + (long)k * sc + i * ldc + (r >> 1))[r & 1] = (T)v;
+    reinterpret_cast<T*>(C + (long)k * sc + i * ldc + (r >> 1))[r & 1] = (T)((conj && (r & 1)) ? -v : v);
   }
 }
 
@@ -622,6 +622,49 @@ size_t gemm_inner_partials_bytes(int nk, int M, int N, int K) {
 template size_t gemm_inner_partials_bytes<double>(int, int, int, int);
 template size_t gemm_inner_partials_bytes<float>(int, int, int, int);
 
+static int g_inner_lib[2] = {0, 0};
+void set_inner_lib(int precision, int v) { g_inner_lib[precision == GW_FP32] = v ? 1 : 0; }
+int inner_lib(int precision) { return g_inner_lib[precision == GW_FP32]; }
+static rocblas_handle blas_handle();
+
+// Split-K inner product on rocBLAS: one strided-batched GEMM over the K-chunks
+// (plus one for a short last chunk) writes conj(C) partials [chunk][M][N] in the
+// compute type, in the layout k_gemm_inner_reduce sums (with conj = 1). Per chunk,
+// column-major: conj(C)^T (N x M) = Y_chunk^H (N x Kc) X_chunk (Kc x M).
+template <typename T>
+static int inner_rocblas(int k, int M, int N, int K, int kc, int nfull, const cpx<T>* X, long ldx,
+                         const cpx<T>* Y, long ldy, T* part, hipStream_t st) {
+  rocblas_handle h = blas_handle();
+  GW_REQUIRE(h, GW_ERR_HIP, "rocblas handle unavailable");
+  GW_REQUIRE(rocblas_set_stream(h, st) == rocblas_status_success, GW_ERR_HIP, "rocblas_set_stream failed");
+  const long slot = (long)M * N;  // complex elements per partial
+  const int rem = K - nfull * kc;
+  rocblas_status rs = rocblas_status_success;
+  for (int pass = 0; pass < 2 && rs == rocblas_status_success; ++pass) {
+    const int cnt = pass == 0 ? nfull : (rem > 0 ? 1 : 0);
+    const int kk = pass == 0 ? kc : rem;
+    if (cnt == 0 || kk == 0) continue;
+    const long off = pass == 0 ? 0 : (long)nfull * kc;
+    cpx<T>* cp = reinterpret_cast<cpx<T>*>(part) + (pass == 0 ? 0 : (long)nfull * slot);
+    if constexpr (sizeof(T) == 8) {
+      const rocblas_double_complex one{1.0, 0.0}, zero{0.0, 0.0};
+      rs = rocblas_zgemm_strided_batched(h, rocblas_operation_conjugate_transpose, rocblas_operation_none, N, M, kk,
+                                         &one, (const rocblas_double_complex*)(Y + off), (rocblas_int)ldy, kc,
+                                         (const rocblas_double_complex*)(X + off), (rocblas_int)ldx, kc, &zero,
+                                         (rocblas_double_complex*)cp, N, slot, cnt);
+    } else {
+      const rocblas_float_complex one{1.0f, 0.0f}, zero{0.0f, 0.0f};
+      rs = rocblas_cgemm_strided_batched(h, rocblas_operation_conjugate_transpose, rocblas_operation_none, N, M, kk,
+                                         &one, (const rocblas_float_complex*)(Y + off), (rocblas_int)ldy, kc,
+                                         (const rocblas_float_complex*)(X + off), (rocblas_int)ldx, kc, &zero,
+                                         (rocblas_float_complex*)cp, N, slot, cnt);
+    }
+  }
+  (void)k;
+  GW_REQUIRE(rs == rocblas_status_success, GW_ERR_HIP, "rocblas inner gemm failed (%d)", (int)rs);
+  return GW_OK;
+}
+
 template <typename T>
 int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, const cpx<T>* Y, long ldy,
                long sy, cpx<T>* C, long ldc, long sc, void* partials, size_t part_bytes, hipStream_t st) {
@@ -636,6 +679,20 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
   GW_REQUIRE(partials && part_bytes >= need, GW_ERR_WORKSPACE, "gemm_inner workspace %zu < %zu", part_bytes, need);
   int kchunk = (K + splits - 1) / splits;
   kchunk = ((kchunk + kKC - 1) / kKC) * kKC;  // multiple of both K granules (16, 32)
+  if (g_inner_lib[sizeof(T) == 4] && ldx < (1L << 31) && ldy < (1L << 31)) {
+    // chunks of kchunk; the partial slots per k are the chunks actually used
+    const int used = (K + kchunk - 1) / kchunk, nfull = K / kchunk;
+    for (int k = 0; k < nk; ++k) {
+      const int rc = inner_rocblas<T>(k, M, N, K, kchunk, nfull, X + (long)k * sx, ldx, Y + (long)k * sy, ldy,
+                                      (T*)partials + (long)k * used * M * 2 * N, st);
+      if (rc) return rc;
+    }
+    const long nreal = (long)M * 2 * N;
+    hipLaunchKernelGGL(k_gemm_inner_reduce<T>, dim3((unsigned)((nreal + 63) / 64), nk), dim3(256), 0, st, M, N,
+                       used, (const T*)partials, C, ldc, sc, 1);
+    GW_CHECK_HIP(hipGetLastError());
+    return GW_OK;
+  }
   const int ntx = (2 * N + kTile - 1) / kTile, nty = (M + kTile - 1) / kTile;
   const long total = (long)ntx * nty * nk * splits;
   auto launch_tile = [&](auto kern, int wm, int wn) {
@@ -662,7 +719,7 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
                        ldx, sx, Y, ldy, sy, splits, kchunk, (T*)partials, ntx, nty, total);
   const long nreal = (long)M * 2 * N;
   hipLaunchKernelGGL(k_gemm_inner_reduce<T>, dim3((unsigned)((nreal + 63) / 64), nk), dim3(256), 0, st, M, N,
-                     splits, (const T*)partials, C, ldc, sc);
+                     splits, (const T*)partials, C, ldc, sc, 0);
   GW_CHECK_HIP(hipGetLastError());
   return GW_OK;
 }

@@ -132,6 +132,8 @@ void set_gemm_tile(int precision, int op, int v);  // op 0 inner, 1 combine, oth
 int gemm_tile(int precision, int op);
 void set_combine_lib(int precision, int v);
 int combine_lib(int precision);
+void set_inner_lib(int precision, int v);
+int inner_lib(int precision);
 
 // Complex GEMM launchers (gemm.hip). All matrices complex interleaved, row-major.
 //   inner:   C[k](M x N) = alpha * X[k]^H-rows . Y[k]-rows  i.e. C[i][j] = sum_g conj(X[i][g]) Y[j][g]

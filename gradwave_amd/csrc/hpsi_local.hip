@@ -1381,6 +1381,12 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
     set_combine_lib(arg, value);
     return GW_OK;
   }
+  if (key == GW_TUNE_INNER_LIB) {
+    GW_REQUIRE(arg == GW_FP64 || arg == GW_FP32, GW_ERR_ARG, "bad precision %d", arg);
+    GW_REQUIRE(value == 0 || value == 1, GW_ERR_ARG, "bad inner-product choice %d", value);
+    set_inner_lib(arg, value);
+    return GW_OK;
+  }
   if (key == GW_TUNE_GEMM_TILE) {  // arg = precision + 256 * (0 both, 1 inner, 2 combine)
     const int prec = arg & 255, op = arg >> 8;
     GW_REQUIRE(value >= 0 && value <= 5, GW_ERR_ARG, "bad tile variant %d", value);
@@ -1401,6 +1407,7 @@ extern "C" int gw_tune_get(int key, int arg) {
   }
   if (key == GW_TUNE_XPASS) return xpass_encode(g_xpass[xp_index(arg == GW_FP64 ? GW_FP64 : GW_FP32)]);
   if (key == GW_TUNE_COMBINE_LIB) return (arg == GW_FP64 || arg == GW_FP32) ? combine_lib(arg) : -1;
+  if (key == GW_TUNE_INNER_LIB) return (arg == GW_FP64 || arg == GW_FP32) ? inner_lib(arg) : -1;
   return -1;
 }
 extern "C" int gw_version(void) { return 1; }

@@ -131,6 +131,7 @@ def test_autotuner_choices_are_valid_and_exact(tmp_path, monkeypatch):
     assert wgs in autotune.GEMM_TARGETS and tiles["fp64"] in autotune.GEMM_TILES
     assert tiles["fp32"] in autotune.GEMM_TILES and tiles["combine_lib"] == {"fp64": 0, "fp32": 0}
     assert set(tiles["combine"].values()) <= set(autotune.GEMM_TILES)
+    assert tiles["inner_lib"] == {"fp64": 0, "fp32": 0}
     assert L.lib().gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP32 + 512) == tiles["combine"]["fp32"]
     assert L.lib().gw_tune_get(L.GW_TUNE_GEMM_TILE, L.GW_FP64 + 256) == tiles["fp64"]
     for n, a in radix.items():

@@ -176,3 +176,31 @@ def test_combine_on_rocblas_matches(dtype):
                 assert float((z.cpu().to(torch.complex128) - want).abs().max()) < TOL[dtype] * scale
     finally:
         lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 0)
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_inner_on_rocblas_matches(dtype):
+    """The autotuner may run the split-K chunks of the inner product on rocBLAS;
+    shapes cover a short last chunk, K below one chunk, batches and strided rows."""
+    from gradwave_amd import _lib as L
+
+    say(f"inner on rocBLAS {dtype}")
+    lib = L.lib()
+    prec = L.precision_of(dtype)
+    try:
+        assert lib.gw_tune_set(L.GW_TUNE_INNER_LIB, prec, 1) == 0
+        for (nk, M, N, K) in [(1, 256, 160, 37073), (3, 17, 33, 1000), (2, 70, 5, 4097), (1, 9, 7, 5)]:
+            x, y = _rand(nk, M, K, seed=1, dtype=dtype), _rand(nk, N, K, seed=2, dtype=dtype)
+            want = torch.matmul(x.to(torch.complex128).conj(), y.to(torch.complex128).mT)
+            got = LA.inner(x.to(dev()), y.to(dev())).cpu().to(torch.complex128)
+            scale = float((x.abs() ** 2).sum(-1).max().sqrt() * (y.abs() ** 2).sum(-1).max().sqrt())
+            assert float((got - want).abs().max()) < TOL[dtype] * scale
+        # strided rows: every other row of a wider block
+        xw, yw = _rand(1, 40, 3000, seed=6, dtype=dtype), _rand(1, 24, 3000, seed=7, dtype=dtype)
+        xs, ys = xw[:, ::2], yw[:, ::2]
+        want = torch.matmul(xs.to(torch.complex128).conj(), ys.to(torch.complex128).mT)
+        got = LA.inner(xw.to(dev())[:, ::2], yw.to(dev())[:, ::2]).cpu().to(torch.complex128)
+        scale = float((xs.abs() ** 2).sum(-1).max().sqrt() * (ys.abs() ** 2).sum(-1).max().sqrt())
+        assert float((got - want).abs().max()) < TOL[dtype] * scale
+    finally:
+        lib.gw_tune_set(L.GW_TUNE_INNER_LIB, prec, 0)
