@@ -7,17 +7,20 @@
 //  inner   C[i][j] = sum_g conj(X[i][g]) Y[j][g]            (long K = npw)
 //          D (M x 2N) = Xr (M x 2K) . B (2K x 2N),
 //          B[kk][2j] = Yr_j[kk],  B[kk][2j+1] = (Yi, -Yr)_j[kk]
-//          split-K over the grid, fp64 partial sums reduced by a second kernel.
+//          split-K over the grid; partials in the compute type, summed in fp64
+//          by a second kernel (or the K-chunks as one rocBLAS batched GEMM).
 //  combine Z[a][g] = beta Z + sum_j U'[j][a] Y[j][g]        (long N = npw)
 //          Zr (A x 2G) = U2 (A x 2J) . E (2J x 2G),
 //          U2[a][2j] = Re U'[j][a], U2[a][2j+1] = Im U'[j][a],
 //          E[2j][2g+c] = (Yr, Yi)[c], E[2j+1][2g+c] = (-Yi, Yr)[c]
 //
 // The reduction index of an MFMA step is free to permute as long as A and B
-// agree, so each lane loads V consecutive reals of its row (a 64-byte burst
-// for fp64) and feeds them to V consecutive MFMA steps. Operands stream from
-// global memory straight into VGPRs (no LDS staging yet). MFMA 16x16x4 lane
-// layout: A[i=l%16][k=l/16], B[k=l/16][j=l%16], D col l%16 and row Mfma<T>::row.
+// agree, so each lane reads V consecutive reals of its row and feeds them to V
+// consecutive MFMA steps. k_gemm_inner / k_gemm_combine stream operands from
+// global memory straight into VGPRs; the _lds and _tile kernels stage 32-complex
+// K-chunks of both operands in padded LDS tiles (next chunk prefetched into
+// registers). MFMA 16x16x4 lane layout: A[i=l%16][k=l/16], B[k=l/16][j=l%16],
+// D col l%16 and row Mfma<T>::row.
 #include <algorithm>
 #include <cstdlib>
 #include <mutex>
