@@ -21,6 +21,7 @@ Modules and the reference files they restate (paths under the reference's
                       solvers/_ms.py:26-52, solvers/registry.py:86-182
 * ``systems``      -- the synthetic Si/Al workloads (cells, k-lists, jitter)
 * ``dense``        -- brute-force dense H matrix (closed-form check of the apply)
+* ``generalized``  -- scf/uspp_batch.py:27-233 (S-metric generalized Davidson)
 * ``energies``     -- band energy and frozen-potential forces
                       (core/energies/{local_pp,nl_pp}.py, postscf/forces.py:138-223)
 
