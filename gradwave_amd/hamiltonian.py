@@ -210,8 +210,8 @@ class BatchedHamiltonian:
         if not _TOEPLITZ_ON_GPU:
             return False
         key = (self.plan.nk, self.plan.npw_max, self.shape, c.dtype)
-        verdict = _TOEP_TRIAL.get(key)
-        if verdict is None:
+        use_toep = _TOEP_TRIAL.get(key)
+        if use_toep is None:
             def timed(f):
                 f()
                 torch.cuda.synchronize()
@@ -222,11 +222,11 @@ class BatchedHamiltonian:
 
             t_fft = timed(lambda: self.apply_terms(c, L.GW_TERM_LOCAL))
             t_toep = timed(lambda: self.local_toeplitz(c))
-            verdict = t_toep < _TOEP_TRIAL_MARGIN * t_fft
-            _TOEP_TRIAL[key] = verdict
-            if not verdict:
+            use_toep = t_toep < _TOEP_TRIAL_MARGIN * t_fft
+            _TOEP_TRIAL[key] = use_toep
+            if not use_toep:
                 self._toep_M.clear()
-        return verdict
+        return use_toep
 
     __call__ = apply
 

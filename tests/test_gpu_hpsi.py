@@ -1,7 +1,7 @@
 """GPU parity of the H|psi> kernels against the CPU oracle (core/batch.py:405-505).
 
 Integer index work (pack/unpack through the plane/stick maps) is bit-exact;
-the fp64 apply matches the oracle to 1e-11 relative (the BASELINE contract is
+the fp64 apply matches the oracle to 1e-11 relative (the eigenvalue contract is
 1e-10 Ha on eigenvalues); the fp32 apply to 2e-5 relative.
 """
 
