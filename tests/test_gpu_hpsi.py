@@ -2,7 +2,7 @@
 
 Integer index work (pack/unpack through the plane/stick maps) is bit-exact;
 the fp64 apply matches the oracle to 1e-11 relative (the eigenvalue contract is
-1e-10 Ha on eigenvalues); the fp32 apply to 2e-5 relative.
+1e-10 Ha); the fp32 apply to 2e-5 relative.
 """
 
 import pytest
