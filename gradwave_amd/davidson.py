@@ -357,21 +357,60 @@ def davidson_batched_ms(h_apply, x0, t, mask, tol=1e-9, max_iter=40, max_dim_fac
 
 
 @torch.no_grad()
+def _refine_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, sweeps: int):
+    """``sweeps`` FP64 Davidson sweeps from drafted vectors, with no convergence
+    gate: Rayleigh-Ritz of the orthonormalised block with the FP64 operator, then
+    per sweep every band's Teter-preconditioned residual (davidson.py:729-743)
+    joins the basis and Rayleigh-Ritz runs over the grown basis. The returned
+    residual norms are FP64 norms of the FP64 operator; the last step applies H
+    to nothing it does not use."""
+    nk, nb, m = x0.shape
+    counts = {"low": 0, "full": 0}
+    basis = _Basis(nk, nb * (sweeps + 1), m, x0.dtype, x0.device)
+    v = yield from _orthonormalize_steps(x0.contiguous(), mask)
+    hv = yield from _apply_steps(v, False, counts)
+    basis.set(v, hv)
+    sel = torch.arange(nb, device=x0.device).expand(nk, nb).contiguous()
+    for sweep in range(sweeps + 1):
+        v, hv = basis.views()
+        w, u = yield ("eigh", basis.subspace())
+        u = u[:, :, :nb].to(v.dtype).contiguous()
+        eig = w[:, :nb].real.to(x0.real.dtype)
+        x, hx = LA.combine(u, v), LA.combine(u, hv)
+        r, rn = LA.residual(x, hx, eig)
+        if sweep == sweeps:
+            break
+        d = LA.teter_expand(x, r, t, sel)
+        d = yield from _orthonormalize_steps(d, mask, against=v)
+        hd = yield from _apply_steps(d, False, counts)
+        basis.append(d, hd)
+    return BatchedDavidsonResult(eig, x, sweeps, rn.to(x0.real.dtype), 0, counts["full"])
+
+
+@torch.no_grad()
 def davidson_mixed(h_apply, x0, t, mask, tol: float = 1e-9, draft_tol: float = 1e-4,
-                   max_iter: int = 100, max_dim_factor: int = 4,
+                   max_iter: int = 100, max_dim_factor: int = 4, sweeps: int | None = None,
                    profile_out: dict | None = None) -> BatchedDavidsonResult:
     """Precision form: the draft solve runs on complex64 blocks -- FP32 FFTs, V(r)
     multiply and MFMA contractions with FP32 accumulation -- to ``draft_tol``,
     keeping the subspace matrix, its eigensolve, the Cholesky factors and every
-    residual norm in FP64; the polish solve then runs in complex128 to ``tol``
-    from the drafted Ritz vectors, so convergence is always certified by FP64
-    residuals of an FP64 operator. The reference's draft/polish scaffold is
-    solvers/_ms.py:26-52 (davidson_batched_ms gates it on crossover < tol)."""
+    residual norm in FP64. With ``sweeps=None`` the polish solve then runs in
+    complex128 to ``tol`` from the drafted Ritz vectors, so convergence is
+    certified by FP64 residuals of an FP64 operator; with ``sweeps=n`` the draft
+    is refined by n FP64 Davidson sweeps instead (no gate; the result carries
+    their FP64 residual norms, and ``tol`` is unused). The reference's
+    draft/polish scaffold is solvers/_ms.py:26-52 (davidson_batched_ms gates it
+    on crossover < tol)."""
     hi = x0.dtype
     d = davidson_batched(h_apply, x0.to(LOW), t.to(torch.float32), mask, tol=draft_tol,
                          max_iter=max_iter, max_dim_factor=max_dim_factor, profile_out=profile_out)
-    p = davidson_batched(h_apply, d.eigenvectors.to(hi), t, mask, tol=tol, max_iter=max_iter,
-                         max_dim_factor=max_dim_factor, profile_out=profile_out)
+    if sweeps is not None:
+        if sweeps < 0:
+            raise ValueError(f"sweeps must be >= 0, got {sweeps}")
+        p = drive(_refine_steps(d.eigenvectors.to(hi), t, mask, int(sweeps)), h_apply)
+    else:
+        p = davidson_batched(h_apply, d.eigenvectors.to(hi), t, mask, tol=tol, max_iter=max_iter,
+                             max_dim_factor=max_dim_factor, profile_out=profile_out)
     p.n_apply_low += d.n_apply_full
     p.n_draft_iter = d.n_iter
     return p

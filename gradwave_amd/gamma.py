@@ -398,20 +398,59 @@ def davidson_gamma(ham: GammaHamiltonian, x0: torch.Tensor, tol: float = 1e-9, m
     return GammaDavidsonResult(eig, unembed_real(gb, x), it, rn)
 
 
+def refine_real(h_apply, x0: torch.Tensor, t_g: torch.Tensor, sweeps: int = 1):
+    """``sweeps`` Davidson sweeps of a real symmetric operator from drafted
+    vectors, with no convergence gate: Rayleigh-Ritz of the orthonormalised
+    block, then per sweep every band's Teter-preconditioned residual
+    (solvers/precond.py:20-39) joins the basis and Rayleigh-Ritz runs over the
+    grown basis. Returns (eig, x, sweeps, residual norms) in x0's precision
+    with FP64 subspace matrices, eigensolves and norms."""
+    nb = x0.shape[0]
+    dt = x0.dtype
+    f64 = torch.float64
+    t_g = t_g.to(x0.device, dt)
+    v = _orthonormalize(x0.clone())
+    if v.shape[0] < nb:
+        raise ValueError("drafted vectors are linearly dependent")
+    hv = h_apply(v)
+    for sweep in range(sweeps + 1):
+        s = LA.rinner(v, hv)
+        w, u = torch.linalg.eigh(0.5 * (s + s.T))
+        eig = w[:nb]
+        ub = u[:, :nb].to(dt)
+        x, hx = LA.rcombine(ub, v), LA.rcombine(ub, hv)
+        r = hx - eig[:, None].to(dt) * x
+        rn = torch.linalg.norm(r.to(f64), dim=1)
+        if sweep == sweeps:
+            break
+        tb = (x * x * t_g).sum(1, dtype=f64)
+        d = _orthonormalize(_teter(r, t_g.to(f64), tb), against=v)
+        v = torch.cat([v, d])
+        hv = torch.cat([hv, h_apply(d)])
+    return eig, x, sweeps, rn
+
+
 def davidson_gamma_mixed(ham: GammaHamiltonian, x0: torch.Tensor, tol: float = 1e-9, draft_tol: float = 1e-4,
-                         max_iter: int = 100, max_dim_factor: int = 4,
+                         max_iter: int = 100, max_dim_factor: int = 4, sweeps: int | None = None,
                          profile_out: dict | None = None) -> GammaDavidsonResult:
     """Precision form at Gamma: the draft solve keeps its basis in FP32 and
     applies H in FP32 (FFTs, V multiply, MFMA projector and subspace GEMMs with
     FP32 accumulation and FP64 split-K sums) to ``draft_tol``, with the subspace
-    eigensolve, Cholesky factors and residual norms in FP64; the polish solve
-    restarts from the draft Ritz vectors in FP64 and converges on FP64
-    residuals of the FP64 operator to ``tol``."""
+    eigensolve, Cholesky factors and residual norms in FP64. With
+    ``sweeps=None`` the polish solve restarts from the draft Ritz vectors in
+    FP64 and converges on FP64 residuals of the FP64 operator to ``tol``; with
+    ``sweeps=n`` n FP64 refinement sweeps (``refine_real``) replace it and
+    ``tol`` is unused."""
     gb = ham.gb
     t_phi = kinetic_phi(gb)
     x32 = embed_real(gb, x0.to(torch.complex128)).to(torch.float32)
     _, xd, itd, _ = davidson_real(ham.apply_real, x32, t_phi, tol=draft_tol, max_iter=max_iter,
                                   max_dim_factor=max_dim_factor, profile_out=profile_out)
-    eig, x, it, rn = davidson_real(ham.apply_real, xd.to(torch.float64), t_phi, tol=tol, max_iter=max_iter,
-                                   max_dim_factor=max_dim_factor, profile_out=profile_out)
+    if sweeps is not None:
+        if sweeps < 0:
+            raise ValueError(f"sweeps must be >= 0, got {sweeps}")
+        eig, x, it, rn = refine_real(ham.apply_real, xd.to(torch.float64), t_phi, int(sweeps))
+    else:
+        eig, x, it, rn = davidson_real(ham.apply_real, xd.to(torch.float64), t_phi, tol=tol, max_iter=max_iter,
+                                       max_dim_factor=max_dim_factor, profile_out=profile_out)
     return GammaDavidsonResult(eig, unembed_real(gb, x), it, rn, n_draft_iter=itd)

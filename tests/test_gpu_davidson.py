@@ -98,6 +98,41 @@ def test_si8_mixed_precision_two_stage():
     assert r_skip.n_apply_low == 0 and float(r_skip.residual_norms.max()) < TOL
 
 
+def test_si8_mixed_refinement_sweeps():
+    """Draft + n FP64 refinement sweeps: Ritz values are upper bounds of the
+    converged FP64 eigenvalues and do not rise from one sweep to the next (the
+    basis only grows); one sweep lands within 1e-6 Ha; the returned residual
+    norms are those of the FP64 operator."""
+    from gradwave_amd.davidson import LOW, _refine_steps, drive
+
+    say("davidson si8 draft + fp64 refinement sweeps")
+    s = system("si8")
+    cpu, gpu, p = select_k(s, [0, 7, 20])
+    h = BatchedHamiltonian(gpu, s.shape, s.v_r.to(dev()), p.to(dev()))
+    x0 = rand_block(3, 24, cpu.npw_max, cpu.mask, seed=1).to(dev())
+    r64 = davidson_batched(h.apply, x0, gpu.t, gpu.mask, tol=TOL, max_iter=200)
+    draft = davidson_batched(h.apply, x0.to(LOW), gpu.t.to(torch.float32), gpu.mask, tol=1e-3, max_iter=200)
+    xd = draft.eigenvectors.to(torch.complex128)
+    res = [drive(_refine_steps(xd, gpu.t, gpu.mask, n), h.apply) for n in (0, 1, 2)]
+    eps = 1e-12 * HARTREE_EV
+    for n, r in enumerate(res):
+        assert r.n_iter == n
+        assert float((r.eigenvalues - r64.eigenvalues).min()) > -eps
+        hx = h.apply(r.eigenvectors)
+        rn = torch.linalg.norm(hx - r.eigenvalues[..., None] * r.eigenvectors, dim=-1)
+        assert float((rn - r.residual_norms).abs().max()) < 1e-10
+        gram = LA.inner(r.eigenvectors, r.eigenvectors).cpu()
+        assert float((gram - torch.eye(24, dtype=gram.dtype)).abs().max()) < 1e-10
+    assert float((res[1].eigenvalues - res[0].eigenvalues).max()) < eps
+    assert float((res[2].eigenvalues - res[1].eigenvalues).max()) < eps
+    assert float((res[1].eigenvalues - r64.eigenvalues).abs().max()) < 1e-6 * HARTREE_EV
+    rms = davidson_mixed(h.apply, x0, gpu.t, gpu.mask, draft_tol=1e-3, max_iter=200, sweeps=1)
+    assert rms.n_iter == 1 and rms.n_draft_iter == draft.n_iter and rms.n_apply_low > 0
+    assert float((rms.eigenvalues - r64.eigenvalues).abs().max()) < 1e-6 * HARTREE_EV
+    with pytest.raises(ValueError):
+        davidson_mixed(h.apply, x0, gpu.t, gpu.mask, draft_tol=1e-3, max_iter=200, sweeps=-1)
+
+
 def test_sternheimer_cg_matches_dense_solve():
     """cg_sternheimer (postscf/_response.py:502-681) on Si2, two k-points, four
     occupied bands: against a dense solve of (H - eps_n + s P_occ) x = rhs with

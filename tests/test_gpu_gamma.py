@@ -92,6 +92,19 @@ def test_gamma_davidson_matches_dense_and_complex():
     mixed = G.davidson_gamma_mixed(ours, x0.to(dev()), tol=tol, draft_tol=1e-3, max_iter=300)
     assert float(mixed.residual_norms.max()) <= tol
     assert float((mixed.eigenvalues.cpu() - dense).abs().max()) / HARTREE_EV < 1e-10
+    # draft + FP64 refinement sweeps: upper bounds, non-increasing with sweeps, 1e-6 Ha after one
+    t_phi = G.kinetic_phi(gb)
+    x32 = G.embed_real(gb, x0.to(dev())).to(torch.float32)
+    _, xd, _, _ = G.davidson_real(ours.apply_real, x32, t_phi, tol=1e-3, max_iter=300)
+    sw = [G.refine_real(ours.apply_real, xd.to(torch.float64), t_phi, n) for n in (0, 1)]
+    for e, x, it, rn in sw:
+        assert float((e.cpu() - dense).min()) > -1e-12 * HARTREE_EV
+        hx = ours.apply_real(x)
+        assert float((torch.linalg.norm(hx - e[:, None] * x, dim=1) - rn).abs().max()) < 1e-10
+    assert float((sw[1][0] - sw[0][0]).max()) < 1e-12 * HARTREE_EV
+    one = G.davidson_gamma_mixed(ours, x0.to(dev()), draft_tol=1e-3, max_iter=300, sweeps=1)
+    assert one.n_iter == 1
+    assert float((one.eigenvalues.cpu() - dense).abs().max()) / HARTREE_EV < 1e-6
     # the complex batched solver on the same frozen potential
     d = dev()
     bk = SimpleNamespace(npw=s.bk.npw.to(d), mask=s.bk.mask.to(d), flat_idx=s.bk.flat_idx.to(d), t=s.bk.t.to(d),
