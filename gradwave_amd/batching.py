@@ -33,7 +33,7 @@ import torch
 
 from gradwave_amd import _lib as L
 from gradwave_amd import workloads as W
-from gradwave_amd.davidson import BatchedDavidsonResult, davidson_steps
+from gradwave_amd.davidson import BatchedDavidsonResult, _refine_steps, davidson_steps
 from gradwave_amd.hamiltonian import BatchedHamiltonian
 
 
@@ -226,6 +226,15 @@ def davidson_ragged(batch: RaggedBatch, x0s: list[torch.Tensor], ts: list[torch.
     ``check_finite`` raises on the first non-finite result (debugging aid)."""
     gens = [davidson_steps(x0, t, m, tol=tol, max_iter=max_iter, max_dim_factor=max_dim_factor)
             for x0, t, m in zip(x0s, ts, masks)]
+    return _lockstep(batch, gens, stats, check_finite)
+
+
+def _lockstep(batch: RaggedBatch, gens: list, stats: RaggedSolveStats | None,
+              check_finite: bool) -> list:
+    """Advance step generators (``davidson_steps``, ``_refine_steps``) together:
+    each round serves every pending apply with one ragged apply per precision,
+    every pending eigensolve with one batched call and every host read with one
+    copy."""
     results: list = [None] * len(gens)
     pending: dict[int, tuple] = {}
 
@@ -292,17 +301,25 @@ def davidson_ragged(batch: RaggedBatch, x0s: list[torch.Tensor], ts: list[torch.
 
 
 def davidson_ragged_mixed(batch: RaggedBatch, x0s, ts, masks, tol: float = 1e-9, draft_tol: float = 1e-4,
-                          max_iter: int = 100, max_dim_factor: int = 4,
+                          max_iter: int = 100, max_dim_factor: int = 4, sweeps: int | None = None,
                           stats: RaggedSolveStats | None = None) -> list[BatchedDavidsonResult]:
     """Precision form of the batch: all instances draft in complex64 (FP32 FFTs,
     V multiply and MFMA GEMMs) to ``draft_tol`` in lockstep, then polish in
     complex128 to ``tol`` from the drafted Ritz vectors, FP64 subspace algebra
-    and residual norms throughout (the scaffold of solvers/_ms.py:26-52)."""
+    and residual norms throughout (the scaffold of solvers/_ms.py:26-52). With
+    ``sweeps=n`` the polish is n FP64 refinement sweeps per instance in lockstep
+    (no gate, ``tol`` unused), as in ``davidson_mixed``."""
     hi = [x.dtype for x in x0s]
     d = davidson_ragged(batch, [x.to(torch.complex64) for x in x0s], [t.to(torch.float32) for t in ts], masks,
                         tol=draft_tol, max_iter=max_iter, max_dim_factor=max_dim_factor, stats=stats)
-    p = davidson_ragged(batch, [r.eigenvectors.to(h) for r, h in zip(d, hi)], ts, masks, tol=tol,
-                        max_iter=max_iter, max_dim_factor=max_dim_factor, stats=stats)
+    if sweeps is not None:
+        if sweeps < 0:
+            raise ValueError(f"sweeps must be >= 0, got {sweeps}")
+        gens = [_refine_steps(r.eigenvectors.to(h), t, m, int(sweeps)) for r, h, t, m in zip(d, hi, ts, masks)]
+        p = _lockstep(batch, gens, stats, False)
+    else:
+        p = davidson_ragged(batch, [r.eigenvectors.to(h) for r, h in zip(d, hi)], ts, masks, tol=tol,
+                            max_iter=max_iter, max_dim_factor=max_dim_factor, stats=stats)
     for pi, di in zip(p, d):
         pi.n_apply_low += di.n_apply_full
         pi.n_draft_iter = di.n_iter

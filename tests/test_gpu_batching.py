@@ -117,6 +117,16 @@ def test_lockstep_davidson_matches_independent_solves():
     for r, m in zip(res, mixed):
         assert float(m.residual_norms.max()) < tol
         assert float((m.eigenvalues - r.eigenvalues).abs().max()) / HARTREE_EV < 1e-10
+    # sweep form in lockstep: each instance as its own draft + one FP64 sweep would give
+    swept = B.davidson_ragged_mixed(batch, x0s, [w.bk.t for w in wls], [w.bk.mask for w in wls],
+                                    draft_tol=1e-3, max_iter=200, sweeps=1)
+    for i, (w, h, x0, r, sw) in enumerate(zip(wls, hams, x0s, res, swept)):
+        assert sw.n_iter == 1
+        assert float((sw.eigenvalues - r.eigenvalues).min()) > -1e-12 * HARTREE_EV, (i, w.name)
+        assert float((sw.eigenvalues - r.eigenvalues).abs().max()) / HARTREE_EV < 1e-6, (i, w.name)
+        hx = h.apply(sw.eigenvectors)
+        rn = torch.linalg.norm(hx - sw.eigenvalues[..., None] * sw.eigenvectors, dim=-1)
+        assert float((rn - sw.residual_norms).abs().max()) < 1e-10, (i, w.name)
 
 
 def test_autotuner_choices_are_valid_and_exact(tmp_path, monkeypatch):
