@@ -24,6 +24,7 @@
 #include <cstdarg>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 #include <numeric>
 #include <vector>
 
@@ -188,6 +189,14 @@ template <typename T> __device__ __forceinline__ cpx<T> buf_load(__amdgpu_buffer
     return __builtin_bit_cast(cpx<T>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
   else
     return __builtin_bit_cast(cpx<T>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+// two adjacent FP32 complex values, moved as one 16-byte access
+struct alignas(16) cpx2f { float2 a, b; };
+__device__ __forceinline__ cpx2f buf_load2(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(cpx2f, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_store2(cpx2f v, __amdgpu_buffer_rsrc_t r, int off) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, off, 0, 0);
 }
 template <typename T> __device__ __forceinline__ void buf_store(cpx<T> v, __amdgpu_buffer_rsrc_t r, int off) {
   if constexpr (sizeof(cpx<T>) == 8)
@@ -391,7 +400,13 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
   constexpr int LW = W::LW;
   constexpr int LC = NWX * LW;                  // lines per workgroup
   constexpr int NT = 64 * NWX;
-  constexpr int KS = (N * LC + NT - 1) / NT;    // staged elements per thread (na <= N rows)
+  // FP32 tiles move in pairs of adjacent lines (16-byte accesses): every tile row
+  // starts at an even line (LC, j0 and n2*n3 are even), so a pair never straddles
+  // a row or the last line
+  constexpr int PK = (sizeof(T) == 4 && LC % 2 == 0 && N % 2 == 0) ? 2 : 1;
+  using SV = typename std::conditional<PK == 2, cpx2f, cpx<T>>::type;
+  constexpr int LCU = LC / PK;                  // staging units per tile row
+  constexpr int KS = (N * LCU + NT - 1) / NT;   // staged units per thread (na <= N rows)
   constexpr int LDT = LC + kXTilePad;           // tile row stride
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* E = reinterpret_cast<cpx<T>*>(smem_raw);
@@ -413,18 +428,26 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
     tiles[tstride + na * LDT + i] = czero<T>();
   }
   const int j0 = chunk * LC;
-  // staging map: thread element k is tile entry e = threadIdx.x + k*NT = a*LC + jj
-  // <-> box[band][a][j0 + jj]; entries past the tile or the last line are masked
-  const int ne = na * LC;
-  const int ks = (ne + NT - 1) / NT;  // staged elements per thread actually needed (uniform)
+  // staging map: thread unit k is tile unit e = threadIdx.x + k*NT = a*LCU + jj/PK
+  // <-> box[band][a][j0 + jj .. + PK-1]; units past the tile or the last line are masked
+  const int ne = na * LCU;
+  const int ks = (ne + NT - 1) / NT;  // staged units per thread actually needed (uniform)
   int soff[KS], tix[KS];
 #pragma unroll
   for (int k = 0; k < KS; ++k) {
     const int e = threadIdx.x + k * NT;
-    const int a = e / LC, jj = e - a * LC;
+    const int a = e / LCU, jj = (e - a * LCU) * PK;
     soff[k] = (e < ne && j0 + jj < n23) ? (a * n23 + j0 + jj) * (int)sizeof(cpx<T>) : kBufOff;
     tix[k] = e < ne ? a * LDT + jj : 0;
   }
+  auto sload = [](__amdgpu_buffer_rsrc_t r, int off) -> SV {
+    if constexpr (PK == 2) return buf_load2(r, off);
+    else return buf_load<T>(r, off);
+  };
+  auto sstore = [](SV v, __amdgpu_buffer_rsrc_t r, int off) {
+    if constexpr (PK == 2) buf_store2(v, r, off);
+    else buf_store<T>(v, r, off);
+  };
   const unsigned band_bytes = (unsigned)((long)na * n23 * sizeof(cpx<T>));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = lane / LW, l = lane - t * LW;
@@ -445,16 +468,16 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
   T vr[S];
 #pragma unroll
   for (int s = 0; s < S; ++s) vr[s] = (t < M && jok) ? vx[(long)W::rpos(t, s) * n23 + j0 + col] : T(0);
-  cpx<T> st[KS];
+  SV st[KS];
   {
     const auto r = buf_rsrc(box + (long)b0 * na * n23, band_bytes);
 #pragma unroll
     for (int k = 0; k < KS; ++k)
-      if (k < ks) st[k] = buf_load<T>(r, soff[k]);
+      if (k < ks) st[k] = sload(r, soff[k]);
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       const int e = threadIdx.x + k * NT;
-      if (k < ks && e < ne) tiles[tix[k]] = st[k];
+      if (k < ks && e < ne) *reinterpret_cast<SV*>(tiles + tix[k]) = st[k];
     }
   }
   // V(r) and the first tile have landed on every path: no load is pending at the
@@ -471,7 +494,7 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
       const auto r = buf_rsrc(box + (long)(more ? bi + 1 : bi) * na * n23, more ? band_bytes : 0u);
 #pragma unroll
       for (int k = 0; k < KS; ++k)
-        if (k < ks) st[k] = buf_load<T>(r, soff[k]);
+        if (k < ks) st[k] = sload(r, soff[k]);
     }
 #pragma unroll
     for (int q = 0; q < M; ++q) x[q] = cur[rd[q]];
@@ -486,12 +509,12 @@ k_xpass_stage(Geom g, int nbt, int bgroup, cpx<T>* __restrict__ box, const T* __
       const auto r = buf_rsrc(box + (long)bi * na * n23, band_bytes);
 #pragma unroll
       for (int k = 0; k < KS; ++k)
-        if (k < ks) buf_store<T>(cur[tix[k]], r, soff[k]);
+        if (k < ks) sstore(*reinterpret_cast<const SV*>(cur + tix[k]), r, soff[k]);
     }
 #pragma unroll
     for (int k = 0; k < KS; ++k) {
       const int e = threadIdx.x + k * NT;
-      if (k < ks && e < ne) nxt[tix[k]] = st[k];
+      if (k < ks && e < ne) *reinterpret_cast<SV*>(nxt + tix[k]) = st[k];
     }
     __syncthreads();
   }
