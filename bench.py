@@ -624,12 +624,10 @@ def cpu_baseline(args, wl) -> dict:
         h.apply(c)  # warm-up
         ts = []
         budget = time.perf_counter() + seconds
-        while len(ts) < 5 or (len(ts) < 20 and time.perf_counter() < budget):
+        while len(ts) < 5 or (len(ts) < 400 and time.perf_counter() < budget):
             a = time.perf_counter()
             h.apply(c)
             ts.append(time.perf_counter() - a)
-            if time.perf_counter() > budget and len(ts) >= 3:
-                break
         return statistics.median(ts), len(ts)
 
     med, n = median_apply(12.0)
