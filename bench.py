@@ -167,6 +167,9 @@ def main() -> None:
     roof["algorithmic_per_launch"] = work
     roof["stage_ms_fp32"] = st32
     roof["stage_ms_fp64"] = st64
+    if kind == "hbm":
+        roof["copy_GBps"] = copy_bandwidth(dev)
+        roof["frac_of_copy"] = roof["achieved"] / roof["copy_GBps"]
 
     result = {
         "metric": "Hψ band-applies/s",
@@ -411,6 +414,29 @@ def bench_ragged(args) -> None:
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)
+
+
+def copy_bandwidth(dev, nbytes: int = 2 << 30, reps: int = 10) -> float:
+    """Bytes read + written per second by a plain device-to-device copy of
+    ``nbytes`` (torch copy_), timed with HIP events: the bandwidth a streaming
+    kernel can reach on this box, beside the published 8 TB/s."""
+    import torch
+
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) * 1e-3 / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / t / 1e9
 
 
 def toeplitz_rates(args, wl, x32, x64) -> dict:
