@@ -43,6 +43,32 @@ def test_si2_all_k_fp64_matches_oracle_and_dense():
     assert float((rn - r.residual_norms.cpu()).abs().max()) < 1e-12
 
 
+def test_registry_solvers_on_si2():
+    """The SCF's call through the registry (scf/loop.py:953-955:
+    get_solver(name)(apply, coeffs, t_solve, mask, tol=, nbands=)) runs the
+    MI355X solvers: "davidson" gives davidson_batched's result exactly; the
+    two-stage "davidson_ms" skips its draft when crossover >= tol (_ms.py:47) and
+    otherwise drafts in complex64 (its applies counted as low precision)."""
+    from gradwave_amd import registry as R
+
+    say("registry davidson / davidson_ms si2")
+    s = system("si2")
+    cpu, gpu, p = select_k(s, [0, 5])
+    h = BatchedHamiltonian(gpu, s.shape, s.v_r.to(dev()), p.to(dev()))
+    x0 = rand_block(2, 8, cpu.npw_max, cpu.mask, seed=4).to(dev())
+    want = davidson_batched(h.apply, x0, gpu.t, gpu.mask, tol=TOL, max_iter=100)
+    got = R.get("davidson")(h.apply, x0, gpu.t, gpu.mask, tol=TOL, nbands=8, max_iter=100)
+    assert torch.equal(got.eigenvalues, want.eigenvalues) and got.n_iter == want.n_iter
+    assert torch.equal(got.residual_norms, want.residual_norms)
+    assert got.diagnostics["solver"] == "davidson" and not got.diagnostics["hit_max_iter"]
+    ms = R.get("davidson_ms")(h.apply, x0, gpu.t, gpu.mask, tol=TOL, nbands=8, max_iter=100)
+    assert ms.diagnostics["n_apply_low"] == 0 and torch.equal(ms.eigenvalues, want.eigenvalues)
+    loose = 1e-3  # eV
+    ms = R.get("davidson_ms")(h.apply, x0, gpu.t, gpu.mask, tol=loose, nbands=8, max_iter=100, crossover=1e-4)
+    assert ms.diagnostics["n_apply_low"] > 0 and float(ms.residual_norms.max()) < loose
+    assert float((ms.eigenvalues - want.eigenvalues).abs().max()) < 1e-6 * HARTREE_EV
+
+
 def _synthetic(nk, npw, seed):
     """The reference's synthetic operator (test_davidson_fp32_expansion.py:24-42)."""
     torch.manual_seed(seed)
