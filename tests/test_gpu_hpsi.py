@@ -131,17 +131,25 @@ def test_apply_si64_gamma_90cubed(lds_x):
 
 
 def test_apply_cols_matches_apply():
-    """apply_cols (core/batch.py:507-547) on a compacted column batch == apply rows."""
+    """apply_cols (core/batch.py:507-547) on a compacted column batch == apply rows;
+    both bump the band-k apply tally like the reference (batch.py:111-123, 485-486, 520-521)."""
+    from gradwave_amd.hamiltonian import happly_tally, reset_happly_tally
+
     say("apply_cols si2")
     s = system("si2")
     cpu, h = _gpu_h(s)
     nk, m = cpu.mask.shape
     c = rand_block(nk, 4, m, cpu.mask, seed=6).to(dev())
+    reset_happly_tally()
     full = h.apply(c)
+    assert happly_tally() == nk * 4
     kcol = torch.tensor([3, 0, 3, 7, 9, 0], device=dev())
     bcol = torch.tensor([1, 2, 0, 3, 3, 0], device=dev())
     cols = h.apply_cols(c[kcol, bcol], kcol)
+    assert happly_tally() == nk * 4 + 6
     assert float((cols - full[kcol, bcol]).abs().max()) < 1e-12 * float(full.abs().max())
+    assert h.apply_cols(c[kcol[:0], bcol[:0]], kcol[:0]).shape == (0, m)
+    assert h.apply(c[:, :0]).shape == (nk, 0, m)  # an empty band block is returned as is
 
 
 def test_random_potential_and_hermiticity():
