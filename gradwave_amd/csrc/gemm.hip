@@ -616,16 +616,28 @@ static int inner_splits(int nk, int M, int N, int K) {
   return (int)std::min<long>(s, 4096);
 }
 
+static int g_inner_lib[2] = {0, 0};
+
+// K-chunks of the rocBLAS inner product: enough that the strided batch of
+// 128x128-tile GEMMs reaches about 2048 tiles, at most 128 (every chunk adds an
+// M x N partial to the reduce) and at least 64 K-values long
+static int lib_splits(int M, int N, int K) {
+  const long tiles = (long)((M + 127) / 128) * ((N + 127) / 128);
+  long s = std::min<long>((2048 + tiles - 1) / tiles, 128);
+  s = std::min<long>(s, std::max<long>(1, K / 64));
+  return (int)std::max<long>(1, s);
+}
+
 template <typename T>
 size_t gemm_inner_partials_bytes(int nk, int M, int N, int K) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
-  const int s = inner_splits(nk, M, N, K);
+  int s = inner_splits(nk, M, N, K);
+  if (g_inner_lib[sizeof(T) == 4]) s = std::max(s, lib_splits(M, N, K));
   return (size_t)nk * s * M * 2 * N * sizeof(T);
 }
 template size_t gemm_inner_partials_bytes<double>(int, int, int, int);
 template size_t gemm_inner_partials_bytes<float>(int, int, int, int);
 
-static int g_inner_lib[2] = {0, 0};
 void set_inner_lib(int precision, int v) { g_inner_lib[precision == GW_FP32] = v ? 1 : 0; }
 int inner_lib(int precision) { return g_inner_lib[precision == GW_FP32]; }
 static rocblas_handle blas_handle();
@@ -677,12 +689,13 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
       for (int i = 0; i < M; ++i) hipMemsetAsync(C + (long)k * sc + (long)i * ldc, 0, N * sizeof(cpx<T>), st);
     return GW_OK;
   }
-  const int splits = inner_splits(nk, M, N, K);
+  const bool use_lib = g_inner_lib[sizeof(T) == 4] && ldx < (1L << 31) && ldy < (1L << 31);
+  const int splits = use_lib ? lib_splits(M, N, K) : inner_splits(nk, M, N, K);
   const size_t need = (size_t)nk * splits * M * 2 * N * sizeof(T);
   GW_REQUIRE(partials && part_bytes >= need, GW_ERR_WORKSPACE, "gemm_inner workspace %zu < %zu", part_bytes, need);
   int kchunk = (K + splits - 1) / splits;
   kchunk = ((kchunk + kKC - 1) / kKC) * kKC;  // multiple of both K granules (16, 32)
-  if (g_inner_lib[sizeof(T) == 4] && ldx < (1L << 31) && ldy < (1L << 31)) {
+  if (use_lib) {
     // chunks of kchunk; the partial slots per k are the chunks actually used
     const int used = (K + kchunk - 1) / kchunk, nfull = K / kchunk;
     for (int k = 0; k < nk; ++k) {
