@@ -616,6 +616,25 @@ static int inner_splits(int nk, int M, int N, int K) {
   return (int)std::min<long>(s, 4096);
 }
 
+template <typename T>
+__global__ void k_zero_rows(int M, int N, cpx<T>* __restrict__ C, long ldc, long sc) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)M * N) return;
+  const int i = (int)(e / N), j = (int)(e - (long)i * N);
+  C[(long)blockIdx.y * sc + (long)i * ldc + j] = cpx<T>{T(0), T(0)};
+}
+
+// Z = beta Z (beta = 0 writes zeros, whatever Z held): a combine over an empty J
+template <typename T>
+__global__ void k_scale_rows(int A, int G, cpx<T>* __restrict__ Z, long ldz, long sz, T beta) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)A * G) return;
+  const int a = (int)(e / G), g = (int)(e - (long)a * G);
+  cpx<T>* z = Z + (long)blockIdx.y * sz + (long)a * ldz + g;
+  const cpx<T> v = *z;
+  *z = beta != T(0) ? cpx<T>{beta * v.x, beta * v.y} : cpx<T>{T(0), T(0)};
+}
+
 static int g_inner_lib[2] = {0, 0};
 
 // K-chunks of the rocBLAS inner product: enough that the strided batch of
@@ -684,9 +703,10 @@ template <typename T>
 int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, const cpx<T>* Y, long ldy,
                long sy, cpx<T>* C, long ldc, long sc, void* partials, size_t part_bytes, hipStream_t st) {
   if (M <= 0 || N <= 0 || nk <= 0) return GW_OK;
-  if (K <= 0) {
-    for (int k = 0; k < nk; ++k)
-      for (int i = 0; i < M; ++i) hipMemsetAsync(C + (long)k * sc + (long)i * ldc, 0, N * sizeof(cpx<T>), st);
+  if (K <= 0) {  // an empty contraction: C = 0, by a kernel (safe inside a captured graph)
+    hipLaunchKernelGGL(k_zero_rows<T>, dim3((unsigned)(((long)M * N + 255) / 256), nk), dim3(256), 0, st, M, N, C,
+                       ldc, sc);
+    GW_CHECK_HIP(hipGetLastError());
     return GW_OK;
   }
   const bool use_lib = g_inner_lib[sizeof(T) == 4] && ldx < (1L << 31) && ldy < (1L << 31);
@@ -872,6 +892,12 @@ template <typename T>
 int gemm_combine(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su, int conj_u, const cpx<T>* Y,
                  long ldy, long sy, cpx<T>* Z, long ldz, long sz, T beta, hipStream_t st) {
   if (A <= 0 || G <= 0 || nk <= 0) return GW_OK;
+  if (J <= 0) {
+    hipLaunchKernelGGL(k_scale_rows<T>, dim3((unsigned)(((long)A * G + 255) / 256), nk), dim3(256), 0, st, A, G, Z,
+                       ldz, sz, beta);
+    GW_CHECK_HIP(hipGetLastError());
+    return GW_OK;
+  }
   if (g_combine_lib[sizeof(T) == 4] && J > 0 && ldu < (1L << 31) && ldy < (1L << 31) && ldz < (1L << 31))
     return combine_rocblas<T>(nk, A, G, J, U, ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, st);
   const int ntx = (2 * G + kTile - 1) / kTile, nty = (A + kTile - 1) / kTile;
@@ -918,7 +944,9 @@ extern "C" size_t gw_zgemm_inner_workspace_bytes(int precision, int nk, int M, i
 extern "C" int gw_zgemm_inner(int precision, int nk, int M, int N, int K, const void* X, long ldx, long sx,
                               const void* Y, long ldy, long sy, void* C, long ldc, long sc, void* ws,
                               size_t ws_bytes, void* stream) {
-  GW_REQUIRE(X && Y && C && nk >= 0 && M >= 0 && N >= 0 && K >= 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(nk >= 0 && M >= 0 && N >= 0 && K >= 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE((X || (long)M * K == 0) && (Y || (long)N * K == 0) && (C || (long)M * N == 0), GW_ERR_ARG,
+             "null operand of non-zero size");
   GW_REQUIRE(ldx >= K && ldy >= K && ldc >= N, GW_ERR_ARG, "leading dimension too small");
   if (precision == GW_FP64)
     return gemm_inner<double>(nk, M, N, K, (const double2*)X, ldx, sx, (const double2*)Y, ldy, sy, (double2*)C,
@@ -930,7 +958,9 @@ extern "C" int gw_zgemm_inner(int precision, int nk, int M, int N, int K, const 
 extern "C" int gw_zgemm_combine(int precision, int nk, int A, int G, int J, const void* U, long ldu, long su,
                                 int conj_u, const void* Y, long ldy, long sy, void* Z, long ldz, long sz,
                                 double beta, void* stream) {
-  GW_REQUIRE(U && Y && Z && nk >= 0 && A >= 0 && G >= 0 && J >= 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(nk >= 0 && A >= 0 && G >= 0 && J >= 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE((U || (long)J * A == 0) && (Y || (long)J * G == 0) && (Z || (long)A * G == 0), GW_ERR_ARG,
+             "null operand of non-zero size");
   GW_REQUIRE(ldu >= A && ldy >= G && ldz >= G, GW_ERR_ARG, "leading dimension too small");
   if (precision == GW_FP64)
     return gemm_combine<double>(nk, A, G, J, (const double2*)U, ldu, su, conj_u, (const double2*)Y, ldy, sy,

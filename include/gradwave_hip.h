@@ -109,7 +109,8 @@ int gw_box_to_sphere(gw_plan* plan, int precision, const void* box, void* c_out,
  *          U' = U or conj(U); U (nk, J, A); Y (nk, J, G); Z (nk, A, G).
  *          e.g. Ritz vectors x = u^T v (davidson.py:653), back-projection (batch.py:500).
  * Strides are in complex elements. The inner product needs scratch for its
- * split-K partial sums: gw_zgemm_inner_workspace_bytes. */
+ * split-K partial sums: gw_zgemm_inner_workspace_bytes. An operand of zero size
+ * may be null; K = 0 writes C = 0 and J = 0 leaves Z = beta*Z (zero for beta = 0). */
 size_t gw_zgemm_inner_workspace_bytes(int precision, int nk, int M, int N, int K);
 int gw_zgemm_inner(int precision, int nk, int M, int N, int K, const void* X, long ldx, long sx,
                    const void* Y, long ldy, long sy, void* C, long ldc, long sc,

@@ -204,3 +204,21 @@ def test_inner_on_rocblas_matches(dtype):
         assert float((got - want).abs().max()) < TOL[dtype] * scale
     finally:
         lib.gw_tune_set(L.GW_TUNE_INNER_LIB, prec, 0)
+
+
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_empty_contraction_is_zero(dtype):
+    """K = 0: the inner product is an M x N zero matrix (written by a kernel, so
+    an output full of garbage must come back zeroed); the combine with J = 0
+    leaves beta * Z."""
+    say(f"gemm empty contraction {dtype}")
+    x = torch.zeros(2, 3, 0, dtype=dtype, device=dev())
+    y = torch.zeros(2, 4, 0, dtype=dtype, device=dev())
+    out = torch.full((2, 3, 4), 7.0, dtype=dtype, device=dev())
+    c = LA.inner(x, y, out=out)
+    assert c.shape == (2, 3, 4) and float(c.abs().max().cpu()) == 0.0
+    z = torch.full((2, 3, 5), 2.0, dtype=dtype, device=dev())
+    u = torch.zeros(2, 0, 3, dtype=dtype, device=dev())
+    v = torch.zeros(2, 0, 5, dtype=dtype, device=dev())
+    r = LA.combine(u, v, out=z, beta=0.5)
+    assert float((r - 1.0).abs().max().cpu()) == 0.0
