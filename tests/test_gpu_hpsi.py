@@ -242,6 +242,33 @@ def test_si216_gamma_workload_against_oracle():
     assert _rel(got32, want) < 2e-5
 
 
+@pytest.mark.parametrize("name", ["si64_gamma", "si216_gamma"])
+def test_full_blocks_size_independent_properties(name):
+    """At the benchmark sizes (Si64: 160 bands x 37,073 PW, 90^3; Si216: 480 x
+    80,797, 120^3, 864 projectors), where the oracle would take minutes, the
+    apply is checked through properties fixed by the mathematics: H Hermitian
+    (<y_i, H x_j> = <H y_i, x_j> for all band pairs), H linear, every band
+    independent of the others (a sub-block at an odd offset, launched with other
+    band groups and split-K widths, gives the same rows), and FP32 within 2e-5."""
+    from gradwave_amd import linalg as LA
+    from gradwave_amd import workloads as W
+
+    say(f"full-size properties {name}")
+    wl = getattr(W, name)(device=dev())
+    h = BatchedHamiltonian(wl.bk, wl.shape, wl.v_r, wl.p)
+    x = W.start_vectors(wl, seed=0)
+    y = W.start_vectors(wl, seed=1)
+    hx, hy = h.apply(x), h.apply(y)
+    a, b = LA.inner(y, hx), LA.inner(hy, x)
+    assert float((a - b).abs().max() / a.abs().max()) < 1e-12
+    z = h.apply((x + 2j * y).contiguous())
+    assert float((z - (hx + 2j * hy)).abs().max() / z.abs().max()) < 1e-13
+    sub = h.apply(x[:, 37:101].contiguous())
+    assert float((sub - hx[:, 37:101]).abs().max() / hx.abs().max()) < 1e-13
+    h32 = h.apply(x.to(torch.complex64)).to(torch.complex128)
+    assert float((h32 - hx).abs().max() / hx.abs().max()) < 2e-5
+
+
 @pytest.mark.parametrize("name,dtype", [("si2", torch.complex128), ("si2", torch.complex64),
                                         ("si8", torch.complex128)])
 def test_toeplitz_local_matches_oracle_and_fft(name, dtype):
