@@ -2088,7 +2088,13 @@ struct gw_batch {
     unsigned char* ws;
   };
   std::vector<Bound> bound;
+  // the instances' nonlocal chains (small, latency-bound GEMMs) are spread over
+  // the launch stream and kBatchLanes - 1 side streams of default priority,
+  // forked from and joined back into the launch stream by events
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
 };
+constexpr int kBatchLanes = 4;  // the launch stream + 3 side streams (4 hardware queues per process)
 
 static size_t batch_ws_bytes(const gw_batch* b, const int* nb, int precision) {
   size_t tot = 0;
@@ -2112,6 +2118,14 @@ extern "C" int gw_batch_create(gw_batch** out, gw_plan* const* plans, int n) {
     delete b;
     GW_REQUIRE(false, GW_ERR_HIP, "hipMalloc of the batch table failed");
   }
+  bool ok = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+  for (int j = 0; j < kBatchLanes - 1 && ok; ++j)
+    ok = hipStreamCreateWithFlags(&b->side[j], hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&b->ev_join[j], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    gw_batch_destroy(b);
+    GW_REQUIRE(false, GW_ERR_HIP, "creating the batch's side streams failed");
+  }
   const int want = kMaxLds;
   (void)hipFuncSetAttribute((const void*)k_plane_inv_rag<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_plane_inv_rag<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
@@ -2131,6 +2145,12 @@ extern "C" int gw_batch_create(gw_batch** out, gw_plan* const* plans, int n) {
 
 extern "C" int gw_batch_destroy(gw_batch* b) {
   if (!b) return GW_OK;
+  for (int j = 0; j < kBatchLanes - 1; ++j) {
+    if (b->side[j]) (void)hipStreamSynchronize(b->side[j]);
+    if (b->side[j]) (void)hipStreamDestroy(b->side[j]);
+    if (b->ev_join[j]) (void)hipEventDestroy(b->ev_join[j]);
+  }
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   if (b->table) (void)hipFree(b->table);
   delete b;
   return GW_OK;
@@ -2225,24 +2245,41 @@ static int batch_apply_t(gw_batch* b, hipStream_t st) {
   GW_CHECK_HIP(hipGetLastError());
   if (!(b->terms & GW_TERM_NONLOCAL)) return GW_OK;
   const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
+  GW_CHECK_HIP(hipEventRecord(b->ev_fork, st));
+  for (int j = 0; j < kBatchLanes - 1; ++j) GW_CHECK_HIP(hipStreamWaitEvent(b->side[j], b->ev_fork, 0));
+  auto join = [&]() {  // the launch stream waits for every side stream's chains
+    for (int j = 0; j < kBatchLanes - 1; ++j) {
+      (void)hipEventRecord(b->ev_join[j], b->side[j]);
+      (void)hipStreamWaitEvent(st, b->ev_join[j], 0);
+    }
+  };
+  int lane = 0;
   for (int i = 0; i < n; ++i) {
     Plan& P = b->plans[i]->p;
     const gw_batch::Bound& B = b->bound[i];
     if (!P.nproj || B.nb == 0) continue;
+    // instance i's chain on one lane; the chains write disjoint workspaces and outputs
+    const hipStream_t ls = lane == 0 ? st : b->side[lane - 1];
+    lane = (lane + 1) % kBatchLanes;
     const ApplyLayout L = apply_layout(&P, B.nb, prec);
     const cpx<T>* pr = prec == GW_FP64 ? (const cpx<T>*)P.p64 : (const cpx<T>*)P.p32;
     cpx<T>* becp = (cpx<T>*)(B.ws + L.becp_off);
     cpx<T>* u = (cpx<T>*)(B.ws + L.u_off);
     const long pstride = (long)P.nproj * P.npw_max;
     int rc = gemm_inner<T>(P.nk, P.nproj, B.nb, P.npw_max, pr, P.npw_max, pstride, (const cpx<T>*)B.c, B.ldc, B.sc,
-                           becp, B.nb, (long)P.nproj * B.nb, B.ws + L.part_off, L.part_bytes, st);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_dmul<T>, dim3(grid_blocks((long)P.nproj * B.nb, 256), P.nk), dim3(256), 0, st, P.nproj,
-                       B.nb, P.d_colptr, P.d_row, P.d_val, becp, u);
-    rc = gemm_combine<T>(P.nk, B.nb, P.npw_max, P.nproj, u, B.nb, (long)P.nproj * B.nb, 0, pr, P.npw_max, pstride,
-                         (cpx<T>*)B.out, B.ldo, B.so, T(1), st);
-    if (rc) return rc;
+                           becp, B.nb, (long)P.nproj * B.nb, B.ws + L.part_off, L.part_bytes, ls);
+    if (rc == GW_OK) {
+      hipLaunchKernelGGL(k_dmul<T>, dim3(grid_blocks((long)P.nproj * B.nb, 256), P.nk), dim3(256), 0, ls, P.nproj,
+                         B.nb, P.d_colptr, P.d_row, P.d_val, becp, u);
+      rc = gemm_combine<T>(P.nk, B.nb, P.npw_max, P.nproj, u, B.nb, (long)P.nproj * B.nb, 0, pr, P.npw_max,
+                           pstride, (cpx<T>*)B.out, B.ldo, B.so, T(1), ls);
+    }
+    if (rc) {
+      join();
+      return rc;
+    }
   }
+  join();
   GW_CHECK_HIP(hipGetLastError());
   return GW_OK;
 }
