@@ -69,6 +69,51 @@ def test_registry_solvers_on_si2():
     assert float((ms.eigenvalues - want.eigenvalues).abs().max()) < 1e-6 * HARTREE_EV
 
 
+def test_orthonormalize_degenerate_rows_and_breakdown():
+    """davidson.py:406-457 on the GPU. A zero row gets the reference's seeded
+    jitter, and the result then spans exactly what the oracle's spans. It is
+    orthonormal, zero in the padding and orthogonal to the basis it was
+    projected against (test_davidson_qr_offload.py:77-87 checks the same for
+    QR). A repeated row and a pair equal to 1e-9 break CholQR2, so the
+    Householder fallback runs and the result is still orthonormal and masked.
+    Orthogonality to the basis is not required there: the reference's
+    row-norm test does not catch such rows, and its QR completes the rank
+    freely."""
+    from gradwave_amd.davidson import _orthonormalize_b
+    from oracle.davidson import _orthonormalize_b as ref_orth
+
+    say("orthonormalize degenerate rows / breakdown si2")
+    s = system("si2")
+    cpu, gpu, p = select_k(s, [0, 3])
+    nk, m = cpu.mask.shape
+    pad = ~cpu.mask[:, None, :]
+    basis = ref_orth(rand_block(nk, 4, m, cpu.mask, seed=21), cpu.mask)
+    probe = rand_block(nk, 5, m, cpu.mask, seed=24)
+
+    def check(q, against=True):
+        gram = LA.inner(q, q).cpu()
+        assert float((gram - torch.eye(q.shape[1], dtype=gram.dtype)).abs().max()) < 1e-10
+        qc = q.cpu()
+        assert float(qc[pad.expand_as(qc)].abs().max()) == 0.0
+        if against:
+            assert float((qc.conj() @ basis.mT).abs().max()) < 1e-10
+        return qc
+
+    v = rand_block(nk, 6, m, cpu.mask, seed=22)
+    v[:, 2] = 0.0
+    q = check(_orthonormalize_b(v.to(dev()), gpu.mask, against=basis.to(dev())))
+    qr = ref_orth(v, cpu.mask, against=basis)
+
+    def span(x):  # projections of fixed probe rows onto span(x)
+        return (probe @ x.conj().mT) @ x
+
+    assert float((span(q) - span(qr)).abs().max()) < 1e-10
+    for bad in ("repeat", "near"):
+        w = rand_block(nk, 6, m, cpu.mask, seed=23)
+        w[:, 4] = w[:, 1] if bad == "repeat" else w[:, 1] + 1e-9 * rand_block(nk, 1, m, cpu.mask, seed=25)[:, 0]
+        check(_orthonormalize_b(w.to(dev()), gpu.mask, against=basis.to(dev())), against=False)
+
+
 def _synthetic(nk, npw, seed):
     """The reference's synthetic operator (test_davidson_fp32_expansion.py:24-42)."""
     torch.manual_seed(seed)
