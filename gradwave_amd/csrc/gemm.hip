@@ -174,18 +174,24 @@ __device__ __forceinline__ double reduce_splits(const T* __restrict__ pk, long s
   return (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
 }
 
+// block bx of k-point k: 64 consecutive real elements of C
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_gemm_inner_reduce(int M, int N, int splits, const T* __restrict__ part, cpx<T>* __restrict__ C, long ldc,
-                    long sc, int conj) {
-  const int k = blockIdx.y;
+__device__ __forceinline__ void reduce_body(long bx, int k, int M, int N, int splits, const T* __restrict__ part,
+                                            cpx<T>* __restrict__ C, long ldc, long sc, int conj) {
   const long nreal = (long)M * 2 * N;
-  const long e = (long)blockIdx.x * 64 + (threadIdx.x & 63);  // real element i*2N + 2j + c
+  const long e = bx * 64 + (threadIdx.x & 63);  // real element i*2N + 2j + c
   const double v = reduce_splits<T>(part + (long)k * splits * nreal, nreal, e, e < nreal, splits);
   if (threadIdx.x < 64 && e < nreal) {
     const long i = e / (2 * N), r = e - i * 2 * N;
     reinterpret_cast<T*>(C + (long)k * sc + i * ldc + (r >> 1))[r & 1] = (T)((conj && (r & 1)) ? -v : v);
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_inner_reduce(int M, int N, int splits, const T* __restrict__ part, cpx<T>* __restrict__ C, long ldc,
+                    long sc, int conj) {
+  reduce_body<T>(blockIdx.x, blockIdx.y, M, N, splits, part, C, ldc, sc, conj);
 }
 
 
@@ -201,17 +207,15 @@ template <> struct Lds<double> { static constexpr int PAD = 1; };  // 16-byte ro
 template <> struct Lds<float> { static constexpr int PAD = 2; };
 template <typename T> constexpr int kLd = kKC + Lds<T>::PAD;  // LDS row stride (complex)
 
+// work item `wid` of one inner product: tile (bx, by) of k-point bz / splits, split bz % splits
 template <typename T>
-__global__ void __launch_bounds__(256)
-k_gemm_inner_lds(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long sx,
-                 const cpx<T>* __restrict__ Y, long ldy, long sy, int splits, int kchunk,
-                 T* __restrict__ part, int ntx, int nty, long total) {
+__device__ __forceinline__ void inner_lds_body(long wid, int M, int N, int K, const cpx<T>* __restrict__ X,
+                                               long ldx, long sx, const cpx<T>* __restrict__ Y, long ldy, long sy,
+                                               int splits, int kchunk, T* __restrict__ part, int ntx, int nty) {
   using A4 = typename Mfma<T>::acc;
   constexpr int LD = kLd<T>;
   __shared__ cpx<T> As[64 * LD];
   __shared__ cpx<T> Bs[32 * LD];
-  const long wid = gemm_work_id(total);
-  if (wid < 0) return;
   const int bx = (int)(wid % ntx), by = (int)((wid / ntx) % nty), bz = (int)(wid / ((long)ntx * nty));
   const int k = bz / splits, sp = bz - k * splits;
   const int gb = sp * kchunk, ge = min(K, gb + kchunk);
@@ -297,15 +301,24 @@ k_gemm_inner_lds(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, lo
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_gemm_combine_lds(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu, long su, int conj_u,
-                   const cpx<T>* __restrict__ Y, long ldy, long sy, cpx<T>* __restrict__ Z, long ldz, long sz,
-                   T beta, int ntx, int nty, long total) {
+k_gemm_inner_lds(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long sx,
+                 const cpx<T>* __restrict__ Y, long ldy, long sy, int splits, int kchunk,
+                 T* __restrict__ part, int ntx, int nty, long total) {
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  inner_lds_body<T>(wid, M, N, K, X, ldx, sx, Y, ldy, sy, splits, kchunk, part, ntx, nty);
+}
+
+// work item `wid` of one back-projection: tile (by, bx) of k-point wid / (ntx nty)
+template <typename T>
+__device__ __forceinline__ void combine_lds_body(long wid, int A, int G, int J, const cpx<T>* __restrict__ U,
+                                                 long ldu, long su, int conj_u, const cpx<T>* __restrict__ Y,
+                                                 long ldy, long sy, cpx<T>* __restrict__ Z, long ldz, long sz,
+                                                 T beta, int ntx, int nty) {
   using A4 = typename Mfma<T>::acc;
   constexpr int LD = kLd<T>;
   __shared__ cpx<T> Ut[64 * LD];  // [a][j]
   __shared__ cpx<T> Yt[32 * LD];  // [g][j]
-  const long wid = gemm_work_id(total);
-  if (wid < 0) return;
   const int by = (int)(wid % nty), bx = (int)((wid / nty) % ntx), k = (int)(wid / ((long)ntx * nty));
   const int a0 = by * 64, g0c = bx * 32;
   const cpx<T>* Uk = U + (long)k * su;
@@ -389,6 +402,16 @@ k_gemm_combine_lds(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu, 
           *zp = (beta != T(0)) ? v + beta * (*zp) : v;
         }
       }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_combine_lds(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu, long su, int conj_u,
+                   const cpx<T>* __restrict__ Y, long ldy, long sy, cpx<T>* __restrict__ Z, long ldz, long sz,
+                   T beta, int ntx, int nty, long total) {
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  combine_lds_body<T>(wid, A, G, J, U, ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, ntx, nty);
 }
 
 // Tile-shape variants of the two LDS-staged kernels: each wave computes WM x WN
@@ -587,6 +610,86 @@ k_gemm_combine_tile(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu,
         }
       }
 }
+
+// ---------------------------------------------------------------------------
+// Grouped launches: the jobs' work items are concatenated; a workgroup finds its
+// job by binary search over the jobs' first items and runs the single-job body.
+template <typename Job>
+__device__ __forceinline__ int group_find(const Job* __restrict__ jobs, int n, long w) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_group_inner(const GroupInner* __restrict__ jobs, int n, long total) {
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const GroupInner& j = jobs[group_find(jobs, n, wid)];
+  inner_lds_body<T>(wid - j.first, j.M, j.N, j.K, (const cpx<T>*)j.X, j.ldx, j.sx, (const cpx<T>*)j.Y, j.ldy,
+                    j.sy, j.splits, j.kchunk, (T*)j.part, j.ntx, j.nty);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_group_reduce(const GroupReduce* __restrict__ jobs, int n, long total) {
+  const long b = blockIdx.x;
+  if (b >= total) return;
+  const GroupReduce& j = jobs[group_find(jobs, n, b)];
+  const long local = b - j.first;
+  reduce_body<T>(local % j.nblk, (int)(local / j.nblk), j.M, j.N, j.splits, (const T*)j.part, (cpx<T>*)j.C, j.ldc,
+                 j.sc, 0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_group_combine(const GroupCombine* __restrict__ jobs, int n, long total, T beta) {
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const GroupCombine& j = jobs[group_find(jobs, n, wid)];
+  combine_lds_body<T>(wid - j.first, j.A, j.G, j.J, (const cpx<T>*)j.U, j.ldu, j.su, 0, (const cpx<T>*)j.Y, j.ldy,
+                      j.sy, (cpx<T>*)j.Z, j.ldz, j.sz, beta, j.ntx, j.nty);
+}
+
+static int inner_splits(int nk, int M, int N, int K);
+void gemm_inner_shape(int nk, int M, int N, int K, int* splits, int* kchunk) {
+  const int s = inner_splits(nk, M, N, K);
+  int kc = (K + s - 1) / s;
+  *splits = s;
+  *kchunk = ((kc + kKC - 1) / kKC) * kKC;
+}
+
+template <typename T>
+int gemm_group_inner(const GroupInner* jobs, int n, long total, hipStream_t st) {
+  if (n <= 0 || total <= 0) return GW_OK;
+  hipLaunchKernelGGL(k_group_inner<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, jobs, n, total);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+template <typename T>
+int gemm_group_reduce(const GroupReduce* jobs, int n, long total, hipStream_t st) {
+  if (n <= 0 || total <= 0) return GW_OK;
+  hipLaunchKernelGGL(k_group_reduce<T>, dim3((unsigned)total), dim3(256), 0, st, jobs, n, total);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+template <typename T>
+int gemm_group_combine(const GroupCombine* jobs, int n, long total, T beta, hipStream_t st) {
+  if (n <= 0 || total <= 0) return GW_OK;
+  hipLaunchKernelGGL(k_group_combine<T>, dim3((unsigned)(8 * ((total + 7) / 8))), dim3(256), 0, st, jobs, n, total,
+                     beta);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+template int gemm_group_inner<double>(const GroupInner*, int, long, hipStream_t);
+template int gemm_group_inner<float>(const GroupInner*, int, long, hipStream_t);
+template int gemm_group_reduce<double>(const GroupReduce*, int, long, hipStream_t);
+template int gemm_group_reduce<float>(const GroupReduce*, int, long, hipStream_t);
+template int gemm_group_combine<double>(const GroupCombine*, int, long, double, hipStream_t);
+template int gemm_group_combine<float>(const GroupCombine*, int, long, float, hipStream_t);
 
 // per precision (GW_FP64, GW_FP32): 0: (2, 2), 1: (4, 2), 2: (2, 4), 3: (4, 4) MFMA blocks per wave
 // tile variant per [precision (0 FP64, 1 FP32)][contraction (0 inner, 1 combine)]

@@ -135,6 +135,33 @@ int combine_lib(int precision);
 void set_inner_lib(int precision, int v);
 int inner_lib(int precision);
 
+// Grouped launches (the cells of a ragged batch): one launch per contraction over
+// the concatenated work items of many independent jobs; `first` is each job's
+// first work item (strictly increasing), workgroups find their job by search.
+struct GroupInner {  // C_partials = X^H Y per K-split (the kernel path of gemm_inner)
+  const void* X; const void* Y; void* part;
+  long ldx, sx, ldy, sy, first;
+  int M, N, K, splits, kchunk, ntx, nty;
+};
+struct GroupReduce {  // C = sum of the splits' partials; nblk blocks of 64 reals per k
+  const void* part; void* C;
+  long ldc, sc, first;
+  int M, N, splits, nblk;
+};
+struct GroupCombine {  // Z = beta Z + U^T Y
+  const void* U; const void* Y; void* Z;
+  long ldu, su, ldy, sy, ldz, sz, first;
+  int A, G, J, ntx, nty;
+};
+// split count and K-chunk of gemm_inner's kernel path for this shape
+void gemm_inner_shape(int nk, int M, int N, int K, int* splits, int* kchunk);
+template <typename T>
+int gemm_group_inner(const GroupInner* jobs, int n, long total, hipStream_t st);
+template <typename T>
+int gemm_group_reduce(const GroupReduce* jobs, int n, long total, hipStream_t st);
+template <typename T>
+int gemm_group_combine(const GroupCombine* jobs, int n, long total, T beta, hipStream_t st);
+
 // Complex GEMM launchers (gemm.hip). All matrices complex interleaved, row-major.
 //   inner:   C[k](M x N) = alpha * X[k]^H-rows . Y[k]-rows  i.e. C[i][j] = sum_g conj(X[i][g]) Y[j][g]
 //   combine: Z[k](A x G) = beta * Z + sum_j U[k][j][a] * Y[k][j][g]   (U optionally conjugated)

@@ -742,6 +742,40 @@ __global__ void k_dmul(int nproj, int nb, const int* __restrict__ colptr, const 
   u[(long)k * tot + e] = acc;
 }
 
+// the D multiply of many cells in one launch: job j covers blocks first ..
+// first + nk*nblk - 1, block b of it is k = b / nblk, elements 256 (b % nblk) + tid
+struct GroupDmul {
+  const int* colptr; const int* row; const double* val; const void* becp; void* u;
+  long first;
+  int nproj, nb, nblk;
+};
+template <typename T>
+__global__ void __launch_bounds__(256) k_group_dmul(const GroupDmul* __restrict__ jobs, int n, long total) {
+  const long blk = blockIdx.x;
+  if (blk >= total) return;
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  const GroupDmul& d = jobs[lo];
+  const long local = blk - d.first;
+  const int k = (int)(local / d.nblk);
+  const long tot = (long)d.nproj * d.nb;
+  const long e = (local % d.nblk) * 256 + threadIdx.x;
+  if (e >= tot) return;
+  const int q = (int)(e / d.nb), b = (int)(e - (long)q * d.nb);
+  const cpx<T>* bk = (const cpx<T>*)d.becp + (long)k * tot;
+  cpx<T> acc = czero<T>();
+  for (int i = d.colptr[q]; i < d.colptr[q + 1]; ++i) {
+    const T dv = (T)d.val[i];
+    const cpx<T> v = bk[(long)d.row[i] * d.nb + b];
+    acc.x += dv * v.x; acc.y += dv * v.y;
+  }
+  ((cpx<T>*)d.u)[(long)k * tot + e] = acc;
+}
+
 // ---- wave-line path: natural order on every axis, cubic y/z ----
 
 
@@ -2093,6 +2127,13 @@ struct gw_batch {
   // forked from and joined back into the launch stream by events
   hipStream_t side[3] = {nullptr, nullptr, nullptr};
   hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
+  // grouped nonlocal term of the bound call (kernel path): one launch per
+  // contraction over all cells; tables [inner | reduce | dmul | combine] on the device
+  bool grouped = false;
+  void* gtab = nullptr;
+  size_t gtab_cap = 0;
+  int gn = 0;                         // jobs per table
+  long gt_inner = 0, gt_reduce = 0, gt_dmul = 0, gt_combine = 0;  // work items / blocks
 };
 constexpr int kBatchLanes = 4;  // the launch stream + 3 side streams (4 hardware queues per process)
 
@@ -2118,7 +2159,9 @@ extern "C" int gw_batch_create(gw_batch** out, gw_plan* const* plans, int n) {
     delete b;
     GW_REQUIRE(false, GW_ERR_HIP, "hipMalloc of the batch table failed");
   }
-  bool ok = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+  b->gtab_cap = (size_t)n * (sizeof(GroupInner) + sizeof(GroupReduce) + sizeof(GroupDmul) + sizeof(GroupCombine));
+  bool ok = hipMalloc(&b->gtab, b->gtab_cap) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
   for (int j = 0; j < kBatchLanes - 1 && ok; ++j)
     ok = hipStreamCreateWithFlags(&b->side[j], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&b->ev_join[j], hipEventDisableTiming) == hipSuccess;
@@ -2151,6 +2194,7 @@ extern "C" int gw_batch_destroy(gw_batch* b) {
     if (b->ev_join[j]) (void)hipEventDestroy(b->ev_join[j]);
   }
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->gtab) (void)hipFree(b->gtab);
   if (b->table) (void)hipFree(b->table);
   delete b;
   return GW_OK;
@@ -2216,6 +2260,60 @@ static int batch_bind_t(gw_batch* b, int terms, const void* const* c, const long
     off += align256(L.total);
   }
   GW_CHECK_HIP(hipMemcpy(b->table, R.data(), (size_t)n * sizeof(RagInst<T>), hipMemcpyHostToDevice));
+  b->grouped = false;
+  if ((terms & GW_TERM_NONLOCAL) && !inner_lib(prec) && !combine_lib(prec)) {
+    std::vector<GroupInner> gi;
+    std::vector<GroupReduce> gr;
+    std::vector<GroupDmul> gd;
+    std::vector<GroupCombine> gc;
+    long ti = 0, tr = 0, td = 0, tc = 0;
+    for (int i = 0; i < n; ++i) {
+      Plan& P = b->plans[i]->p;
+      const gw_batch::Bound& B = b->bound[i];
+      if (!P.nproj || B.nb == 0) continue;
+      const ApplyLayout L = apply_layout(&P, B.nb, prec);
+      const void* pr = prec == GW_FP64 ? (const void*)P.p64 : (const void*)P.p32;
+      unsigned char* becp = B.ws + L.becp_off;
+      unsigned char* u = B.ws + L.u_off;
+      const long pstride = (long)P.nproj * P.npw_max;
+      int splits = 1, kchunk = 1;
+      gemm_inner_shape(P.nk, P.nproj, B.nb, P.npw_max, &splits, &kchunk);
+      GW_REQUIRE((size_t)P.nk * splits * P.nproj * 2 * B.nb * sizeof(T) <= L.part_bytes, GW_ERR_WORKSPACE,
+                 "instance %d: split-K partials exceed the workspace", i);
+      const int intx = (2 * B.nb + 63) / 64, inty = (P.nproj + 63) / 64;
+      gi.push_back(GroupInner{pr, B.c, B.ws + L.part_off, P.npw_max, pstride, B.ldc, B.sc, ti, P.nproj, B.nb,
+                              P.npw_max, splits, kchunk, intx, inty});
+      ti += (long)intx * inty * P.nk * splits;
+      const int nblk = (int)(((long)P.nproj * 2 * B.nb + 63) / 64);
+      gr.push_back(GroupReduce{B.ws + L.part_off, becp, B.nb, (long)P.nproj * B.nb, tr, P.nproj, B.nb, splits,
+                               nblk});
+      tr += (long)nblk * P.nk;
+      const int dblk = (int)(((long)P.nproj * B.nb + 255) / 256);
+      gd.push_back(GroupDmul{P.d_colptr, P.d_row, P.d_val, becp, u, td, P.nproj, B.nb, dblk});
+      td += (long)dblk * P.nk;
+      const int cntx = (2 * P.npw_max + 63) / 64, cnty = (B.nb + 63) / 64;
+      gc.push_back(GroupCombine{u, pr, B.out, B.nb, (long)P.nproj * B.nb, P.npw_max, pstride, B.ldo, B.so, tc,
+                                B.nb, P.npw_max, P.nproj, cntx, cnty});
+      tc += (long)cntx * cnty * P.nk;
+    }
+    const int gn = (int)gi.size();
+    if (gn > 0) {
+      const size_t bytes = (size_t)gn * (sizeof(GroupInner) + sizeof(GroupReduce) + sizeof(GroupDmul) +
+                                         sizeof(GroupCombine));
+      // sized for every cell at creation and never moved, so a captured graph keeps valid tables
+      GW_REQUIRE(b->gtab && bytes <= b->gtab_cap, GW_ERR_HIP, "group tables exceed their allocation");
+      std::vector<unsigned char> host(bytes);
+      size_t o = 0;
+      std::memcpy(host.data() + o, gi.data(), gn * sizeof(GroupInner)); o += gn * sizeof(GroupInner);
+      std::memcpy(host.data() + o, gr.data(), gn * sizeof(GroupReduce)); o += gn * sizeof(GroupReduce);
+      std::memcpy(host.data() + o, gd.data(), gn * sizeof(GroupDmul)); o += gn * sizeof(GroupDmul);
+      std::memcpy(host.data() + o, gc.data(), gn * sizeof(GroupCombine));
+      GW_CHECK_HIP(hipMemcpy(b->gtab, host.data(), bytes, hipMemcpyHostToDevice));
+    }
+    b->grouped = true;
+    b->gn = gn;
+    b->gt_inner = ti; b->gt_reduce = tr; b->gt_dmul = td; b->gt_combine = tc;
+  }
   b->prec = prec;
   b->terms = terms;
   b->rmax = 1;
@@ -2245,6 +2343,20 @@ static int batch_apply_t(gw_batch* b, hipStream_t st) {
   GW_CHECK_HIP(hipGetLastError());
   if (!(b->terms & GW_TERM_NONLOCAL)) return GW_OK;
   const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
+  if (b->grouped) {  // one launch per contraction over every cell
+    if (b->gn == 0) return GW_OK;
+    const int gn = b->gn;
+    const GroupInner* gi = (const GroupInner*)b->gtab;
+    const GroupReduce* gr = (const GroupReduce*)(gi + gn);
+    const GroupDmul* gd = (const GroupDmul*)(gr + gn);
+    const GroupCombine* gc = (const GroupCombine*)(gd + gn);
+    int rc = gemm_group_inner<T>(gi, gn, b->gt_inner, st);
+    if (!rc) rc = gemm_group_reduce<T>(gr, gn, b->gt_reduce, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_group_dmul<T>, dim3((unsigned)b->gt_dmul), dim3(256), 0, st, gd, gn, b->gt_dmul);
+    GW_CHECK_HIP(hipGetLastError());
+    return gemm_group_combine<T>(gc, gn, b->gt_combine, T(1), st);
+  }
   GW_CHECK_HIP(hipEventRecord(b->ev_fork, st));
   for (int j = 0; j < kBatchLanes - 1; ++j) GW_CHECK_HIP(hipStreamWaitEvent(b->side[j], b->ev_fork, 0));
   auto join = [&]() {  // the launch stream waits for every side stream's chains

@@ -222,11 +222,14 @@ int gw_tune_get(int key, int arg);
  * No reference counterpart: the reference applies one operator per call
  * (core/batch.py:480-505); a batch of independent cells (different boxes,
  * spheres, potentials, projector and band counts) runs here as ONE launch per
- * local-term pass over all instances' work items, then each instance's
- * projector GEMMs, spread over the launch stream and three side streams of the
- * batch (default priority; forked from and joined back into the launch stream
- * by events, so a graph capture of apply_bound records them too; calls through
- * one batch must not overlap). Plans must take the generic-size plane kernels
+ * local-term pass over all instances' work items, then the projector GEMMs of
+ * every instance as one launch per contraction (inner product, split-K sum, D
+ * multiply, back-projection; job tables uploaded by bind). With a rocBLAS
+ * contraction selected (GW_TUNE_*_LIB) the instances' GEMM chains run instead
+ * on the launch stream and three side streams of the batch (default priority;
+ * forked from and joined back into the launch stream by events, so a graph
+ * capture of apply_bound records them too). Calls through one batch must not
+ * overlap. Plans must take the generic-size plane kernels
  * (x-plane fits LDS). bind uploads the per-instance block pointers, strides and
  * band counts (nb[i] = 0 leaves instance i out of the call) and is synchronous;
  * apply_bound only launches (so it can be captured in a HIP graph); apply = both.
