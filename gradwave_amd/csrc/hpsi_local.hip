@@ -273,17 +273,40 @@ __device__ void xpass_body(const Geom& g, int L, int nbt, int chunk, int bgrp, c
     }
     vr[i] = v;
   }
+  // The first PF passes of a band's active-plane elements are loaded one band
+  // ahead, so their latency overlaps the previous band's transforms (none for
+  // the 16-radix codelets, whose registers are spent already).
+  constexpr int PF = RMAX <= 10 ? 8 : 0;
+  const int nal = g.na * L;
+  cpx<T> pf[PF > 0 ? PF : 1];
+  auto prefetch = [&](int bi) {
+    const cpx<T>* bb = box + (long)bi * g.na * n23 + j0;
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int e = threadIdx.x + i * blockDim.x;
+      const int a = e / L, l = e - a * L;
+      pf[i] = (e < nal && l < nl) ? bb[(long)a * n23 + l] : czero<T>();
+    }
+  };
+  if (bgrp * kXBandGroup < nbt) prefetch(bgrp * kXBandGroup);
   for (int bg = 0; bg < kXBandGroup; ++bg) {
     const int bi = bgrp * kXBandGroup + bg;
     if (bi >= nbt) break;
     for (int e = threadIdx.x; e < ne; e += blockDim.x) tile[e] = czero<T>();
     __syncthreads();
     cpx<T>* bb = box + (long)bi * g.na * n23 + j0;
-    for (int e = threadIdx.x; e < g.na * L; e += blockDim.x) {
+#pragma unroll
+    for (int i = 0; i < PF; ++i) {
+      const int e = threadIdx.x + i * blockDim.x;
+      const int a = e / L, l = e - a * L;
+      if (e < nal && l < nl) tile[g.planes[a] * L + l] = pf[i];
+    }
+    for (int e = threadIdx.x + PF * blockDim.x; e < nal; e += blockDim.x) {
       const int a = e / L, l = e - a * L;
       if (l < nl) tile[g.planes[a] * L + l] = bb[(long)a * n23 + l];
     }
     __syncthreads();
+    if (bg + 1 < kXBandGroup && bi + 1 < nbt) prefetch(bi + 1);
     line_fft_inverse<T, true, RMAX>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
     __syncthreads();
 #pragma unroll
