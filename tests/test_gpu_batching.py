@@ -48,19 +48,32 @@ def _rel(a, b):
     return float((a - b).abs().max() / b.abs().max())
 
 
+@pytest.mark.parametrize("library", [0, 1])
 @pytest.mark.parametrize("dtype,tol", [(torch.complex128, 1e-12), (torch.complex64, 2e-5)])
-def test_ragged_apply_matches_single_operators(dtype, tol):
+def test_ragged_apply_matches_single_operators(dtype, tol, library):
+    """The batched apply against each operator on its own. With the kernels here
+    the cells' projector contractions run as one grouped launch each; with the
+    rocBLAS contractions selected, as per-cell chains over four streams."""
+    from gradwave_amd import _lib as L
     from gradwave_amd import batching as B
 
     wls = _instances()
-    say("ragged apply " + ", ".join(w.name for w in wls))
+    say(f"ragged apply library={library} " + ", ".join(w.name for w in wls))
     hams = _hams(wls)
     batch = B.RaggedBatch(hams)
     nbs = [w.nbands for w in wls]
     nbs[3] = 0  # one instance sits the call out
     nbs[5] = 3
     cs = _blocks(wls, dtype, nbs=nbs)
-    outs = batch.apply(cs)
+    lib = L.lib()
+    prec = L.precision_of(dtype)
+    try:
+        L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, library), "gw_tune_set")
+        L.check(lib.gw_tune_set(L.GW_TUNE_INNER_LIB, prec, library), "gw_tune_set")
+        outs = batch.apply(cs)
+    finally:
+        L.check(lib.gw_tune_set(L.GW_TUNE_COMBINE_LIB, prec, 0), "gw_tune_set")
+        L.check(lib.gw_tune_set(L.GW_TUNE_INNER_LIB, prec, 0), "gw_tune_set")
     assert outs[3] is None
     for i, (h, c, o) in enumerate(zip(hams, cs, outs)):
         if c is None:
