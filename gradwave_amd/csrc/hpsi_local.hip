@@ -807,6 +807,7 @@ __global__ void __launch_bounds__(256) k_group_dmul(const GroupDmul* __restrict_
 // waves per workgroup of the wave-line plane kernels: kPW, or 4 when the
 // compact stick rows plus kPW exchange areas exceed LDS (fp64, 120^3)
 constexpr int kPW = 8;
+constexpr size_t kLdsPerCu = 160 * 1024;
 constexpr int kAdjBandGroup = 16;  // bands reduced per x-pass workgroup in the V adjoint
 
 template <typename T, int N, int M, int S, int NW>
@@ -1013,11 +1014,16 @@ static size_t plane_w_lds(const Plan& P) {
          N * sizeof(int);
 }
 
-// waves per plane workgroup for this plan, 0 if even 4 waves do not fit
+// waves per plane workgroup for this plan: 16 when one 16-wave workgroup keeps
+// more waves resident than kPW-wave ones (large boxes, where the stick rows
+// leave room for one workgroup per CU only), else kPW, else 4; 0 if none fits
 template <typename T, int N>
 static int plane_waves(const Plan& P) {
   const size_t cap = (size_t)std::min(P.lds_max, kMaxLds);
-  if (plane_w_lds<T, N, kPW>(P) <= cap) return kPW;
+  auto resident = [&](int nw, size_t lds) { return lds <= cap ? nw * (int)(kLdsPerCu / lds) : 0; };
+  const int w16 = resident(16, plane_w_lds<T, N, 16>(P)), w8 = resident(kPW, plane_w_lds<T, N, kPW>(P));
+  if (sizeof(T) == 4 && w16 > w8) return 16;
+  if (w8 > 0) return kPW;
   if (plane_w_lds<T, N, 4>(P) <= cap) return 4;
   return 0;
 }
@@ -1055,7 +1061,8 @@ static int launch_wave_path_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, 
   GW_REQUIRE((long)P.na * N * N * (long)sizeof(cpx<T>) <= (long)kBufOff, GW_ERR_UNSUPPORTED,
              "one band's x-planes exceed 2 GiB");
   hipEvent_t ev = tm.begin();
-  if (nw == kPW) launch_planes<T, N, kPW>(true, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
+  if (nw == 16) launch_planes<T, N, 16>(true, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
+  else if (nw == kPW) launch_planes<T, N, kPW>(true, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
   else launch_planes<T, N, 4>(true, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
   tm.end(ST_PLANE_INV, ev);
   ev = tm.begin();
@@ -1090,7 +1097,8 @@ static int launch_wave_path_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, 
                        tw_of<T>(&P, 0), nchunk, xtotal);
   tm.end(ST_XPASS, ev);
   ev = tm.begin();
-  if (nw == kPW) launch_planes<T, N, kPW>(false, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
+  if (nw == 16) launch_planes<T, N, 16>(false, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
+  else if (nw == kPW) launch_planes<T, N, kPW>(false, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
   else launch_planes<T, N, 4>(false, P, g, total, c, ldc, sc, box, t, out, ldo, so, scale, kin, st);
   tm.end(ST_PLANE_FWD, ev);
   GW_CHECK_HIP(hipGetLastError());
@@ -1172,7 +1180,10 @@ static int adjv_wave_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, long ld
   const int nw = plane_waves<T, N>(P);
   GW_REQUIRE(nw > 0, GW_ERR_UNSUPPORTED, "x-plane sticks exceed LDS");
   cpx<T>* nullout = nullptr;
-  if (nw == kPW) {
+  if (nw == 16) {
+    launch_planes<T, N, 16>(true, P, g, total, c, ldc, sc, boxc, nullptr, nullout, 0, 0, T(1), false, st);
+    launch_planes<T, N, 16>(true, P, g, total, gr, ldg, sg, boxg, nullptr, nullout, 0, 0, T(1), false, st);
+  } else if (nw == kPW) {
     launch_planes<T, N, kPW>(true, P, g, total, c, ldc, sc, boxc, nullptr, nullout, 0, 0, T(1), false, st);
     launch_planes<T, N, kPW>(true, P, g, total, gr, ldg, sg, boxg, nullptr, nullout, 0, 0, T(1), false, st);
   } else {
@@ -1253,7 +1264,9 @@ static int density_wave_n(Plan& P, const Geom& g, int nbt, const cpx<T>* c, long
   const int nw = plane_waves<T, N>(P);
   GW_REQUIRE(nw > 0, GW_ERR_UNSUPPORTED, "x-plane sticks exceed LDS");
   cpx<T>* nullout = nullptr;
-  if (nw == kPW)
+  if (nw == 16)
+    launch_planes<T, N, 16>(true, P, g, total, c, ldc, sc, box, nullptr, nullout, 0, 0, T(1), false, st);
+  else if (nw == kPW)
     launch_planes<T, N, kPW>(true, P, g, total, c, ldc, sc, box, nullptr, nullout, 0, 0, T(1), false, st);
   else
     launch_planes<T, N, 4>(true, P, g, total, c, ldc, sc, box, nullptr, nullout, 0, 0, T(1), false, st);
@@ -1311,6 +1324,7 @@ static void set_wave_attr_nw(int want) {
 }
 template <typename T, int N>
 static void set_wave_attr_n(int want) {
+  set_wave_attr_nw<T, N, 16>(want);
   set_wave_attr_nw<T, N, kPW>(want);
   set_wave_attr_nw<T, N, 4>(want);
 }
