@@ -22,11 +22,17 @@ Modules and the reference files they restate (paths under the reference's
 * ``systems``      -- the synthetic Si/Al workloads (cells, k-lists, jitter)
 * ``dense``        -- brute-force dense H matrix (closed-form check of the apply)
 * ``generalized``  -- scf/uspp_batch.py:27-233 (S-metric generalized Davidson)
-* ``energies``     -- band energy and frozen-potential forces
-                      (core/energies/{local_pp,nl_pp}.py, postscf/forces.py:138-223)
+* ``energies``     -- band energy from eigenvalues (postscf/mae.py:349) and at
+                      fixed orbitals, with frozen-potential forces by autograd
+                      in the positions (postscf/forces.py:168-213 pattern)
+* ``gamma``        -- core/gamma.py:70-290 (Gamma half sphere, real embedding)
 
-Parity status of each function is stated in its module docstring; every function
-here is pinned by at least one independent check in ``tests/`` (closed forms,
-brute force, invariants or the golden vectors written from the reference's leaf
-modules under ``tests/golden/``).
+Parity status of each function is stated in its module docstring. Each public
+function is run by at least one test under ``tests/`` that a plausible mistake
+would fail (closed forms, brute force, invariants, finite differences, or the
+golden vectors written from the reference's leaf modules under
+``tests/golden/``): ``test_oracle_closed_forms.py``, ``test_oracle_golden.py``,
+``test_oracle_adjoint.py``, ``test_oracle_gamma.py``,
+``test_oracle_generalized.py`` and ``test_oracle_solvers.py`` (two-stage solve,
+registry adapter, apply_cols, band energy).
 """

@@ -69,6 +69,32 @@ def test_position_gradient_matches_central_fd(case):
     assert float((fd - grad).norm() / grad.norm()) < 1e-6
 
 
+def test_energies_module_forces_match_central_fd(case):
+    """oracle.energies.band_energy_forces (the forces.py:168-213 pattern) on a
+    padded block with per-band weights: F = -dE/dtau against central FD."""
+    from oracle.energies import band_energy_forces, band_energy_psi
+
+    s, cs, _, _ = case
+    m = max(int(s.spheres[ik].npw) for ik in SEL)
+    c = torch.zeros(len(SEL), 4, m, dtype=torch.complex128)
+    for j, ik in enumerate(SEL):
+        c[j, :, : cs[ik].shape[1]] = cs[ik]
+    w = torch.tensor([[2.0, 2.0, 1.0, 0.5], [0.25, 2.0, 0.0, 1.0]], dtype=torch.float64)
+    e, f = band_energy_forces(s, c, w, kpoints=SEL)
+    h = 1e-4 * BOHR_ANG
+    fd = torch.zeros_like(f)
+    with torch.no_grad():
+        for a in range(f.shape[0]):
+            for x in range(3):
+                d = torch.zeros_like(s.positions)
+                d[a, x] = h
+                ep = band_energy_psi(s, c, w, s.positions + d, kpoints=SEL)
+                em = band_energy_psi(s, c, w, s.positions - d, kpoints=SEL)
+                fd[a, x] = -(ep - em) / (2 * h)
+    assert float((fd - f).norm() / f.norm()) < 1e-6
+    assert abs(e - float(band_energy_psi(s, c, w, s.positions, kpoints=SEL))) == 0.0
+
+
 def test_operator_input_gradients_match_directional_fd(case):
     s, cs, ws, pds = case
     g = torch.Generator().manual_seed(5)
