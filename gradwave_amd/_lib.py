@@ -51,6 +51,9 @@ SIGNATURES = {
     "gw_residual": (_I, [_I, _I, _I, _I, _P, _P, _L, _L, _P, _P, _P, _P]),
     "gw_teter_expand": (_I, [_I, _I, _I, _I, _I, _P, _P, _L, _L, _P, _L, _P, _P, _L, _L, _P]),
     "gw_row_norms": (_I, [_I, _I, _I, _I, _P, _L, _L, _P, _P]),
+    "gw_cg_update": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _L, _P, _L, _P, _P, _P, _P, _P]),
+    "gw_cg_direction": (_I, [_I, _I, _I, _P, _P, _P, _P, _L, _P, _L, _P]),
+    "gw_row_dots": (_I, [_I, _I, _I, _P, _P, _P, _L, _P, _L, _P, _P, _P]),
     "gw_hpsi_adjoint_v_workspace_bytes": (_Z, [_P, _I, _I]),
     "gw_hpsi_adjoint_v": (_I, [_P, _I, _P, _L, _L, _P, _L, _L, _I, _P, _P, _Z, _P]),
     "gw_density_workspace_bytes": (_Z, [_P, _I, _I]),
@@ -69,8 +72,9 @@ SIGNATURES = {
     "gw_batch_create": (_I, [C.POINTER(_P), _P, _I]),
     "gw_batch_destroy": (_I, [_P]),
     "gw_batch_workspace_bytes": (_Z, [_P, _P, _I]),
-    "gw_batch_bind": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _Z]),
+    "gw_batch_bind": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P, C.POINTER(_P)]),
     "gw_batch_apply_bound": (_I, [_P, _P]),
+    "gw_batch_binding_destroy": (_I, [_P]),
     "gw_batch_apply": (_I, [_P, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _Z, _P]),
     "gw_toeplitz_build": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "gw_gamma_pack": (_I, [_I, _I, _I, _I, _P, _L, _P, _P, _P, _L, _P]),

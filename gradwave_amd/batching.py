@@ -9,7 +9,9 @@ between tiny launches. Here:
   projector counts and band counts -- as ONE launch per local-term pass over the
   concatenated work items of every instance (``gw_batch_apply``), followed by
   each instance's MFMA projector GEMMs; ``capture`` records a fixed-shape
-  batched apply in a HIP graph and ``replay`` re-launches it.
+  batched apply in a HIP graph and ``replay`` re-launches it; the graph owns
+  its binding (device tables, workspace), so eager calls in between leave it
+  valid.
 * ``davidson_ragged`` runs one ``davidson_steps`` generator per instance (the
   same solver as ``davidson_batched``) in lockstep, decode-style: each round it
   gathers every pending apply request into one ragged apply, the pending
@@ -49,7 +51,6 @@ class RaggedBatch:
         self._lib = L.lib()
         L.check(self._lib.gw_batch_create(C.byref(self.handle), arr, n), "gw_batch_create")
         self._ws: dict[int, torch.Tensor] = {}
-        self._keep = None
         self.device = self.hams[0].device
 
     def __len__(self) -> int:
@@ -106,33 +107,68 @@ class RaggedBatch:
                                          ws.numel(), L.stream_ptr(self.device)), "gw_batch_apply")
         return outs
 
-    def bind(self, cs: list, outs: list, terms: int = L.GW_TERM_ALL) -> None:
-        """Fix the blocks of later ``apply_bound`` calls (synchronous upload)."""
+    def bind(self, cs: list, outs: list, terms: int = L.GW_TERM_ALL) -> "RaggedBinding":
+        """Fix the blocks of a batched apply: a binding with device tables and a
+        workspace of its own (uploaded on the current stream), launched by
+        ``RaggedBinding.apply``; later calls through this batch leave it valid."""
         prec, cp, ldc, sc, op, ldo, so, nb = self._args(cs, outs)
         if prec is None:
             raise ValueError("nothing to bind")
-        ws = self._workspace(nb, prec)
+        need = int(self._lib.gw_batch_workspace_bytes(self.handle, nb, prec))
+        ws = torch.empty(max(need, 256), dtype=torch.uint8, device=self.device)
+        handle = C.c_void_p()
         L.check(self._lib.gw_batch_bind(self.handle, prec, terms, cp, ldc, sc, op, ldo, so, nb, ws.data_ptr(),
-                                        ws.numel()), "gw_batch_bind")
-        self._keep = (list(cs), list(outs), ws)
+                                        ws.numel(), L.stream_ptr(self.device), C.byref(handle)), "gw_batch_bind")
+        return RaggedBinding(self, handle, (list(cs), list(outs), ws))
 
-    def apply_bound(self) -> None:
-        L.check(self._lib.gw_batch_apply_bound(self.handle, L.stream_ptr(self.device)), "gw_batch_apply_bound")
-
-    def capture(self, cs: list, outs: list, terms: int = L.GW_TERM_ALL) -> torch.cuda.CUDAGraph:
+    def capture(self, cs: list, outs: list, terms: int = L.GW_TERM_ALL) -> "RaggedGraph":
         """Bind fixed blocks and record the batched apply in a HIP graph; the
-        graph's ``replay()`` re-runs it on whatever the bound inputs hold."""
-        self.bind(cs, outs, terms)
-        self.apply_bound()  # warm (first-use module loads stay out of the graph)
+        graph's ``replay()`` re-runs it on whatever the bound inputs hold. The
+        graph owns its binding (tables, workspace, references to the blocks), so
+        later ``apply``/``bind``/``capture`` calls through the batch do not
+        touch what it replays."""
+        binding = self.bind(cs, outs, terms)
+        binding.apply()  # warm (first-use module loads stay out of the graph)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(s):
             with torch.cuda.graph(g, stream=s):
-                self.apply_bound()
+                binding.apply()
         torch.cuda.current_stream().wait_stream(s)
-        return g
+        return RaggedGraph(g, binding)
+
+
+class RaggedBinding:
+    """One bound batched apply (``RaggedBatch.bind``): owns its device tables and
+    workspace and keeps its input and output blocks alive."""
+
+    def __init__(self, batch: RaggedBatch, handle, keep) -> None:
+        self.batch, self.handle, self._keep = batch, handle, keep
+
+    def apply(self) -> None:
+        L.check(self.batch._lib.gw_batch_apply_bound(self.handle, L.stream_ptr(self.batch.device)),
+                "gw_batch_apply_bound")
+
+    def __del__(self) -> None:
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self.batch._lib.gw_batch_binding_destroy(h)
+            except Exception:
+                pass
+            self.handle = C.c_void_p()
+
+
+class RaggedGraph:
+    """A HIP graph of one bound batched apply; holds the binding it replays."""
+
+    def __init__(self, graph: torch.cuda.CUDAGraph, binding: RaggedBinding) -> None:
+        self.graph, self.binding = graph, binding
+
+    def replay(self) -> None:
+        self.graph.replay()
 
 
 # ---------------------------------------------------------------- lockstep solves

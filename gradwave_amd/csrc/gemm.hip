@@ -762,7 +762,7 @@ template size_t gemm_inner_partials_bytes<float>(int, int, int, int);
 
 void set_inner_lib(int precision, int v) { g_inner_lib[precision == GW_FP32] = v ? 1 : 0; }
 int inner_lib(int precision) { return g_inner_lib[precision == GW_FP32]; }
-static rocblas_handle blas_handle();
+static rocblas_handle blas_handle(hipStream_t st);
 
 // Split-K inner product on rocBLAS: one strided-batched GEMM over the K-chunks
 // (plus one for a short last chunk) writes conj(C) partials [chunk][M][N] in the
@@ -771,9 +771,8 @@ static rocblas_handle blas_handle();
 template <typename T>
 static int inner_rocblas(int k, int M, int N, int K, int kc, int nfull, const cpx<T>* X, long ldx,
                          const cpx<T>* Y, long ldy, T* part, hipStream_t st) {
-  rocblas_handle h = blas_handle();
+  rocblas_handle h = blas_handle(st);
   GW_REQUIRE(h, GW_ERR_HIP, "rocblas handle unavailable");
-  GW_REQUIRE(rocblas_set_stream(h, st) == rocblas_status_success, GW_ERR_HIP, "rocblas_set_stream failed");
   const long slot = (long)M * N;  // complex elements per partial
   const int rem = K - nfull * kc;
   rocblas_status rs = rocblas_status_success;
@@ -956,22 +955,37 @@ static int g_combine_lib[2] = {0, 0};
 void set_combine_lib(int precision, int v) { g_combine_lib[precision == GW_FP32] = v ? 1 : 0; }
 int combine_lib(int precision) { return g_combine_lib[precision == GW_FP32]; }
 
-static rocblas_handle blas_handle() {
+// One rocBLAS handle per (device, stream), bound to that stream once: GEMMs
+// issued on different streams (the ragged batch's side lanes) never share a
+// handle's stream binding or its device workspace.
+static rocblas_handle blas_handle(hipStream_t st) {
+  struct Entry {
+    int dev;
+    hipStream_t st;
+    rocblas_handle h;
+  };
   static std::mutex mu;
-  static rocblas_handle handles[64] = {nullptr};
+  static std::vector<Entry> handles;
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   std::lock_guard<std::mutex> lock(mu);
-  if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success) handles[dev] = nullptr;
-  return handles[dev];
+  for (const Entry& e : handles)
+    if (e.dev == dev && e.st == st) return e.h;
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return nullptr;
+  if (rocblas_set_stream(h, st) != rocblas_status_success) {
+    (void)rocblas_destroy_handle(h);
+    return nullptr;
+  }
+  handles.push_back(Entry{dev, st, h});
+  return h;
 }
 
 template <typename T>
 static int combine_rocblas(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su, int conj_u,
                            const cpx<T>* Y, long ldy, long sy, cpx<T>* Z, long ldz, long sz, T beta, hipStream_t st) {
-  rocblas_handle h = blas_handle();
+  rocblas_handle h = blas_handle(st);
   GW_REQUIRE(h, GW_ERR_HIP, "rocblas handle unavailable");
-  GW_REQUIRE(rocblas_set_stream(h, st) == rocblas_status_success, GW_ERR_HIP, "rocblas_set_stream failed");
   const rocblas_operation tb = conj_u ? rocblas_operation_conjugate_transpose : rocblas_operation_transpose;
   rocblas_status rs;
   if constexpr (sizeof(T) == 8) {

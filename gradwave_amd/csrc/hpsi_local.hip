@@ -2144,35 +2144,74 @@ __global__ void __launch_bounds__(kPlaneThreads) k_plane_fwd_rag(const RagInst<T
 
 }  // namespace gw
 
-struct gw_batch {
-  std::vector<gw_plan*> plans;
-  void* table = nullptr;  // device copy of the descriptors
-  size_t table_cap = 0;
-  // bound call
+// Launch parameters of one bound batched call and the device copy of its
+// descriptor tables: [RagInst x n | GroupInner | GroupReduce | GroupDmul |
+// GroupCombine] (gn each). The tables reach the device by hipMemcpyAsync on the
+// launch stream from pinned staging, so an upload is ordered after every
+// earlier launch on that stream and no host-synchronous copy is needed.
+struct BatchBound {
+  const void* c; long ldc, sc;
+  void* out; long ldo, so;
+  int nb;
+  unsigned char* ws;
+};
+struct BatchBind {
   int prec = -1, terms = 0, rmax = 16;
   long tot13 = 0, tot2 = 0;
   size_t lds13 = 0, lds2 = 0;
-  struct Bound {
-    const void* c; long ldc, sc;
-    void* out; long ldo, so;
-    int nb;
-    unsigned char* ws;
-  };
-  std::vector<Bound> bound;
+  std::vector<BatchBound> bound;
+  bool grouped = false;
+  int gn = 0;                                         // jobs per group table
+  long gt_inner = 0, gt_reduce = 0, gt_dmul = 0, gt_combine = 0;  // work items / blocks
+  size_t gtab_off = 0;                                // byte offset of the group tables
+  unsigned char* dev = nullptr;                       // device tables
+  unsigned char* host = nullptr;                      // pinned staging of the same bytes
+  size_t cap = 0;
+  hipEvent_t done = nullptr;  // recorded after the last launch that read `dev` (ring slots)
+  bool pending = false;
+};
+
+static size_t batch_table_cap(int n) {
+  return align256((size_t)n * sizeof(RagInst<double>)) +
+         (size_t)n * (sizeof(GroupInner) + sizeof(GroupReduce) + sizeof(GroupDmul) + sizeof(GroupCombine));
+}
+
+static bool bind_alloc(BatchBind& s, int n) {
+  s.cap = batch_table_cap(n);
+  if (hipMalloc(&s.dev, s.cap) != hipSuccess) return false;
+  if (hipHostMalloc(&s.host, s.cap, hipHostMallocDefault) != hipSuccess) return false;
+  return hipEventCreateWithFlags(&s.done, hipEventDisableTiming) == hipSuccess;
+}
+
+static void bind_free(BatchBind& s) {
+  if (s.done && s.pending) (void)hipEventSynchronize(s.done);
+  if (s.dev) (void)hipFree(s.dev);
+  if (s.host) (void)hipHostFree(s.host);
+  if (s.done) (void)hipEventDestroy(s.done);
+  s.dev = s.host = nullptr;
+  s.done = nullptr;
+}
+
+constexpr int kBatchLanes = 4;  // the launch stream + 3 side streams (4 hardware queues per process)
+constexpr int kBatchRing = 4;   // table slots of eager (unbound) calls
+
+struct gw_batch {
+  std::vector<gw_plan*> plans;
   // the instances' nonlocal chains (small, latency-bound GEMMs) are spread over
   // the launch stream and kBatchLanes - 1 side streams of default priority,
   // forked from and joined back into the launch stream by events
-  hipStream_t side[3] = {nullptr, nullptr, nullptr};
-  hipEvent_t ev_fork = nullptr, ev_join[3] = {nullptr, nullptr, nullptr};
-  // grouped nonlocal term of the bound call (kernel path): one launch per
-  // contraction over all cells; tables [inner | reduce | dmul | combine] on the device
-  bool grouped = false;
-  void* gtab = nullptr;
-  size_t gtab_cap = 0;
-  int gn = 0;                         // jobs per table
-  long gt_inner = 0, gt_reduce = 0, gt_dmul = 0, gt_combine = 0;  // work items / blocks
+  hipStream_t side[kBatchLanes - 1] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_join[kBatchLanes - 1] = {nullptr, nullptr, nullptr};
+  BatchBind ring[kBatchRing];
+  int next = 0;
 };
-constexpr int kBatchLanes = 4;  // the launch stream + 3 side streams (4 hardware queues per process)
+
+// A bound call with tables of its own (never rewritten by later calls), so a
+// HIP graph that recorded gw_batch_apply_bound stays valid until destroyed.
+struct gw_batch_binding {
+  gw_batch* b;
+  BatchBind s;
+};
 
 static size_t batch_ws_bytes(const gw_batch* b, const int* nb, int precision) {
   size_t tot = 0;
@@ -2191,20 +2230,14 @@ extern "C" int gw_batch_create(gw_batch** out, gw_plan* const* plans, int n) {
     }
     b->plans.push_back(plans[i]);
   }
-  b->table_cap = (size_t)n * sizeof(RagInst<double>);
-  if (hipMalloc(&b->table, b->table_cap) != hipSuccess) {
-    delete b;
-    GW_REQUIRE(false, GW_ERR_HIP, "hipMalloc of the batch table failed");
-  }
-  b->gtab_cap = (size_t)n * (sizeof(GroupInner) + sizeof(GroupReduce) + sizeof(GroupDmul) + sizeof(GroupCombine));
-  bool ok = hipMalloc(&b->gtab, b->gtab_cap) == hipSuccess;
-  ok = ok && hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
+  bool ok = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess;
   for (int j = 0; j < kBatchLanes - 1 && ok; ++j)
     ok = hipStreamCreateWithFlags(&b->side[j], hipStreamNonBlocking) == hipSuccess &&
          hipEventCreateWithFlags(&b->ev_join[j], hipEventDisableTiming) == hipSuccess;
+  for (int j = 0; j < kBatchRing && ok; ++j) ok = bind_alloc(b->ring[j], n);
   if (!ok) {
     gw_batch_destroy(b);
-    GW_REQUIRE(false, GW_ERR_HIP, "creating the batch's side streams failed");
+    GW_REQUIRE(false, GW_ERR_HIP, "allocating the batch's tables, side streams or events failed");
   }
   const int want = kMaxLds;
   (void)hipFuncSetAttribute((const void*)k_plane_inv_rag<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
@@ -2231,8 +2264,7 @@ extern "C" int gw_batch_destroy(gw_batch* b) {
     if (b->ev_join[j]) (void)hipEventDestroy(b->ev_join[j]);
   }
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
-  if (b->gtab) (void)hipFree(b->gtab);
-  if (b->table) (void)hipFree(b->table);
+  for (int j = 0; j < kBatchRing; ++j) bind_free(b->ring[j]);
   delete b;
   return GW_OK;
 }
@@ -2242,13 +2274,15 @@ extern "C" size_t gw_batch_workspace_bytes(const gw_batch* b, const int* nb, int
   return batch_ws_bytes(b, nb, precision);
 }
 
+// Fill s's tables for this call on the host and upload them on `st`.
 template <typename T>
-static int batch_bind_t(gw_batch* b, int terms, const void* const* c, const long* ldc, const long* sc,
-                        void* const* out, const long* ldo, const long* so, const int* nb, unsigned char* ws) {
+static int batch_bind_t(gw_batch* b, BatchBind& s, int terms, const void* const* c, const long* ldc, const long* sc,
+                        void* const* out, const long* ldo, const long* so, const int* nb, unsigned char* ws,
+                        hipStream_t st) {
   const int n = (int)b->plans.size();
   const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
   std::vector<RagInst<T>> R(n);
-  b->bound.assign(n, gw_batch::Bound{});
+  std::vector<BatchBound> bound(n, BatchBound{});
   long w13 = 0, w2 = 0;
   size_t lds13 = 0, lds2 = 0, off = 0;
   for (int i = 0; i < n; ++i) {
@@ -2260,7 +2294,6 @@ static int batch_bind_t(gw_batch* b, int terms, const void* const* c, const long
       R[i].nchunk = 1;
       R[i].w13 = w13;
       R[i].w2 = w2;
-      b->bound[i] = gw_batch::Bound{nullptr, 0, 0, nullptr, 0, 0, 0, nullptr};
       continue;
     }
     GW_REQUIRE(c[i] && out[i], GW_ERR_ARG, "instance %d: bad block", i);
@@ -2293,20 +2326,19 @@ static int batch_bind_t(gw_batch* b, int terms, const void* const* c, const long
     w2 += (long)d.nchunk * ((d.nbt + kXBandGroup - 1) / kXBandGroup);
     lds13 = std::max(lds13, (size_t)P.n[1] * P.n[2] * sizeof(cpx<T>));
     lds2 = std::max(lds2, (size_t)P.n[0] * d.XL * sizeof(cpx<T>));
-    b->bound[i] = gw_batch::Bound{c[i], ldc[i], sc[i], out[i], ldo[i], so[i], nb[i], ws + off};
+    bound[i] = BatchBound{c[i], ldc[i], sc[i], out[i], ldo[i], so[i], nb[i], ws + off};
     off += align256(L.total);
   }
-  GW_CHECK_HIP(hipMemcpy(b->table, R.data(), (size_t)n * sizeof(RagInst<T>), hipMemcpyHostToDevice));
-  b->grouped = false;
-  if ((terms & GW_TERM_NONLOCAL) && !inner_lib(prec) && !combine_lib(prec)) {
-    std::vector<GroupInner> gi;
-    std::vector<GroupReduce> gr;
-    std::vector<GroupDmul> gd;
-    std::vector<GroupCombine> gc;
-    long ti = 0, tr = 0, td = 0, tc = 0;
+  std::vector<GroupInner> gi;
+  std::vector<GroupReduce> gr;
+  std::vector<GroupDmul> gd;
+  std::vector<GroupCombine> gc;
+  long ti = 0, tr = 0, td = 0, tc = 0;
+  const bool grouped = (terms & GW_TERM_NONLOCAL) && !inner_lib(prec) && !combine_lib(prec);
+  if (grouped) {
     for (int i = 0; i < n; ++i) {
       Plan& P = b->plans[i]->p;
-      const gw_batch::Bound& B = b->bound[i];
+      const BatchBound& B = bound[i];
       if (!P.nproj || B.nb == 0) continue;
       const ApplyLayout L = apply_layout(&P, B.nb, prec);
       const void* pr = prec == GW_FP64 ? (const void*)P.p64 : (const void*)P.p32;
@@ -2333,66 +2365,70 @@ static int batch_bind_t(gw_batch* b, int terms, const void* const* c, const long
                                 B.nb, P.npw_max, P.nproj, cntx, cnty});
       tc += (long)cntx * cnty * P.nk;
     }
-    const int gn = (int)gi.size();
-    if (gn > 0) {
-      const size_t bytes = (size_t)gn * (sizeof(GroupInner) + sizeof(GroupReduce) + sizeof(GroupDmul) +
-                                         sizeof(GroupCombine));
-      // sized for every cell at creation and never moved, so a captured graph keeps valid tables
-      GW_REQUIRE(b->gtab && bytes <= b->gtab_cap, GW_ERR_HIP, "group tables exceed their allocation");
-      std::vector<unsigned char> host(bytes);
-      size_t o = 0;
-      std::memcpy(host.data() + o, gi.data(), gn * sizeof(GroupInner)); o += gn * sizeof(GroupInner);
-      std::memcpy(host.data() + o, gr.data(), gn * sizeof(GroupReduce)); o += gn * sizeof(GroupReduce);
-      std::memcpy(host.data() + o, gd.data(), gn * sizeof(GroupDmul)); o += gn * sizeof(GroupDmul);
-      std::memcpy(host.data() + o, gc.data(), gn * sizeof(GroupCombine));
-      GW_CHECK_HIP(hipMemcpy(b->gtab, host.data(), bytes, hipMemcpyHostToDevice));
-    }
-    b->grouped = true;
-    b->gn = gn;
-    b->gt_inner = ti; b->gt_reduce = tr; b->gt_dmul = td; b->gt_combine = tc;
   }
-  b->prec = prec;
-  b->terms = terms;
-  b->rmax = 1;
-  for (gw_plan* pl : b->plans) b->rmax = std::max(b->rmax, plan_rmax(&pl->p));
-  b->tot13 = w13;
-  b->tot2 = w2;
-  b->lds13 = lds13;
-  b->lds2 = lds2;
+  const int gn = (int)gi.size();
+  const size_t gtab_off = align256((size_t)n * sizeof(RagInst<T>));
+  const size_t bytes =
+      gtab_off + (size_t)gn * (sizeof(GroupInner) + sizeof(GroupReduce) + sizeof(GroupDmul) + sizeof(GroupCombine));
+  GW_REQUIRE(s.dev && s.host && bytes <= s.cap, GW_ERR_HIP, "batch tables exceed their allocation (%zu > %zu)",
+             bytes, s.cap);
+  // the staging buffer may still feed an earlier upload of this slot
+  if (s.pending) GW_CHECK_HIP(hipEventSynchronize(s.done));
+  unsigned char* h = s.host;
+  std::memcpy(h, R.data(), (size_t)n * sizeof(RagInst<T>));
+  size_t o = gtab_off;
+  std::memcpy(h + o, gi.data(), gn * sizeof(GroupInner)); o += gn * sizeof(GroupInner);
+  std::memcpy(h + o, gr.data(), gn * sizeof(GroupReduce)); o += gn * sizeof(GroupReduce);
+  std::memcpy(h + o, gd.data(), gn * sizeof(GroupDmul)); o += gn * sizeof(GroupDmul);
+  std::memcpy(h + o, gc.data(), gn * sizeof(GroupCombine));
+  GW_CHECK_HIP(hipMemcpyAsync(s.dev, h, bytes, hipMemcpyHostToDevice, st));
+  s.bound = std::move(bound);
+  s.grouped = grouped;
+  s.gn = gn;
+  s.gtab_off = gtab_off;
+  s.gt_inner = ti; s.gt_reduce = tr; s.gt_dmul = td; s.gt_combine = tc;
+  s.prec = prec;
+  s.terms = terms;
+  s.rmax = 1;
+  for (gw_plan* pl : b->plans) s.rmax = std::max(s.rmax, plan_rmax(&pl->p));
+  s.tot13 = w13;
+  s.tot2 = w2;
+  s.lds13 = lds13;
+  s.lds2 = lds2;
   return GW_OK;
 }
 
 template <typename T>
-static int batch_apply_t(gw_batch* b, hipStream_t st) {
+static int batch_apply_t(gw_batch* b, const BatchBind& s, hipStream_t st) {
   const int n = (int)b->plans.size();
-  const RagInst<T>* R = (const RagInst<T>*)b->table;
-  if (b->tot13 == 0) return GW_OK;
-  const dim3 g13((unsigned)(8 * ((b->tot13 + 7) / 8))), g2((unsigned)(8 * ((b->tot2 + 7) / 8)));
-  if (b->rmax <= 10) {
-    hipLaunchKernelGGL((k_plane_inv_rag<T, 10>), g13, dim3(kPlaneThreads), b->lds13, st, R, n, b->tot13);
-    hipLaunchKernelGGL((k_xpass_rag<T, 10>), g2, dim3(kXThreads), b->lds2, st, R, n, b->tot2);
-    hipLaunchKernelGGL((k_plane_fwd_rag<T, 10>), g13, dim3(kPlaneThreads), b->lds13, st, R, n, b->tot13);
+  const RagInst<T>* R = (const RagInst<T>*)s.dev;
+  if (s.tot13 == 0) return GW_OK;
+  const dim3 g13((unsigned)(8 * ((s.tot13 + 7) / 8))), g2((unsigned)(8 * ((s.tot2 + 7) / 8)));
+  if (s.rmax <= 10) {
+    hipLaunchKernelGGL((k_plane_inv_rag<T, 10>), g13, dim3(kPlaneThreads), s.lds13, st, R, n, s.tot13);
+    hipLaunchKernelGGL((k_xpass_rag<T, 10>), g2, dim3(kXThreads), s.lds2, st, R, n, s.tot2);
+    hipLaunchKernelGGL((k_plane_fwd_rag<T, 10>), g13, dim3(kPlaneThreads), s.lds13, st, R, n, s.tot13);
   } else {
-    hipLaunchKernelGGL(k_plane_inv_rag<T>, g13, dim3(kPlaneThreads), b->lds13, st, R, n, b->tot13);
-    hipLaunchKernelGGL(k_xpass_rag<T>, g2, dim3(kXThreads), b->lds2, st, R, n, b->tot2);
-    hipLaunchKernelGGL(k_plane_fwd_rag<T>, g13, dim3(kPlaneThreads), b->lds13, st, R, n, b->tot13);
+    hipLaunchKernelGGL(k_plane_inv_rag<T>, g13, dim3(kPlaneThreads), s.lds13, st, R, n, s.tot13);
+    hipLaunchKernelGGL(k_xpass_rag<T>, g2, dim3(kXThreads), s.lds2, st, R, n, s.tot2);
+    hipLaunchKernelGGL(k_plane_fwd_rag<T>, g13, dim3(kPlaneThreads), s.lds13, st, R, n, s.tot13);
   }
   GW_CHECK_HIP(hipGetLastError());
-  if (!(b->terms & GW_TERM_NONLOCAL)) return GW_OK;
+  if (!(s.terms & GW_TERM_NONLOCAL)) return GW_OK;
   const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
-  if (b->grouped) {  // one launch per contraction over every cell
-    if (b->gn == 0) return GW_OK;
-    const int gn = b->gn;
-    const GroupInner* gi = (const GroupInner*)b->gtab;
+  if (s.grouped) {  // one launch per contraction over every cell
+    if (s.gn == 0) return GW_OK;
+    const int gn = s.gn;
+    const GroupInner* gi = (const GroupInner*)(s.dev + s.gtab_off);
     const GroupReduce* gr = (const GroupReduce*)(gi + gn);
     const GroupDmul* gd = (const GroupDmul*)(gr + gn);
     const GroupCombine* gc = (const GroupCombine*)(gd + gn);
-    int rc = gemm_group_inner<T>(gi, gn, b->gt_inner, st);
-    if (!rc) rc = gemm_group_reduce<T>(gr, gn, b->gt_reduce, st);
+    int rc = gemm_group_inner<T>(gi, gn, s.gt_inner, st);
+    if (!rc) rc = gemm_group_reduce<T>(gr, gn, s.gt_reduce, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_group_dmul<T>, dim3((unsigned)b->gt_dmul), dim3(256), 0, st, gd, gn, b->gt_dmul);
+    hipLaunchKernelGGL(k_group_dmul<T>, dim3((unsigned)s.gt_dmul), dim3(256), 0, st, gd, gn, s.gt_dmul);
     GW_CHECK_HIP(hipGetLastError());
-    return gemm_group_combine<T>(gc, gn, b->gt_combine, T(1), st);
+    return gemm_group_combine<T>(gc, gn, s.gt_combine, T(1), st);
   }
   GW_CHECK_HIP(hipEventRecord(b->ev_fork, st));
   for (int j = 0; j < kBatchLanes - 1; ++j) GW_CHECK_HIP(hipStreamWaitEvent(b->side[j], b->ev_fork, 0));
@@ -2405,7 +2441,7 @@ static int batch_apply_t(gw_batch* b, hipStream_t st) {
   int lane = 0;
   for (int i = 0; i < n; ++i) {
     Plan& P = b->plans[i]->p;
-    const gw_batch::Bound& B = b->bound[i];
+    const BatchBound& B = s.bound[i];
     if (!P.nproj || B.nb == 0) continue;
     // instance i's chain on one lane; the chains write disjoint workspaces and outputs
     const hipStream_t ls = lane == 0 ? st : b->side[lane - 1];
@@ -2433,31 +2469,73 @@ static int batch_apply_t(gw_batch* b, hipStream_t st) {
   return GW_OK;
 }
 
-extern "C" int gw_batch_bind(gw_batch* b, int precision, int terms, const void* const* c, const long* ldc,
-                             const long* sc, void* const* out, const long* ldo, const long* so, const int* nb,
-                             void* ws, size_t ws_bytes) {
+static int batch_bind(gw_batch* b, BatchBind& s, int precision, int terms, const void* const* c, const long* ldc,
+                      const long* sc, void* const* out, const long* ldo, const long* so, const int* nb, void* ws,
+                      size_t ws_bytes, hipStream_t st) {
   GW_REQUIRE(b && c && ldc && sc && out && ldo && so && nb && ws, GW_ERR_ARG, "bad argument");
   GW_REQUIRE(precision == GW_FP64 || precision == GW_FP32, GW_ERR_ARG, "bad precision");
   GW_REQUIRE(terms & GW_TERM_LOCAL, GW_ERR_UNSUPPORTED, "batched apply needs the local term");
   const size_t need = batch_ws_bytes(b, nb, precision);
   GW_REQUIRE(ws_bytes >= need, GW_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, need);
   if (precision == GW_FP64)
-    return batch_bind_t<double>(b, terms, c, ldc, sc, out, ldo, so, nb, (unsigned char*)ws);
-  return batch_bind_t<float>(b, terms, c, ldc, sc, out, ldo, so, nb, (unsigned char*)ws);
+    return batch_bind_t<double>(b, s, terms, c, ldc, sc, out, ldo, so, nb, (unsigned char*)ws, st);
+  return batch_bind_t<float>(b, s, terms, c, ldc, sc, out, ldo, so, nb, (unsigned char*)ws, st);
 }
 
-extern "C" int gw_batch_apply_bound(gw_batch* b, void* stream) {
-  GW_REQUIRE(b && b->prec >= 0, GW_ERR_ARG, "batch not bound");
-  return b->prec == GW_FP64 ? batch_apply_t<double>(b, (hipStream_t)stream)
-                            : batch_apply_t<float>(b, (hipStream_t)stream);
+static int batch_launch(gw_batch* b, const BatchBind& s, hipStream_t st) {
+  GW_REQUIRE(s.prec >= 0, GW_ERR_ARG, "batch not bound");
+  return s.prec == GW_FP64 ? batch_apply_t<double>(b, s, st) : batch_apply_t<float>(b, s, st);
+}
+
+extern "C" int gw_batch_bind(gw_batch* b, int precision, int terms, const void* const* c, const long* ldc,
+                             const long* sc, void* const* out, const long* ldo, const long* so, const int* nb,
+                             void* ws, size_t ws_bytes, void* stream, gw_batch_binding** binding) {
+  GW_REQUIRE(b && binding, GW_ERR_ARG, "bad argument");
+  auto* bd = new gw_batch_binding();
+  bd->b = b;
+  if (!bind_alloc(bd->s, (int)b->plans.size())) {
+    bind_free(bd->s);
+    delete bd;
+    GW_REQUIRE(false, GW_ERR_HIP, "allocating the binding's tables failed");
+  }
+  const int rc = batch_bind(b, bd->s, precision, terms, c, ldc, sc, out, ldo, so, nb, ws, ws_bytes,
+                            (hipStream_t)stream);
+  if (rc) {
+    bind_free(bd->s);
+    delete bd;
+    return rc;
+  }
+  *binding = bd;
+  return GW_OK;
+}
+
+extern "C" int gw_batch_apply_bound(gw_batch_binding* bd, void* stream) {
+  GW_REQUIRE(bd && bd->b, GW_ERR_ARG, "null binding");
+  return batch_launch(bd->b, bd->s, (hipStream_t)stream);
+}
+
+extern "C" int gw_batch_binding_destroy(gw_batch_binding* bd) {
+  if (!bd) return GW_OK;
+  bind_free(bd->s);
+  delete bd;
+  return GW_OK;
 }
 
 extern "C" int gw_batch_apply(gw_batch* b, int precision, int terms, const void* const* c, const long* ldc,
                               const long* sc, void* const* out, const long* ldo, const long* so, const int* nb,
                               void* ws, size_t ws_bytes, void* stream) {
-  const int rc = gw_batch_bind(b, precision, terms, c, ldc, sc, out, ldo, so, nb, ws, ws_bytes);
+  GW_REQUIRE(b, GW_ERR_ARG, "null batch");
+  hipStream_t st = (hipStream_t)stream;
+  BatchBind& s = b->ring[b->next];
+  b->next = (b->next + 1) % kBatchRing;
+  // this slot's device tables may still be read by a launch on another stream
+  if (s.pending) GW_CHECK_HIP(hipStreamWaitEvent(st, s.done, 0));
+  int rc = batch_bind(b, s, precision, terms, c, ldc, sc, out, ldo, so, nb, ws, ws_bytes, st);
+  if (!rc) rc = batch_launch(b, s, st);
   if (rc) return rc;
-  return gw_batch_apply_bound(b, stream);
+  GW_CHECK_HIP(hipEventRecord(s.done, st));
+  s.pending = true;
+  return GW_OK;
 }
 
 extern "C" size_t gw_density_workspace_bytes(const gw_plan* h, int nb, int precision) {

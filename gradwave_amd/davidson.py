@@ -94,6 +94,15 @@ def _hermitize(s: torch.Tensor) -> torch.Tensor:
     return (0.5 * (s + s.conj().transpose(-1, -2))).to(torch.complex128)
 
 
+def _gram(v: torch.Tensor) -> torch.Tensor:
+    """G[k][i][j] = sum_g v_i conj(v_j) in complex128, Hermitised. A complex64
+    block is upcast before the contraction, as davidson.py:213-216 does, so the
+    products of its FP32 values are exact and the sums FP64."""
+    w = v if v.dtype == torch.complex128 else v.to(torch.complex128)
+    g = LA.inner(w, w).conj()
+    return 0.5 * (g + g.mH)
+
+
 def _cholqr2_dev(v: torch.Tensor):
     """Orthonormal rows spanning v (nk, n, npw) by two rounds of fp64 Gram ->
     Cholesky -> triangular inverse -> combination (davidson.py:195-222 in row form:
@@ -102,9 +111,7 @@ def _cholqr2_dev(v: torch.Tensor):
     eye = torch.eye(n, dtype=torch.complex128, device=v.device)
     infos = []
     for _ in range(2):
-        g = LA.inner(v, v).to(torch.complex128).conj()  # G[i][j] = sum v_i conj(v_j)
-        g = 0.5 * (g + g.mH)
-        lo, info = torch.linalg.cholesky_ex(g)
+        lo, info = torch.linalg.cholesky_ex(_gram(v))
         infos.append(info.max())
         linv = torch.linalg.solve_triangular(lo, eye.expand_as(lo), upper=False)
         v = LA.combine(linv.transpose(-1, -2).to(v.dtype).contiguous(), v)
@@ -112,12 +119,13 @@ def _cholqr2_dev(v: torch.Tensor):
 
 
 def _project(v: torch.Tensor, against: torch.Tensor | None) -> torch.Tensor:
-    """Two passes of v -= (v a^H) a (davidson.py:428-432)."""
+    """Two passes of v -= (v a^H) a (davidson.py:428-432), in place: callers pass
+    a block of their own (the masked copy made by the orthonormalisation)."""
     if against is None or against.shape[1] == 0:
         return v
     for _ in range(2):
         coef = LA.inner(against, v)  # coef[j][i] = sum conj(a_j) v_i = (v a^H)[i][j]
-        v = LA.combine((-coef).contiguous(), against, out=v.clone(), beta=1.0)
+        LA.combine((-coef).contiguous(), against, out=v, beta=1.0)
     return v
 
 
@@ -266,7 +274,11 @@ def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: f
 
     cert_floor = _FP32_CERT_FLOOR_EPS * max(float(t.max()), 1.0) if use_fp32_exp else 0.0
     prev_rnmax = float("inf")
-    basis = _Basis(nk, max(max_dim, nb), m, x0.dtype, device)
+    # a restart keeps nb rows and appends up to nb more, so the buffers hold
+    # max(max_dim, 2 nb) rows: max_dim < 2 nb is legal (max_dim_factor=1, or
+    # a k-point with fewer than 2 nb plane waves), and the reference's
+    # concatenation grows past max_dim there too (davidson.py:757-759)
+    basis = _Basis(nk, max(max_dim, 2 * nb), m, x0.dtype, device)
 
     v0 = yield from _orthonormalize_steps(x0.contiguous(), mask)
     hv0 = yield from _apply_steps(v0, use_fp32_exp, counts)
@@ -307,9 +319,7 @@ def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: f
         tm("restart")
         if basis.n + n_add > max_dim:
             # restart on the Ritz block with its drift repaired: x = L q, hx -> L^-1 hx
-            g = LA.inner(x, x).to(torch.complex128).conj()
-            g = 0.5 * (g + g.mH)
-            lo, info = torch.linalg.cholesky_ex(g)
+            lo, info = torch.linalg.cholesky_ex(_gram(x))
             if int(info.max()) == 0:
                 eye = torch.eye(nb, dtype=torch.complex128, device=device).expand_as(lo)
                 linv = torch.linalg.solve_triangular(lo, eye, upper=False)

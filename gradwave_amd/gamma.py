@@ -276,7 +276,9 @@ def _orthonormalize(v: torch.Tensor, against: torch.Tensor | None = None) -> tor
         vh = v / nrm[:, None].to(v.dtype)
         q = vh
         for it in range(2):
-            g = LA.rinner(q, q).double()
+            # FP32 rows are upcast first: exact products, FP64 sums (davidson.py:213-216)
+            q64 = q if q.dtype == torch.float64 else q.double()
+            g = LA.rinner(q64, q64)
             lo, info = torch.linalg.cholesky_ex(0.5 * (g + g.T))
             d = lo.diagonal()
             if int(info) != 0 or (it == 0 and (bool((d * nrm <= 1e-10).any()) or float(d.min()) < 1e-6)):

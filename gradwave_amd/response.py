@@ -1,143 +1,176 @@
-"""Conduction-projected Sternheimer solves on MI355X (postscf/_response.py:502-681).
+"""Conduction-projected Sternheimer solves on MI355X.
 
-``cg_sternheimer(h, bk, c_occ, eps_occ, rhs, x0, shift)`` solves, for every
-occupied band n of every k at once,
+Contract (postscf/_response.py:502-595, norm-conserving branch): for every
+occupied band n of every k, solve
 
     (H - eps_n + shift * P_occ) dpsi = rhs,    P_occ = sum_m |c_m><c_m|,
 
-with the Teter-preconditioned conjugate gradient of the reference and returns
-(1 - P_occ) dpsi. As in the reference, when the operator has ``apply_cols``
-the iteration is per (k, band) column: full-block steps while more than a
-quarter of the columns are active, then the surviving columns are gathered
-into one compacted batch, and up to three true-residual rounds certify every
-column to ``tol``. The H applies are the HIP kernels; the occupied-space
-projections of the full block are MFMA contractions (``linalg.inner`` /
-``combine``). Norm-conserving operators only (the S-metric branch of the
-reference is not implemented here).
+each (k, n) column to a residual norm below ``tol``, and return
+(1 - P_occ) dpsi; a warm ``x0`` is certified, not trusted; padded plane-wave
+slots of ``x0`` are zeroed; ``rhs`` already lies in the conduction space.
+
+Design. The solver is an active-set column CG whose whole state stays on the
+device in the padded (nk, nocc, npw_max) layout, one row per column:
+
+* every step works on the list of still-active columns only: their search
+  directions are gathered into one compact block by the direction kernel and
+  applied with ``h.apply_cols`` (the operator's compacted column apply, which
+  groups them by k into one HIP apply); operators without ``apply_cols`` get
+  the full block;
+* the occupied-space projections of the step, P_occ (H - eps) p and P_occ p,
+  are MFMA contractions (``linalg.inner`` / ``combine``) over the full layout:
+  P_occ acts column by column, so inactive rows cost a few GEMM flops and
+  change nothing;
+* x += a p, r -= a Ap, the Teter-filtered z = K r, ||r||^2 and Re<r, z> are one
+  fused kernel over the active columns (``gw_cg_update``), p = z + b p another
+  (``gw_cg_direction``), Re<p, Ap> a third (``gw_row_dots``); all sums fp64;
+* a column leaves the active list as soon as its recurrence residual is below
+  ``tol``; when the list is empty the true residual rhs - A x of every column is
+  measured with one block apply, and the columns that drifted above ``tol``
+  start a fresh CG (at most ``_CERTIFY_ROUNDS`` measurements); ``max_iter``
+  bounds the CG steps over all rounds.
+
+The S-metric (ultrasoft/PAW) branch of the reference is not provided.
 """
 
 from __future__ import annotations
 
 import torch
 
+from gradwave_amd import _lib as L
 from gradwave_amd import linalg as LA
 
-
-def _teter_b(r: torch.Tensor, t: torch.Tensor, t_band: torch.Tensor) -> torch.Tensor:
-    """solvers/precond.py:20-39: K(t_G / <T>) r with the coefficient in float64."""
-    x = t.double()[:, None, :] / torch.clamp(t_band.double()[..., None], min=1e-12)
-    x2 = x * x
-    num = 27.0 + 18.0 * x + 12.0 * x2 + 8.0 * x2 * x
-    return r * (num / (num + 16.0 * x2 * x2)).to(r.real.dtype)
+_CERTIFY_ROUNDS = 4
 
 
-def _dot(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
-    """Re sum_g conj(a) b over the last axis."""
-    return (a.conj() * b).real.sum(-1)
+def _i64(t: torch.Tensor) -> torch.Tensor:
+    return t.to(torch.int64).contiguous()
+
+
+class _ColumnCG:
+    """Device state and kernels of one batched Sternheimer solve."""
+
+    def __init__(self, h, t, c_occ, eps_occ, rhs, shift):
+        self.h, self.c_occ, self.rhs, self.shift = h, c_occ.contiguous(), rhs.contiguous(), float(shift)
+        self.nk, self.nocc, self.m = rhs.shape
+        self.nc = self.nk * self.nocc
+        self.dev = rhs.device
+        self.prec = L.precision_of(rhs.dtype)
+        self.t = t.to(torch.float64).contiguous()
+        self.eps = eps_occ.to(rhs.real.dtype).contiguous()
+        self.kcol_all = torch.arange(self.nk, device=self.dev).repeat_interleave(self.nocc)
+        self.z = torch.zeros_like(self.rhs)
+        self.p = torch.zeros_like(self.rhs)
+        self.w = torch.zeros_like(self.rhs)  # A p, full layout (valid at active rows)
+        rows = torch.arange(self.nc, device=self.dev)
+        # <c_n|T|c_n>, the Teter scale of column (k, n) (postscf/_response.py:559-562)
+        tb = torch.empty(self.nc, dtype=torch.float64, device=self.dev)
+        self._dots(rows, self.c_occ, self.c_occ, tb, weight=self.t, wrow=self.kcol_all)
+        self.t_band = tb.clamp_(min=1e-6)
+
+    def _stream(self):
+        return L.stream_ptr(self.dev)
+
+    def _dots(self, rows, a, b, out, weight=None, wrow=None):
+        r = _i64(rows)
+        L.check(L.lib().gw_row_dots(self.prec, int(r.numel()), self.m, r.data_ptr(), a.data_ptr(), b.data_ptr(),
+                                    self.m, weight.data_ptr() if weight is not None else None,
+                                    weight.stride(0) if weight is not None else 0,
+                                    _i64(wrow).data_ptr() if wrow is not None else None, out.data_ptr(),
+                                    self._stream()), "gw_row_dots")
+        return out
+
+    def project_occ(self, y: torch.Tensor, sign: float = -1.0, out: torch.Tensor | None = None) -> torch.Tensor:
+        """out + sign * P_occ y on the full layout (out = y, in place, when None)."""
+        ov = LA.inner(self.c_occ, y)  # ov[n][b] = <c_n|y_b>
+        target = y if out is None else out
+        LA.combine((sign * ov).contiguous(), self.c_occ, out=target, beta=1.0)
+        return target
+
+    def update(self, x, r, rows, alpha, ap):
+        n = int(rows.numel())
+        rn2 = torch.empty(n, dtype=torch.float64, device=self.dev)
+        rz = torch.empty(n, dtype=torch.float64, device=self.dev)
+        L.check(L.lib().gw_cg_update(self.prec, n, self.m, rows.data_ptr(), self.kcol_all[rows].contiguous().data_ptr(),
+                                     alpha.data_ptr(), x.data_ptr(), self.p.data_ptr(), r.data_ptr(), ap.data_ptr(),
+                                     self.m, self.t.data_ptr(), self.t.stride(0), self.t_band.data_ptr(),
+                                     self.z.data_ptr(), rn2.data_ptr(), rz.data_ptr(), self._stream()),
+                "gw_cg_update")
+        return rn2, rz
+
+    def direction(self, rows, beta, compact: bool):
+        n = int(rows.numel())
+        pc = torch.empty(n, self.m, dtype=self.rhs.dtype, device=self.dev) if compact else None
+        L.check(L.lib().gw_cg_direction(self.prec, n, self.m, rows.data_ptr(), beta.data_ptr(), self.z.data_ptr(),
+                                        self.p.data_ptr(), self.m, pc.data_ptr() if pc is not None else None,
+                                        self.m, self._stream()), "gw_cg_direction")
+        return pc
+
+    def apply_a(self, rows, pc):
+        """A p = (1 - P_occ)(H - eps) p + shift P_occ p into self.w at the listed rows."""
+        flat_w = self.w.view(self.nc, self.m)
+        if pc is not None:
+            kc = self.kcol_all[rows]
+            hp = self.h.apply_cols(pc, kc)
+            hp -= self.eps.view(-1)[rows][:, None] * pc
+            flat_w.index_copy_(0, rows, hp)
+        else:
+            self.w.copy_(self.h.apply(self.p))
+            self.w -= self.eps[..., None] * self.p
+        self.project_occ(self.w, -1.0)
+        if self.shift != 0.0:
+            self.project_occ(self.p, self.shift, out=self.w)
+        return self.w
+
+    def true_residual(self, x):
+        """rhs - A x for every column (one block apply)."""
+        hx = self.h.apply(x) - self.eps[..., None] * x
+        self.project_occ(hx, -1.0)
+        if self.shift != 0.0:
+            self.project_occ(x, self.shift, out=hx)
+        return self.rhs - hx
 
 
 @torch.no_grad()
 def cg_sternheimer(h, bk, c_occ: torch.Tensor, eps_occ: torch.Tensor, rhs: torch.Tensor, x0: torch.Tensor,
                    shift: float, tol: float = 1e-8, max_iter: int = 400) -> torch.Tensor:
-    """Batched conduction-projected Sternheimer solve (postscf/_response.py:502-595);
-    c_occ, rhs, x0: (nk, nocc, npw_max) complex, eps_occ (nk, nocc)."""
+    """Batched conduction-projected Sternheimer solve (postscf/_response.py:502-595).
 
-    def p_occ(x: torch.Tensor) -> torch.Tensor:
-        ov = LA.inner(c_occ, x)  # (nk, nocc, nb): ov[n][b] = <c_n|x_b>
-        return LA.combine(ov, c_occ)  # sum_n ov[n][b] c_n
-
-    def a_apply(x: torch.Tensor) -> torch.Tensor:
-        hx = h.apply(x) - eps_occ[..., None] * x
-        return hx - p_occ(hx) + shift * p_occ(x)
-
+    h: operator with ``apply`` (and optionally ``apply_cols``); bk: object with
+    the kinetic table ``t`` (nk, npw_max) and optionally ``mask``;
+    c_occ, rhs, x0: (nk, nocc, npw_max) complex on the GPU; eps_occ (nk, nocc).
+    Returns (1 - P_occ) dpsi."""
+    L.require_gpu(rhs, "rhs")
+    nk, nocc, m = rhs.shape
+    x = x0.to(rhs.dtype).clone().contiguous()
     mask = getattr(bk, "mask", None)
     if mask is not None:
-        x0 = x0 * mask[:, None, :].to(x0.dtype)
-    t_band = torch.clamp((c_occ.conj() * bk.t.to(c_occ.dtype)[:, None, :] * c_occ).real.sum(-1), min=1e-6)
-    if hasattr(h, "apply_cols"):
-        return _cg_percolumn(h, bk, c_occ, eps_occ, rhs, x0, shift, t_band, tol, max_iter, p_occ, a_apply)
-
-    x = x0.clone()
-    r = rhs - a_apply(x)
-    z = _teter_b(r, bk.t, t_band)
-    p = z
-    rz = _dot(r, z)
-    for _ in range(max_iter):
-        ap = a_apply(p)
-        a_cg = rz / torch.clamp(_dot(p, ap), min=1e-300)
-        x = x + a_cg[..., None] * p
-        r = r - a_cg[..., None] * ap
-        if float(torch.linalg.norm(r, dim=-1).max()) < tol:
-            break
-        z = _teter_b(r, bk.t, t_band)
-        rz_new = _dot(r, z)
-        p = z + (rz_new / torch.clamp(rz, min=1e-300))[..., None] * p
-        rz = rz_new
-    return x - p_occ(x)
-
-
-def _cg_percolumn(h, bk, c_occ, eps_occ, rhs, x0, shift, t_band, tol, max_iter, p_occ, a_apply):
-    """postscf/_response.py:598-681: lockstep while > 1/4 of the columns are
-    active, then a compacted column batch through ``h.apply_cols``; up to three
-    true-residual rounds."""
-    nk, nb, _ = rhs.shape
-    total = nk * nb
-    x = x0.clone()
+        x *= mask[:, None, :].to(x.dtype)
+    if nk * nocc == 0:
+        return x
+    cg = _ColumnCG(h, bk.t, c_occ, eps_occ, rhs, shift)
+    compact = hasattr(h, "apply_cols")
     used = 0
-    for _round in range(3):
-        r = rhs - a_apply(x)
-        rn = torch.linalg.norm(r, dim=-1)
-        if used >= max_iter or float(rn.max()) < tol:
+    for _ in range(_CERTIFY_ROUNDS):
+        r = cg.true_residual(x).contiguous()
+        rn = LA.row_norms(r.view(1, cg.nc, m))[0]
+        rows = _i64(torch.nonzero(rn >= tol).flatten())
+        if rows.numel() == 0 or used >= max_iter:
             break
-        z = _teter_b(r, bk.t, t_band)
-        p = z
-        rz = _dot(r, z)
-        while used < max_iter and int((rn >= tol).sum()) > total // 4:
-            ap = a_apply(p)
-            a_cg = rz / torch.clamp(_dot(p, ap), min=1e-300)
-            x = x + a_cg[..., None] * p
-            r = r - a_cg[..., None] * ap
+        zero = torch.zeros(rows.numel(), dtype=torch.float64, device=rhs.device)
+        _, rz = cg.update(x, r, rows, zero, r)  # a = 0: z = K r and Re<r, z> only
+        cg.p.view(cg.nc, m).index_fill_(0, rows, 0)
+        pc = cg.direction(rows, zero, compact)  # p = z
+        while rows.numel() and used < max_iter:
+            ap = cg.apply_a(rows, pc)
+            pap = cg._dots(rows, cg.p, ap, torch.empty(rows.numel(), dtype=torch.float64, device=rhs.device))
+            alpha = rz / pap.clamp(min=1e-300)
+            rn2, rz_new = cg.update(x, r, rows, alpha, ap)
             used += 1
-            rn = torch.linalg.norm(r, dim=-1)
-            if float(rn.max()) < tol:
+            keep = (rn2 >= tol * tol).nonzero().flatten()
+            if keep.numel() == 0:
                 break
-            z = _teter_b(r, bk.t, t_band)
-            rz_new = _dot(r, z)
-            p = z + (rz_new / torch.clamp(rz, min=1e-300))[..., None] * p
-            rz = rz_new
-        if used >= max_iter or float(rn.max()) < tol:
-            continue
-        kcol, bcol = (rn >= tol).nonzero(as_tuple=True)
-        xa, ra, pa = x[kcol, bcol], r[kcol, bcol], p[kcol, bcol]
-        rza = rz[kcol, bcol]
-        c_sel = c_occ[kcol]
-        eps_col = eps_occ[kcol, bcol]
-        t_sel = bk.t[kcol]
-        tb_sel = t_band[kcol, bcol]
-        while used < max_iter and kcol.numel():
-            hv = h.apply_cols(pa, kcol) - eps_col[:, None] * pa
-            ov = torch.einsum("cng,cg->cn", c_sel.conj(), hv)
-            hv = hv - torch.einsum("cn,cng->cg", ov, c_sel)
-            ov2 = torch.einsum("cng,cg->cn", c_sel.conj(), pa)
-            ap_a = hv + shift * torch.einsum("cn,cng->cg", ov2, c_sel)
-            al = rza / torch.clamp(_dot(pa, ap_a), min=1e-300)
-            xa = xa + al[:, None] * pa
-            ra = ra - al[:, None] * ap_a
-            used += 1
-            done = torch.linalg.norm(ra, dim=-1) < tol
-            if bool(done.any()):
-                x[kcol[done], bcol[done]] = xa[done]
-                keep = ~done
-                kcol, bcol = kcol[keep], bcol[keep]
-                xa, ra, pa, rza = xa[keep], ra[keep], pa[keep], rza[keep]
-                c_sel, eps_col = c_sel[keep], eps_col[keep]
-                t_sel, tb_sel = t_sel[keep], tb_sel[keep]
-                if not kcol.numel():
-                    break
-            za = _teter_b(ra[:, None, :], t_sel, tb_sel[:, None])[:, 0]
-            rz_new_a = _dot(ra, za)
-            pa = za + (rz_new_a / torch.clamp(rza, min=1e-300))[:, None] * pa
-            rza = rz_new_a
-        if kcol.numel():
-            x[kcol, bcol] = xa
-    return x - p_occ(x)
+            beta = rz_new[keep] / rz[keep].clamp(min=1e-300)
+            rows, rz = rows[keep].contiguous(), rz_new[keep]
+            pc = cg.direction(rows, beta.contiguous(), compact)
+    return cg.project_occ(x, -1.0)

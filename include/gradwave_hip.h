@@ -148,6 +148,31 @@ int gw_teter_expand(int precision, int nk, int nb, int nsel, int npw, const void
 int gw_row_norms(int precision, int nk, int m, int npw, const void* v, long ld, long s,
                  double* out, void* stream);
 
+/* ---- Sternheimer conjugate-gradient column kernels ----
+ * Replace the per-column updates of cg_sternheimer / _cg_percolumn
+ * (postscf/_response.py:502-690). The CG vectors x, p, r, z, ap live in the
+ * padded layout: column c = k * nocc + n is the row c of a (nk * nocc, ld)
+ * complex array. Each call works on the nrows columns listed in rows (int64
+ * column indices), one workgroup per listed column; sums are fp64.
+ *
+ * gw_cg_update: for listed column j (c = rows[j], k = kcol[j], a = alpha[j]):
+ *   x_c += a p_c, r_c -= a ap_c (both skipped when a == 0),
+ *   z_c = K(t[k] / t_band[c]) r_c with the Teter filter K of solvers/precond.py:20-39,
+ *   rn2[j] = ||r_c||^2, rz[j] = Re <r_c, z_c>.
+ *   t: (nk, ldt) float64 kinetic diagonal; t_band: (nk * nocc) float64.
+ * gw_cg_direction: p_c = z_c + beta[j] p_c; when pc is not null the new p_c is
+ *   also written to the compact row j of pc (row stride ldc) for a column apply.
+ * gw_row_dots: out[j] = Re sum_g conj(a_c[g]) w[wrow[j]][g] b_c[g] with c = rows[j]
+ *   (c = j when rows is null, w = 1 when null, w row 0 when wrow is null). */
+int gw_cg_update(int precision, int nrows, int npw, const int64_t* rows, const int64_t* kcol,
+                 const double* alpha, void* x, const void* p, void* r, const void* ap, long ld,
+                 const double* t, long ldt, const double* t_band, void* z, double* rn2, double* rz,
+                 void* stream);
+int gw_cg_direction(int precision, int nrows, int npw, const int64_t* rows, const double* beta,
+                    const void* z, void* p, long ld, void* pc, long ldc, void* stream);
+int gw_row_dots(int precision, int nrows, int npw, const int64_t* rows, const void* a, const void* b,
+                long ld, const double* w, long ldw, const int64_t* wrow, double* out, void* stream);
+
 /* ---- adjoints (vector-Jacobian products) of H|psi> ----
  * The reference differentiates HamiltonianK.apply (core/hamiltonian.py:179-191),
  * projectors (108-112) and local_potential_g (core/energies/local_pp.py:22-38)
@@ -229,19 +254,29 @@ int gw_tune_get(int key, int arg);
  * on the launch stream and three side streams of the batch (default priority;
  * forked from and joined back into the launch stream by events, so a graph
  * capture of apply_bound records them too). Calls through one batch must not
- * overlap. Plans must take the generic-size plane kernels
- * (x-plane fits LDS). bind uploads the per-instance block pointers, strides and
- * band counts (nb[i] = 0 leaves instance i out of the call) and is synchronous;
- * apply_bound only launches (so it can be captured in a HIP graph); apply = both.
+ * overlap. Plans must take the generic-size plane kernels (x-plane fits LDS).
+ * Each call's descriptor tables (per-instance block pointers, strides and band
+ * counts; nb[i] = 0 leaves instance i out of the call) reach the device by an
+ * asynchronous copy on the launch stream from pinned staging, so they are
+ * ordered after every earlier launch on that stream.
+ * gw_batch_apply: eager call; its tables go to one of a few slots owned by the
+ *   batch (a slot is reused only after the launch that read it has finished).
+ * gw_batch_bind: creates a binding with device tables of its own, uploaded on
+ *   `stream`; gw_batch_apply_bound(binding, stream) only launches, so it can be
+ *   recorded in a HIP graph, and the graph stays valid while the binding, its
+ *   blocks and its workspace live, whatever later calls through the batch do.
+ *   The batch must outlive its bindings.
  * Terms must include GW_TERM_LOCAL. Workspace: gw_batch_workspace_bytes. */
 typedef struct gw_batch gw_batch;
+typedef struct gw_batch_binding gw_batch_binding;
 int gw_batch_create(gw_batch** out, gw_plan* const* plans, int n);
 int gw_batch_destroy(gw_batch* batch);
 size_t gw_batch_workspace_bytes(const gw_batch* batch, const int* nb, int precision);
 int gw_batch_bind(gw_batch* batch, int precision, int terms, const void* const* c, const long* ldc,
                   const long* sc, void* const* out, const long* ldo, const long* so, const int* nb,
-                  void* workspace, size_t workspace_bytes);
-int gw_batch_apply_bound(gw_batch* batch, void* stream);
+                  void* workspace, size_t workspace_bytes, void* stream, gw_batch_binding** binding);
+int gw_batch_apply_bound(gw_batch_binding* binding, void* stream);
+int gw_batch_binding_destroy(gw_batch_binding* binding);
 int gw_batch_apply(gw_batch* batch, int precision, int terms, const void* const* c, const long* ldc,
                    const long* sc, void* const* out, const long* ldo, const long* so, const int* nb,
                    void* workspace, size_t workspace_bytes, void* stream);

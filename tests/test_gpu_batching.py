@@ -83,26 +83,34 @@ def test_ragged_apply_matches_single_operators(dtype, tol, library):
 
 
 def test_ragged_graph_replay_matches_eager():
+    """A captured batched apply keeps replaying correctly while eager applies
+    with other band counts, and a second capture, run through the same batch in
+    between (each graph owns its tables and workspace)."""
     from gradwave_amd import batching as B
 
     wls = _instances()
-    say("ragged graph replay")
+    say("ragged graph replay with eager calls through the same batch in between")
     hams = _hams(wls)
     batch = B.RaggedBatch(hams)
     cs = _blocks(wls, torch.complex128, seed=3)
     outs = [torch.empty_like(c) for c in cs]
     graph = batch.capture(cs, outs)
-    eager_batch = B.RaggedBatch(hams)
+    small = _blocks(wls, torch.complex128, seed=4, nbs=[max(1, w.nbands // 3) for w in wls])
+    small_outs = [torch.empty_like(c) for c in small]
+    graph2 = batch.capture(small, small_outs)
     for rep in range(3):
         g = torch.Generator().manual_seed(10 + rep)
         for c, w in zip(cs, wls):
             c.copy_((torch.randn(c.shape, dtype=torch.complex128, generator=g).to(dev()) * w.bk.mask[:, None, :]))
+        other = batch.apply(_blocks(wls, torch.complex128, seed=20 + rep,
+                                    nbs=[(w.nbands + rep) % 5 for w in wls]))  # eager, other shapes
         graph.replay()
+        graph2.replay()
         torch.cuda.synchronize()
-        want = eager_batch.apply(cs)
-        torch.cuda.synchronize()
-        for o, wv in zip(outs, want):
-            assert _rel(o, wv) < 1e-13
+        for h, c, o, cs2, o2 in zip(hams, cs, outs, small, small_outs):
+            assert _rel(o, h.apply(c)) < 1e-13
+            assert _rel(o2, h.apply(cs2)) < 1e-13
+        assert len(other) == len(wls)
 
 
 def test_lockstep_davidson_matches_independent_solves():

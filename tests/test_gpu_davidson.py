@@ -205,9 +205,11 @@ def test_si8_mixed_refinement_sweeps():
 
 
 def test_sternheimer_cg_matches_dense_solve():
-    """cg_sternheimer (postscf/_response.py:502-681) on Si2, two k-points, four
-    occupied bands: against a dense solve of (H - eps_n + s P_occ) x = rhs with
-    the explicit H; both the per-column path (apply_cols) and the lockstep path."""
+    """cg_sternheimer (contract of postscf/_response.py:502-595) on Si2, two
+    k-points, four occupied bands: against a dense solve of
+    (H - eps_n + s P_occ) x = rhs with the explicit H, through the operator's
+    compacted column apply and through full-block applies; warm starts and
+    padded garbage in x0; complex64 operands."""
     from types import SimpleNamespace
 
     from gradwave_amd.response import cg_sternheimer
@@ -242,7 +244,93 @@ def test_sternheimer_cg_matches_dense_solve():
     d = dev()
     h = BatchedHamiltonian(gpu, s.shape, s.v_r.to(d), p.to(d))
     x0 = torch.zeros_like(rhs).to(d)
-    for op in (h, SimpleNamespace(apply=h.apply)):  # per-column, then lockstep
+    for op in (h, SimpleNamespace(apply=h.apply)):  # compacted column applies, then full-block applies
         got = cg_sternheimer(op, gpu, c_occ.to(d), eps.to(d), rhs.to(d), x0, shift, tol=1e-10, max_iter=500).cpu()
         err = float((got - want).abs().max() / want.abs().max())
         assert err < 1e-7, err
+    # a warm start is certified, not trusted: the exact answer plus a small error,
+    # and garbage in the padded slots (zeroed on entry), both converge to the same
+    warm = want.clone()
+    warm[0, 1] += 1e-3 * torch.randn(m, dtype=torch.complex128, generator=g) * cpu.mask[0]
+    warm[1] += 5.0 * (~cpu.mask[1]).to(torch.complex128)
+    got = cg_sternheimer(h, gpu, c_occ.to(d), eps.to(d), rhs.to(d), warm.to(d), shift, tol=1e-10,
+                         max_iter=500).cpu()
+    assert float((got - want).abs().max() / want.abs().max()) < 1e-7
+    # complex64 operands run the FP32 kernels to an FP32-reachable tolerance
+    got32 = cg_sternheimer(h, gpu, c_occ.to(torch.complex64).to(d), eps.to(d), rhs.to(torch.complex64).to(d),
+                           x0.to(torch.complex64), shift, tol=1e-4, max_iter=500).cpu().to(torch.complex128)
+    assert float((got32 - want).abs().max() / want.abs().max()) < 1e-3
+    # a zero iteration budget returns the projected start
+    z = cg_sternheimer(h, gpu, c_occ.to(d), eps.to(d), rhs.to(d), x0, shift, tol=1e-10, max_iter=0)
+    assert float(z.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("factor", [1, 2])
+def test_small_max_dim_matches_oracle(factor):
+    """max_dim_factor below 2 (max_dim < nb + n_add after a restart) is a legal
+    argument: the reference concatenates past max_dim (davidson.py:757-759).
+    Eigenvalues against the oracle's davidson_batched to 1e-10 Ha."""
+    say(f"davidson si2 max_dim_factor={factor} vs oracle")
+    s = system("si2")
+    sel = [0, 6]
+    cpu, gpu, p = select_k(s, sel)
+    h = BatchedHamiltonian(gpu, s.shape, s.v_r.to(dev()), p.to(dev()))
+    x0 = rand_block(2, 8, cpu.npw_max, cpu.mask, seed=12)
+    r = davidson_batched(h.apply, x0.to(dev()), gpu.t, gpu.mask, tol=TOL, max_iter=400, max_dim_factor=factor)
+    rr = ref_davidson(ref_h(s, sel).apply, x0, cpu.t, cpu.mask, tol=TOL, max_iter=400, max_dim_factor=factor)
+    assert float(r.residual_norms.max()) < TOL and float(rr.residual_norms.max()) < TOL
+    assert float((r.eigenvalues.cpu() - rr.eigenvalues).abs().max()) < 1e-10 * HARTREE_EV
+
+
+def test_two_stage_matches_oracle_two_stage():
+    """GPU davidson_batched_ms against the oracle's (solvers/davidson.py:777-812):
+    the skip case (crossover >= tol) and the draft + polish case (crossover < tol),
+    eigenvalues to 1e-10 Ha, the draft counted as low-precision applies."""
+    from oracle.davidson import davidson_batched_ms as ref_ms
+
+    say("davidson_batched_ms si2 vs oracle")
+    s = system("si2")
+    sel = [1, 9]
+    cpu, gpu, p = select_k(s, sel)
+    h = BatchedHamiltonian(gpu, s.shape, s.v_r.to(dev()), p.to(dev()))
+    ref = ref_h(s, sel)
+    x0 = rand_block(2, 8, cpu.npw_max, cpu.mask, seed=13)
+    for tol, crossover in ((TOL, 1e-4), (1e-6, 1e-7)):
+        got = davidson_batched_ms(h.apply, x0.to(dev()), gpu.t, gpu.mask, tol=tol, max_iter=200,
+                                  crossover=crossover)
+        want = ref_ms(ref.apply, x0, cpu.t, cpu.mask, tol=tol, max_iter=200, crossover=crossover)
+        assert float(got.residual_norms.max()) < tol
+        assert float((got.eigenvalues.cpu() - want.eigenvalues).abs().max()) < 1e-10 * HARTREE_EV
+        assert (got.n_apply_low > 0) == (crossover < tol)
+
+
+def test_cholqr2_complex64_ill_conditioned_matches_oracle():
+    """CholQR2 of an ill-conditioned complex64 block (condition number ~3e3):
+    the Gram is formed in FP64 from upcast rows (davidson.py:213-216), so the
+    rows come out as orthonormal as the oracle's (complex64 storage bound) and
+    span the same space."""
+    from gradwave_amd.davidson import _cholqr2_dev
+    from oracle.davidson import _cholqr2 as ref_cholqr2
+
+    say("cholqr2 complex64 ill-conditioned vs oracle")
+    g = torch.Generator().manual_seed(21)
+    n, m = 24, 3000
+    a = torch.randn(n, m, dtype=torch.complex128, generator=g)
+    u, _, vh = torch.linalg.svd(a, full_matrices=False)
+    sv = torch.logspace(0, -3.5, n, dtype=torch.float64)
+    v = ((u * sv) @ vh).to(torch.complex64)[None]
+    q, infos = _cholqr2_dev(v.to(dev()))
+    assert int(infos.abs().max()) == 0
+    q = q.cpu()
+    qr = ref_cholqr2(v[0].T).T[None]
+    eye = torch.eye(n, dtype=torch.complex128)
+
+    def orth(z):
+        z = z[0].to(torch.complex128)
+        return float((z @ z.mH - eye).abs().max())
+
+    assert orth(q) < max(4 * orth(qr), 1e-5), (orth(q), orth(qr))
+    # same span: the oracle's rows lie in the span of the GPU rows
+    z, zr = q[0].to(torch.complex128), qr[0].to(torch.complex128)
+    resid = zr - (zr @ z.mH) @ z
+    assert float(resid.abs().max()) < 1e-4

@@ -148,6 +148,10 @@ def test_apply_cols_matches_apply():
     cols = h.apply_cols(c[kcol, bcol], kcol)
     assert happly_tally() == nk * 4 + 6
     assert float((cols - full[kcol, bcol]).abs().max()) < 1e-12 * float(full.abs().max())
+    want = ref_h(s).apply_cols(c[kcol, bcol].cpu(), kcol.cpu())  # the oracle's apply_cols
+    assert _rel(cols.cpu(), want) < 1e-12
+    cols32 = h.apply_cols(c[kcol, bcol].to(torch.complex64), kcol).cpu().to(torch.complex128)
+    assert _rel(cols32, want) < 2e-5
     assert h.apply_cols(c[kcol[:0], bcol[:0]], kcol[:0]).shape == (0, m)
     assert h.apply(c[:, :0]).shape == (nk, 0, m)  # an empty band block is returned as is
 
