@@ -1,17 +1,19 @@
 """Benchmark: H|psi> band-applies/s and Davidson s/k-pt on one MI355X.
 
-Workload (default): Si64 Gamma supercell (2x2x2 conventional, L=10.86 A, seeded
-U(+-0.02 A) jitter), 40 Ry, 37,073 plane waves, 90^3 box, 160 bands, 256 KB
-projectors, analytic Gaussian pseudopotential -- the precision form: FFTs,
-V(r) multiply and projector GEMMs in FP32 (MFMA, FP32 accumulate), subspace
-steps in FP64. One step = one H|psi> apply of the full (1, 160, npw) block.
+Workload (default): Si216 Gamma supercell (3x3x3 conventional, L=16.29 A,
+seeded U(+-0.03 A) jitter), 30 Ry, 80,797 plane waves, 120^3 box, 480 bands,
+864 KB projectors, analytic Gaussian pseudopotential. One step = one H|psi>
+apply of the full (1, 480, npw) block.
 
-Printed JSON line (rank 0): value = FP32 band-applies/s with inputs resident in
-HBM; extra keys give the FP64 apply rate, the Davidson solve times (FP64 and
-mixed two-stage, each to 1e-10 Ha residual) and their eigenvalue difference, the
-sweep form of the mixed solve (FP32 draft + final FP64 refinement sweeps, its
-eigenvalue error and the FP64 solve stopped at the same residual), the roofline of the dominant kernel (HIP-event timed on the launch stream) and the
-CPU baseline (the oracle's op-for-op restatement, torch CPU complex128).
+Printed JSON line: value = FP64 (complex128) band-applies/s with the block
+resident in HBM; the FP32 (complex64) rate and its error against FP64 beside
+it; the Davidson solve times per k-point (FP64, the mixed two-stage solve, the
+sweep form: FP32 draft + final FP64 refinement sweeps, and the FP64 solve
+stopped at the sweep's residual) with eigenvalue errors against FP64; the
+adjoint forces; the Gamma real layout; the roofline of the dominant FP64
+kernel and of every stage (HIP-event timed on the launch stream); and the CPU
+baseline (the oracle's op-for-op restatement, torch CPU complex128: a bounded
+apply sample and, for large cells, an extrapolated Davidson estimate).
 """
 
 from __future__ import annotations
@@ -41,17 +43,20 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="si64_gamma",
+    ap.add_argument("--workload", default="si216_gamma",
                     choices=["si64_gamma", "si2_fcc", "si8_mp", "si216_gamma", "ragged32"])
     ap.add_argument("--no-davidson", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-bands", type=int, default=8)
+    ap.add_argument("--cpu-davidson-iters", type=int, default=2,
+                    help="oracle Davidson iterations timed for the CPU solve estimate on large cells")
     ap.add_argument("--davidson-max-iter", type=int, default=300)
     ap.add_argument("--no-forces", action="store_true")
     ap.add_argument("--no-gamma", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile and split-K width")
-    ap.add_argument("--only", choices=["gamma", "forces"], default=None,
-                    help="profiling runs: time just the Gamma-layout apply or just the adjoint forces")
+    ap.add_argument("--only", choices=["apply", "gamma", "forces", "copy"], default=None,
+                    help="profiling runs: time just the block apply (FP64 then FP32), the Gamma-layout "
+                         "apply, the adjoint forces, or a 1 GiB device copy (traffic-counter calibration)")
     ap.add_argument("--profile-davidson", action="store_true")
     ap.add_argument("--tol-ha", type=float, default=1e-10)
     ap.add_argument("--draft-tol", type=float, default=1e-3, help="mixed draft tolerance [eV]")
@@ -115,10 +120,14 @@ def main() -> None:
         total = time.perf_counter() - t_all
         return total / steps, statistics.median(ts)
 
+    ms64, med64 = timed_applies(x64, args.steps, args.warmup)
     ms32, med32 = timed_applies(x32, args.steps, args.warmup)
-    ms64, med64 = timed_applies(x64, max(5, args.steps // 2), 2)
-    print(f"[bench] apply fp32 {ms32 * 1e3:.3f} ms/step, fp64 {ms64 * 1e3:.3f} ms/step",
+    print(f"[bench] apply fp64 {ms64 * 1e3:.3f} ms/step, fp32 {ms32 * 1e3:.3f} ms/step",
           file=sys.stderr, flush=True)
+    h64 = h.apply_terms(x64, L.GW_TERM_ALL)
+    h32 = h.apply_terms(x32, L.GW_TERM_ALL).to(torch.complex128)
+    err32 = float((h32 - h64).abs().max() / h64.abs().max())
+    del h64, h32
 
     # per-stage HIP-event timing (separate loop: events sit between kernels)
     def stages(x, reps=10):
@@ -132,10 +141,9 @@ def main() -> None:
         h.set_flags(0)
         return {k: v[0] / max(v[1], 1) for k, v in st.items()}
 
-    st32, st64 = stages(x32), stages(x64)
-    e32, e64 = 8, 16
+    st64, st32 = stages(x64), stages(x32)
     n23 = wl.shape[1] * wl.shape[2]
-    info = h.info(torch.complex64)
+    info = h.info(torch.complex128)
     na = info["active_planes"]
     nbt = nk * nb
     nproj = int(wl.p.shape[1])
@@ -154,44 +162,50 @@ def main() -> None:
             "backproj": ("mfma", 8.0 * nproj * nb * npw_tot),
         }
 
-    model32 = stage_model(e32)
-    dom = max(st32, key=lambda k: st32[k])
-    kind, work = model32[dom]
-    t_dom = st32[dom] * 1e-3
-    if kind == "hbm":
-        roof = {"kernel": dom, "bound": "hbm", "achieved": work / t_dom / 1e9, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s"}
-    else:
-        roof = {"kernel": dom, "bound": "mfma", "achieved": work / t_dom / 1e12,
-                "peak": MFMA_PEAK["fp32"], "unit": "TFLOP/s"}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["traffic"] = profiled_traffic(args.workload, dom, "float")
-    roof["kernel_name"] = stage_kernel(dom, "float")
-    roof["avg_ms"] = st32[dom]
-    roof["algorithmic_per_launch"] = work
-    roof["stage_ms_fp32"] = st32
+    def roofline_of(stage, st, e, prec):
+        kind, work = stage_model(e)[stage]
+        t = st[stage] * 1e-3
+        if kind == "hbm":
+            r = {"kernel": stage, "bound": "hbm", "achieved": work / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+        else:
+            r = {"kernel": stage, "bound": "mfma", "achieved": work / t / 1e12, "peak": MFMA_PEAK[prec],
+                 "unit": "TFLOP/s"}
+        r["frac"] = r["achieved"] / r["peak"]
+        r["avg_ms"] = st[stage]
+        r["algorithmic_per_launch"] = work
+        return r
+
+    real64 = "double"
+    dom = max(st64, key=lambda k: st64[k])
+    roof = roofline_of(dom, st64, 16, "fp64")
+    roof["kernel_name"] = stage_kernel(dom, real64, wl.name)
+    roof["traffic"], roof["traffic_source"] = profiled_traffic(args.workload, dom, real64)
     roof["stage_ms_fp64"] = st64
-    if kind == "hbm":
-        roof["copy_GBps"] = copy_bandwidth(dev)
-        roof["frac_of_copy"] = roof["achieved"] / roof["copy_GBps"]
+    roof["stage_ms_fp32"] = st32
+    roof["stages_fp64"] = {k: {f: v for f, v in roofline_of(k, st64, 16, "fp64").items() if f != "kernel"}
+                           for k in st64 if k != "dmul"}
+    roof["stages_fp32"] = {k: {f: v for f, v in roofline_of(k, st32, 8, "fp32").items() if f != "kernel"}
+                           for k in st32 if k != "dmul"}
+    roof["copy_GBps"] = copy_bandwidth(dev)
 
     result = {
         "metric": METRIC,
-        "value": nbt / ms32,
+        "value": nbt / ms64,
         "unit": "band-applies/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms32 * 1e3,
+        "ms_per_step": ms64 * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32",
+        "dtype": "fp64",
         "data": "synthetic (analytic Gaussian pseudopotential, seeded start vectors)",
         "config": {"workload": wl.name, "nk": nk, "nb": nb, "npw_max": m, "box": list(wl.shape),
-                   "nproj": nproj, "precision_form": "FP32 FFT/V/projector GEMMs, FP64 subspace"},
-        "fp64_band_applies_per_s": nbt / ms64,
-        "fp64_ms_per_step": ms64 * 1e3,
+                   "nproj": nproj, "apply": "complex128 H|psi> of the full block (value); complex64 below"},
+        "fp32_band_applies_per_s": nbt / ms32,
+        "fp32_ms_per_step": ms32 * 1e3,
+        "fp32_apply_max_rel_err_vs_fp64": err32,
         "roofline": roof,
         "autotune": tuned,
     }
@@ -242,7 +256,7 @@ def main() -> None:
             result["gamma"] = gamma_form(args, wl, x64, r64)
 
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(args, wl)
+        result["cpu_baseline"] = cpu_baseline(args, wl, result.get("davidson", {}).get("fp64_iters"))
 
     print(json.dumps(result), flush=True)
 
@@ -256,7 +270,26 @@ def profile_only(args, wl, x64) -> None:
     from gradwave_amd.adjoint import band_energy_forces
 
     out = {"workload": wl.name, "only": args.only}
-    if args.only == "gamma":
+    if args.only == "copy":
+        out["copy_GBps"] = copy_bandwidth(x64.device, nbytes=1 << 30, reps=5)
+        out["bytes_read_per_copy"] = 1 << 30
+    elif args.only == "apply":
+        from gradwave_amd import _lib as L
+        from gradwave_amd.hamiltonian import BatchedHamiltonian
+
+        h = BatchedHamiltonian(wl.bk, wl.shape, wl.v_r, wl.p)
+        for name, x in (("fp64", x64), ("fp32", x64.to(torch.complex64))):
+            o = torch.empty_like(x)
+            ts = []
+            for i in range(args.warmup + args.steps):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                h.apply_terms(x, L.GW_TERM_ALL, out=o)
+                torch.cuda.synchronize()
+                if i >= args.warmup:
+                    ts.append(time.perf_counter() - t0)
+            out[f"{name}_ms"] = statistics.median(ts) * 1e3
+    elif args.only == "gamma":
         from gradwave_amd import gamma as G
 
         sph = type("Sphere", (), {})()
@@ -597,42 +630,49 @@ def forces_check(wl, psi64, psims) -> dict:
 
 STAGE_KERNEL = {"plane_inv": "k_plane_inv_w", "xpass": "k_xpass_wave", "plane_fwd": "k_plane_fwd_w",
                 "becp": "k_gemm_inner_lds", "dmul": "k_dmul", "backproj": "k_gemm_combine_lds"}
-PROFILE_OF = {"si64_gamma": "hpsi_apply_si64_gamma_90cubed.txt"}
+PROFILE_OF = {"si64_gamma": "hpsi_apply_si64_gamma_90cubed.txt",
+              "si216_gamma": "hpsi_apply_si216_gamma_120cubed.txt"}
 
 
-def stage_kernel(stage: str, real: str) -> str:
+def stage_kernel(stage: str, real: str, workload: str = "") -> str:
     """Kernel that runs a stage: the x pass is k_xpass_stage when the launch shape
-    of that precision stages the planes through LDS (mode digit of GW_TUNE_XPASS)."""
-    if stage == "xpass":
-        from gradwave_amd import _lib as L
+    of that precision stages the planes through LDS (mode digit of GW_TUNE_XPASS);
+    a projector contraction routed to rocBLAS by the autotuner runs its GEMM."""
+    from gradwave_amd import _lib as L
 
-        prec = L.GW_FP32 if real == "float" else L.GW_FP64
-        if L.lib().gw_tune_get(L.GW_TUNE_XPASS, prec) // 10000 == 1:
-            return "k_xpass_stage"
+    prec = L.GW_FP32 if real == "float" else L.GW_FP64
+    if stage == "xpass" and L.lib().gw_tune_get(L.GW_TUNE_XPASS, prec) // 10000 == 1:
+        return "k_xpass_stage"
+    if stage == "becp" and L.lib().gw_tune_get(L.GW_TUNE_INNER_LIB, prec):
+        return "rocBLAS " + ("zgemm" if prec == L.GW_FP64 else "cgemm") + " strided batched (Cijk_*)"
+    if stage == "backproj" and L.lib().gw_tune_get(L.GW_TUNE_COMBINE_LIB, prec):
+        return "rocBLAS " + ("zgemm" if prec == L.GW_FP64 else "cgemm") + " strided batched (Cijk_*)"
     return STAGE_KERNEL[stage]
 
 
 def profiled_traffic(workload: str, stage: str, real: str):
-    """HBM bytes per launch (calibrated FETCH_SIZE + WRITE_SIZE) of the stage's
-    kernel, read from the rocprofv3 summary committed under profiles/, or None."""
+    """(HBM bytes per launch, source) of the stage's kernel: calibrated
+    FETCH_SIZE + WRITE_SIZE from a rocprofv3 --pmc run of the same workload,
+    read from the summary committed under profiles/ (a line
+    ``traffic <stage> <fp64|fp32> <MB read> <MB written>``), or (None, None)."""
     name = PROFILE_OF.get(workload)
     if name is None:
-        return None
+        return None, None
     path = os.path.join(HERE, "profiles", name)
     if not os.path.exists(path):
-        return None
-    key = f"gw::{stage_kernel(stage, real)}<{real}"
+        return None, None
+    prec = "fp64" if real == "double" else "fp32"
     for line in open(path):
-        if line.startswith(key):
-            f = line.split()
+        f = line.split()
+        if len(f) == 5 and f[0] == "traffic" and f[1] == stage and f[2] == prec:
             try:
-                return (float(f[-3]) + float(f[-2])) * 1e6
+                return (float(f[3]) + float(f[4])) * 1e6, f"profiles/{name} (committed rocprofv3 --pmc figure)"
             except ValueError:
-                return None
-    return None
+                return None, None
+    return None, None
 
 
-def cpu_baseline(args, wl) -> dict:
+def cpu_baseline(args, wl, gpu_fp64_iters: int | None = None) -> dict:
     """The oracle's BatchedHamiltonian.apply (core/batch.py:480-505 restated, torch CPU
     complex128) on a bounded band sample of the same workload."""
     import torch
@@ -671,28 +711,62 @@ def cpu_baseline(args, wl) -> dict:
         out["value_all_cores"] = bk.nk * nbs / med_all
         out["cores_all"] = all_cores
         torch.set_num_threads(cores)
-    if bk.nk * wl.nbands * bk.npw_max <= 200_000:  # small cells: the oracle's full Davidson solve too
-        from oracle.davidson import davidson_batched as ref_davidson
-        from gradwave_amd.workloads import HARTREE_EV, start_vectors
+    from oracle.davidson import davidson_batched as ref_davidson
+    from gradwave_amd.workloads import HARTREE_EV, start_vectors
 
-        x0 = start_vectors(wl, seed=0).cpu()
+    x0 = start_vectors(wl, seed=0).cpu()
+    if bk.nk * wl.nbands * bk.npw_max <= 200_000:  # small cells: the oracle's full Davidson solve
         a = time.perf_counter()
         r = ref_davidson(h.apply, x0, cpu.t, cpu.mask, tol=args.tol_ha * HARTREE_EV,
                          max_iter=args.davidson_max_iter)
         t = time.perf_counter() - a
         out["davidson_fp64_s_per_kpt"] = t / bk.nk
         out["davidson_iters"] = r.n_iter
+    elif gpu_fp64_iters and args.cpu_davidson_iters > 0:
+        # large cells: time the oracle's first iterations from the same start block and
+        # extrapolate to the GPU FP64 solve's iteration count (an ESTIMATE: later
+        # iterations carry a larger basis, so it undercounts the CPU time)
+        marks = _Marks()
+        a = time.perf_counter()
+        ref_davidson(h.apply, x0, cpu.t, cpu.mask, tol=args.tol_ha * HARTREE_EV,
+                     max_iter=args.cpu_davidson_iters, history_out=marks)
+        total = time.perf_counter() - a
+        steps = [t1 - t0 for t0, t1 in zip([a] + marks.t[:-1], marks.t)]
+        first, rest = steps[0], steps[1:] or steps
+        per_iter = statistics.mean(rest)
+        est = first + per_iter * (gpu_fp64_iters - 1)
+        out["davidson_fp64_s_per_kpt_estimate"] = est / bk.nk
+        out["davidson_estimate"] = (f"oracle davidson_batched from the seeded start block, {len(steps)} iterations "
+                                    f"timed ({total:.1f} s: first {first:.1f} s incl. setup, then "
+                                    f"{per_iter:.1f} s each) and extrapolated to the GPU FP64 solve's "
+                                    f"{gpu_fp64_iters} iterations")
     return out
 
 
+class _Marks(list):
+    """history_out sink that stamps the wall time of every finished iteration."""
+
+    def __init__(self):
+        super().__init__()
+        self.t = []
+
+    def append(self, item):
+        self.t.append(time.perf_counter())
+        super().append(item)
+
+
 def _cpu_model() -> str:
+    """/proc/cpuinfo model name with its family/model numbers (the name alone can be generic)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = line.split(":", 1)
+                info.setdefault(k.strip(), v.strip())
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    return f"{name} (family {info.get('cpu family', '?')}, model {info.get('model', '?')})"
 
 
 if __name__ == "__main__":

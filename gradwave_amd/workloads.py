@@ -87,6 +87,8 @@ class Workload:
     species_index: torch.Tensor | None = None  # (na,) row of each atom in vloc_tables
     vloc_tables: torch.Tensor | None = None  # (nspecies, n1, n2, n3) eV A^3, G=0 entry alpha
     g_cart: torch.Tensor | None = None  # (n1, n2, n3, 3) A^-1
+    ecut_ry: float = 0.0  # the cutoff as given (Ry); ecut = ecut_ry * RY_EV
+    species: tuple = ()  # Species of each atom
 
 
 def reciprocal(cell: np.ndarray) -> np.ndarray:
@@ -261,7 +263,7 @@ def build(name: str, cell, positions, species: list[Species], ecut_ry: float, kf
     return Workload(name, cell, pos, tuple(shape), ecut, kfrac, np.asarray(kweights, dtype=np.float64),
                     bk, p.contiguous(), v_r, nbands, float(sum(sp.z for sp in species)),
                     volume=vol, species_index=sidx, vloc_tables=torch.stack(tables).contiguous(),
-                    g_cart=g.contiguous())
+                    g_cart=g.contiguous(), ecut_ry=float(ecut_ry), species=tuple(species))
 
 
 def jitter(pos: np.ndarray, delta: float, seed: int) -> np.ndarray:

@@ -72,7 +72,7 @@ def test_position_gradient_matches_central_fd(case):
 def test_energies_module_forces_match_central_fd(case):
     """oracle.energies.band_energy_forces (the forces.py:168-213 pattern) on a
     padded block with per-band weights: F = -dE/dtau against central FD."""
-    from oracle.energies import band_energy_forces, band_energy_psi
+    from oracle.energies import band_energy_forces, band_energy_forces_hf, band_energy_psi
 
     s, cs, _, _ = case
     m = max(int(s.spheres[ik].npw) for ik in SEL)
@@ -93,6 +93,10 @@ def test_energies_module_forces_match_central_fd(case):
                 fd[a, x] = -(ep - em) / (2 * h)
     assert float((fd - f).norm() / f.norm()) < 1e-6
     assert abs(e - float(band_energy_psi(s, c, w, s.positions, kpoints=SEL))) == 0.0
+    # the forces.py route (density local term + <beta|psi> nonlocal term, chunked bands)
+    e_hf, f_hf = band_energy_forces_hf(s, c, w, kpoints=SEL, band_chunk=3)
+    assert abs(e_hf - e) < 1e-12 * abs(e)
+    assert float((f_hf - f).norm() / f.norm()) < 1e-10
 
 
 def test_operator_input_gradients_match_directional_fd(case):
