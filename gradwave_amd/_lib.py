@@ -41,6 +41,8 @@ SIGNATURES = {
     "gw_hpsi_workspace_bytes": (_Z, [_P, _I, _I]),
     "gw_hpsi_apply": (_I, [_P, _I, _I, _P, _P, _I, _P, _Z, _P]),
     "gw_hpsi_apply_strided": (_I, [_P, _I, _I, _P, _L, _L, _P, _L, _L, _I, _P, _Z, _P]),
+    "gw_hpsi_noalloc_max_bytes": (_Z, []),
+    "gw_hpsi_apply_noalloc": (_I, [_P, _I, _I, _P, _L, _L, _P, _L, _L, _I, _P]),
     "gw_wfc_to_real_workspace_bytes": (_Z, [_P, _I, _I]),
     "gw_wfc_to_real": (_I, [_P, _I, _P, _P, _I, _P, _Z, _P]),
     "gw_sphere_to_box": (_I, [_P, _I, _P, _P, _I, _P]),

@@ -739,6 +739,14 @@ __global__ void k_scale_rows(int A, int G, cpx<T>* __restrict__ Z, long ldz, lon
 }
 
 static int g_inner_lib[2] = {0, 0};
+// Per-thread override: while set, every contraction takes the kernels here and
+// none goes to rocBLAS (whose handles and workspaces allocate device memory).
+static thread_local int g_kernels_only = 0;
+int set_kernels_only(int v) {
+  const int old = g_kernels_only;
+  g_kernels_only = v ? 1 : 0;
+  return old;
+}
 
 // K-chunks of the rocBLAS inner product: enough that the strided batch of
 // 128x128-tile GEMMs reaches about 2048 tiles, at most 128 (every chunk adds an
@@ -754,7 +762,7 @@ template <typename T>
 size_t gemm_inner_partials_bytes(int nk, int M, int N, int K) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   int s = inner_splits(nk, M, N, K);
-  if (g_inner_lib[sizeof(T) == 4]) s = std::max(s, lib_splits(M, N, K));
+  if (g_inner_lib[sizeof(T) == 4] && !g_kernels_only) s = std::max(s, lib_splits(M, N, K));
   return (size_t)nk * s * M * 2 * N * sizeof(T);
 }
 template size_t gemm_inner_partials_bytes<double>(int, int, int, int);
@@ -811,7 +819,7 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
     GW_CHECK_HIP(hipGetLastError());
     return GW_OK;
   }
-  const bool use_lib = g_inner_lib[sizeof(T) == 4] && ldx < (1L << 31) && ldy < (1L << 31);
+  const bool use_lib = g_inner_lib[sizeof(T) == 4] && !g_kernels_only && ldx < (1L << 31) && ldy < (1L << 31);
   const int splits = use_lib ? lib_splits(M, N, K) : inner_splits(nk, M, N, K);
   const size_t need = (size_t)nk * splits * M * 2 * N * sizeof(T);
   GW_REQUIRE(partials && part_bytes >= need, GW_ERR_WORKSPACE, "gemm_inner workspace %zu < %zu", part_bytes, need);
@@ -1015,7 +1023,8 @@ int gemm_combine(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su
     GW_CHECK_HIP(hipGetLastError());
     return GW_OK;
   }
-  if (g_combine_lib[sizeof(T) == 4] && J > 0 && ldu < (1L << 31) && ldy < (1L << 31) && ldz < (1L << 31))
+  if (g_combine_lib[sizeof(T) == 4] && !g_kernels_only && J > 0 && ldu < (1L << 31) && ldy < (1L << 31) &&
+      ldz < (1L << 31))
     return combine_rocblas<T>(nk, A, G, J, U, ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, st);
   const int ntx = (2 * G + kTile - 1) / kTile, nty = (A + kTile - 1) / kTile;
   const long total = (long)ntx * nty * nk;

@@ -134,6 +134,8 @@ void set_combine_lib(int precision, int v);
 int combine_lib(int precision);
 void set_inner_lib(int precision, int v);
 int inner_lib(int precision);
+// While set on the calling thread, contractions never go to rocBLAS; returns the previous value.
+int set_kernels_only(int v);
 
 // Grouped launches (the cells of a ragged batch): one launch per contraction over
 // the concatenated work items of many independent jobs; `first` is each job's

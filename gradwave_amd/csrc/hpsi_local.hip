@@ -1506,6 +1506,8 @@ extern "C" int gw_tune_get(int key, int arg) {
 }
 extern "C" int gw_version(void) { return 1; }
 
+static int noalloc_scratch_setup();
+
 extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int npw_max,
                               const int64_t* flat_idx, const int64_t* npw) {
   GW_REQUIRE(out && flat_idx && npw, GW_ERR_ARG, "null argument");
@@ -1694,6 +1696,7 @@ extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int
   (void)hipFuncSetAttribute((const void*)k_x_to_real<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_x_to_real<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   set_wave_lds_attrs(want);
+  (void)noalloc_scratch_setup();  // a failure is reported by gw_hpsi_apply_noalloc itself
   *out = h;
   return GW_OK;
 }
@@ -1942,6 +1945,63 @@ extern "C" int gw_hpsi_apply_strided(gw_plan* h, int precision, int terms, const
                            (unsigned char*)ws, (hipStream_t)stream);
   return apply_t<float>(h, terms, (const float2*)c, ldc, sc, (float2*)out, ldo, so, nb, (unsigned char*)ws,
                         (hipStream_t)stream);
+}
+
+// ---- the no-allocation apply ----
+// Scratch of gw_hpsi_apply_noalloc: one fixed array per GPU in the code
+// object's device memory, held until the process ends. Host code never passes
+// the array's name to a kernel (on the host it is only a stand-in symbol):
+// each device's address comes from hipGetSymbolAddress, taken once per device
+// with that device current (at plan creation, or at the entry point's first
+// call on the device) and kept in g_scratch_ptr.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+static int noalloc_scratch_setup() {
+  int dev = -1;
+  GW_CHECK_HIP(hipGetDevice(&dev));
+  GW_REQUIRE(dev >= 0 && dev < kMaxDevices, GW_ERR_UNSUPPORTED, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return GW_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  GW_REQUIRE(e == hipSuccess && p, GW_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+             hipGetErrorString(e));
+  g_scratch_ptr[dev] = p;
+  return GW_OK;
+}
+
+extern "C" size_t gw_hpsi_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+extern "C" int gw_hpsi_apply_noalloc(gw_plan* h, int precision, int terms, const void* c, long ldc, long sc,
+                                     void* out, long ldo, long so, int nb, void* stream) {
+  GW_REQUIRE(h && c && out && nb > 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(precision == GW_FP64 || precision == GW_FP32, GW_ERR_ARG, "bad precision");
+  // contractions on the kernels here only (rocBLAS handles and workspaces allocate)
+  const int prev = set_kernels_only(1);
+  const size_t need = apply_layout(&h->p, nb, precision).total;
+  int rc = GW_OK;
+  if (need > kNoAllocScratchBytes) {
+    set_error("no-allocation apply: %d bands need %zu bytes of scratch, above the fixed maximum of %zu "
+              "(use gw_hpsi_apply_strided with a workspace)", nb, need, kNoAllocScratchBytes);
+    rc = GW_ERR_WORKSPACE;
+  }
+  if (!rc) rc = noalloc_scratch_setup();
+  if (!rc) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    void* ws = g_scratch_ptr[dev];
+    const int flags = h->p.flags;
+    h->p.flags &= ~GW_PLAN_TIMING;  // timing would create events
+    rc = gw_hpsi_apply_strided(h, precision, terms, c, ldc, sc, out, ldo, so, nb, ws, kNoAllocScratchBytes,
+                               stream);
+    h->p.flags = flags;
+  }
+  set_kernels_only(prev);
+  return rc;
 }
 
 extern "C" int gw_hpsi_apply(gw_plan* h, int precision, int terms, const void* c, void* out, int nb,

@@ -87,6 +87,21 @@ int gw_hpsi_apply(gw_plan* plan, int precision, int terms, const void* c, void* 
 int gw_hpsi_apply_strided(gw_plan* plan, int precision, int terms, const void* c, long ldc,
                           long sc, void* out, long ldo, long so, int nb, void* workspace,
                           size_t workspace_bytes, void* stream);
+/* No-allocation variant of gw_hpsi_apply_strided. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its scratch is a fixed array of gw_hpsi_noalloc_max_bytes() bytes (64 MiB)
+ * in the library's device memory, one per GPU, held until the process ends;
+ * a call whose gw_hpsi_workspace_bytes exceed it fails with GW_ERR_WORKSPACE
+ * before any launch. It calls no hipMalloc/hipFree and no library routine that
+ * allocates: the projector contractions always take the MFMA kernels here
+ * (never rocBLAS), and plan timing is skipped for the call. Every call on a
+ * GPU shares that GPU's scratch, so calls that overlap on one GPU (other
+ * streams, other threads) race; the scratch keeps the last call's contents and
+ * needs no clearing (every byte a call reads it has written first). */
+size_t gw_hpsi_noalloc_max_bytes(void);
+int gw_hpsi_apply_noalloc(gw_plan* plan, int precision, int terms, const void* c, long ldc, long sc,
+                          void* out, long ldo, long so, int nb, void* stream);
 
 /* psi(r) = N ifftn(scatter(c)) on the natural box (nk, nb, n1, n2, n3) complex:
  * core/fftbox.py:63-70 g_to_r / core/batch.py:208-216 g_to_r_b. */
