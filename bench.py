@@ -98,7 +98,8 @@ def main() -> None:
 
         t1 = time.perf_counter()
         tiles, wgs = autotune.tune_gemm([wl], nb=nb)
-        tuned = {"gemm_tile_variant": tiles, "gemm_split_k_target_wgs": wgs, "seconds": time.perf_counter() - t1}
+        tuned = {"gemm_tile_variant": tiles, "gemm_split_k_target_wgs": wgs, "seconds": time.perf_counter() - t1,
+                 "saved_to": autotune.save()}  # later runs with --no-autotune load these choices
         print(f"[bench] autotune GEMM: tiles {tiles}, split-K target {wgs} ({tuned['seconds']:.1f}s)",
               file=sys.stderr, flush=True)
     if args.only is not None:
