@@ -32,6 +32,7 @@ SIGNATURES = {
     "gw_version": (_I, []),
     "gw_plan_create": (_I, [C.POINTER(_P), _I, _I, _I, _I, _I, _P, _P]),
     "gw_plan_destroy": (_I, [_P]),
+    "gw_box_split": (_I, [_I, _P]),
     "gw_plan_set_flags": (_I, [_P, _I]),
     "gw_plan_stage_times": (_I, [_P, _P, _P, _I, _I]),
     "gw_plan_info": (_I, [_P, _I, _P, _I]),

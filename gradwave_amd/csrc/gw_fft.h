@@ -228,6 +228,29 @@ template <int N, int SIGN, typename T> __host__ __device__ __forceinline__ void 
   }
 }
 
+// One radix task of a line pass: NP elements at base[A * q] (q < NP) are
+// DFT'd (size NP, sign SIGN) in place, with the twiddles W^{SIGN * S * r * q}
+// (table tw holds e^{+2 pi i t / n}) applied after the DFT (TW = 1), before it
+// (TW = 2) or not at all (TW = 0). Host and device code share it (the host
+// test of the three-factor split runs it task by task).
+template <int NP, int SIGN, int TW, typename T>
+__host__ __device__ __forceinline__ void line_task(cpx<T>* base, int A, const cpx<T>* __restrict__ tw, int step) {
+  cpx<T> v[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) v[q] = base[A * q];
+  if constexpr (TW == 2) {
+#pragma unroll
+    for (int q = 1; q < NP; ++q) v[q] = twmul<SIGN, T>(v[q], tw[step * q]);
+  }
+  dft<NP, SIGN, T>(v);
+  if constexpr (TW == 1) {
+#pragma unroll
+    for (int q = 1; q < NP; ++q) v[q] = twmul<SIGN, T>(v[q], tw[step * q]);
+  }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) base[A * q] = v[q];
+}
+
 // One pass of a line transform on LDS data. For each (line, r) task:
 //   elements at  lds[line*lstride + (A*q + B*r)*es], q in [0, NP)
 //   are DFT'd (size NP, sign SIGN), optionally multiplied by W_n^{SIGN*r*q}
@@ -244,17 +267,25 @@ __device__ __forceinline__ void line_pass(cpx<T>* __restrict__ lds, const cpx<T>
     if constexpr (LINES_FAST) { line = task % nlines; r = task / nlines; }
     else { r = task % ntask; line = task / ntask; }
     const int lrow = line_map ? line_map[line] : line;
-    cpx<T>* base = lds + lrow * lstride + B * r * es;
-    cpx<T> v[NP];
-#pragma unroll
-    for (int q = 0; q < NP; ++q) v[q] = base[A * q * es];
-    dft<NP, SIGN, T>(v);
-    if constexpr (TWIDDLE) {
-#pragma unroll
-      for (int q = 1; q < NP; ++q) v[q] = twmul<SIGN, T>(v[q], tw[r * q]);
-    }
-#pragma unroll
-    for (int q = 0; q < NP; ++q) base[A * q * es] = v[q];
+    line_task<NP, SIGN, TWIDDLE ? 1 : 0, T>(lds + lrow * lstride + B * r * es, A * es, tw, r);
+  }
+}
+
+// Pass with a two-level task index (the middle pass of a three-factor split):
+// task t < ntask has r = t % R and o = t / R; its elements sit at
+// lds[line*lstride + (C*o + B*r + A*q)*es] and its twiddle step is S * r.
+template <int NP, int SIGN, int TW, bool LINES_FAST, typename T>
+__device__ __forceinline__ void line_pass2(cpx<T>* __restrict__ lds, const cpx<T>* __restrict__ tw,
+                                           int nlines, int lstride, int es, int ntask, int A, int B, int R, int C,
+                                           int S, const int* __restrict__ line_map) {
+  const int total = nlines * ntask;
+  for (int task = threadIdx.x; task < total; task += blockDim.x) {
+    int line, t;
+    if constexpr (LINES_FAST) { line = task % nlines; t = task / nlines; }
+    else { t = task % ntask; line = task / ntask; }
+    const int r = t % R, o = t / R;
+    const int lrow = line_map ? line_map[line] : line;
+    line_task<NP, SIGN, TW, T>(lds + lrow * lstride + (C * o + B * r) * es, A * es, tw, S * r);
   }
 }
 
@@ -274,29 +305,85 @@ __device__ __forceinline__ void line_pass_rt(int np, cpx<T>* lds, const cpx<T>* 
 #undef GW_LP
 }
 
-// Axis plan: n = n1f * n2f. Inverse (G->r, natural -> permuted), then the
-// matching forward (r->G, permuted -> natural). Barriers are the caller's
-// between transforms; one barrier sits between the two passes here.
+template <int SIGN, int TW, bool LINES_FAST, typename T, int RMAX = 16>
+__device__ __forceinline__ void line_pass2_rt(int np, cpx<T>* lds, const cpx<T>* tw, int nlines, int lstride,
+                                              int es, int ntask, int A, int B, int R, int C, int S,
+                                              const int* line_map) {
+#define GW_LP(NN) case NN: if constexpr (NN <= RMAX) line_pass2<NN, SIGN, TW, LINES_FAST, T>(lds, tw, nlines, lstride, es, ntask, A, B, R, C, S, line_map); break;
+  switch (np) {
+    GW_LP(1) GW_LP(2) GW_LP(3) GW_LP(4) GW_LP(5) GW_LP(6) GW_LP(7) GW_LP(8) GW_LP(9)
+    GW_LP(10) GW_LP(12) GW_LP(14) GW_LP(15) GW_LP(16)
+    default: break;
+  }
+#undef GW_LP
+}
+
+// Axis plan: n = n1f * n2f (* n3f). Inverse (G->r, natural -> permuted), then
+// the matching forward (r->G, permuted -> natural). Barriers are the caller's
+// between transforms; barriers sit between the passes here.
+//
+// Two factors (n3f == 1): the decimation-in-frequency split; the value of
+// output index k1 + n1f*k2 lands at position n2f*k1 + k2.
+// Three factors f0 f1 f2 (M = f1 f2): pass 1 is the same first stage over
+// stride M (twiddle W_n^{r k0}); pass 2 splits each length-M sub-line the same
+// way (stride f2, twiddle W_M^{r k1} = W_n^{f0 r k1}); pass 3 transforms the
+// contiguous groups of f2. Output index k0 + f0*k1 + f0*f1*k2 lands at position
+// M*k0 + f2*k1 + k2. The forward runs the adjoint passes in reverse order.
+// RMAX is the largest codelet a kernel instantiates (10 or 16); kernels built
+// with RMAX = kRmaxF3 take codelets up to 16 and also the three-factor split
+// (the only instantiations that carry its code, so the two-factor kernels keep
+// their register budget).
+constexpr int kRmaxF3 = 17;
+
 template <typename T, bool LINES_FAST, int RMAX = 16>
 __device__ __forceinline__ void line_fft_inverse(cpx<T>* lds, const cpx<T>* tw, int n1f, int n2f,
                                                  int nlines, int lstride, int es,
-                                                 const int* line_map) {
-  // pass 1: task j2 < n2f: DFT_n1f over a[n2f*q + j2], twiddle W^{+j2*k1}
-  line_pass_rt<+1, true, LINES_FAST, T, RMAX>(n1f, lds, tw, nlines, lstride, es, n2f, n2f, 1, line_map);
-  __syncthreads();
-  // pass 2: task k1 < n1f: DFT_n2f over a[n2f*k1 + q]
-  line_pass_rt<+1, false, LINES_FAST, T, RMAX>(n2f, lds, tw, nlines, lstride, es, n1f, 1, n2f, line_map);
+                                                 const int* line_map, int n3f = 1) {
+  if (RMAX < kRmaxF3 || n3f == 1) {
+    // pass 1: task j2 < n2f: DFT_n1f over a[n2f*q + j2], twiddle W^{+j2*k1}
+    line_pass_rt<+1, true, LINES_FAST, T, RMAX>(n1f, lds, tw, nlines, lstride, es, n2f, n2f, 1, line_map);
+    __syncthreads();
+    // pass 2: task k1 < n1f: DFT_n2f over a[n2f*k1 + q]
+    line_pass_rt<+1, false, LINES_FAST, T, RMAX>(n2f, lds, tw, nlines, lstride, es, n1f, 1, n2f, line_map);
+    return;
+  }
+  if constexpr (RMAX >= kRmaxF3) {
+    const int f0 = n1f, f1 = n2f, f2 = n3f, M = f1 * f2;
+    line_pass_rt<+1, true, LINES_FAST, T, RMAX>(f0, lds, tw, nlines, lstride, es, M, M, 1, line_map);
+    __syncthreads();
+    line_pass2_rt<+1, 1, LINES_FAST, T, RMAX>(f1, lds, tw, nlines, lstride, es, f0 * f2, f2, 1, f2, M, f0,
+                                              line_map);
+    __syncthreads();
+    line_pass_rt<+1, false, LINES_FAST, T, RMAX>(f2, lds, tw, nlines, lstride, es, f0 * f1, 1, f2, line_map);
+  }
 }
 
 template <typename T, bool LINES_FAST, int RMAX = 16>
 __device__ __forceinline__ void line_fft_forward(cpx<T>* lds, const cpx<T>* tw, int n1f, int n2f,
                                                  int nlines, int lstride, int es,
-                                                 const int* line_map) {
-  // pass 1: task a < n1f: DFT_n2f over a[n2f*a + q], twiddle W^{-a*k2}
-  line_pass_rt<-1, true, LINES_FAST, T, RMAX>(n2f, lds, tw, nlines, lstride, es, n1f, 1, n2f, line_map);
-  __syncthreads();
-  // pass 2: task k2 < n2f: DFT_n1f over a[n2f*q + k2]
-  line_pass_rt<-1, false, LINES_FAST, T, RMAX>(n1f, lds, tw, nlines, lstride, es, n2f, n2f, 1, line_map);
+                                                 const int* line_map, int n3f = 1) {
+  if (RMAX < kRmaxF3 || n3f == 1) {
+    // pass 1: task a < n1f: DFT_n2f over a[n2f*a + q], twiddle W^{-a*k2}
+    line_pass_rt<-1, true, LINES_FAST, T, RMAX>(n2f, lds, tw, nlines, lstride, es, n1f, 1, n2f, line_map);
+    __syncthreads();
+    // pass 2: task k2 < n2f: DFT_n1f over a[n2f*q + k2]
+    line_pass_rt<-1, false, LINES_FAST, T, RMAX>(n1f, lds, tw, nlines, lstride, es, n2f, n2f, 1, line_map);
+    return;
+  }
+  if constexpr (RMAX >= kRmaxF3) {
+    const int f0 = n1f, f1 = n2f, f2 = n3f, M = f1 * f2;
+    // adjoint of pass 3 and of pass 2's twiddle: DFT_f2 over the contiguous group of
+    // (k0, k1), then W_n^{-f0 k1 j} on its element j
+    line_pass2_rt<-1, 1, LINES_FAST, T, RMAX>(f2, lds, tw, nlines, lstride, es, f0 * f1, 1, f2, f1, M, f0,
+                                              line_map);
+    __syncthreads();
+    // adjoint of pass 2's DFT: DFT_f1 over stride f2 inside each sub-line
+    line_pass2_rt<-1, 0, LINES_FAST, T, RMAX>(f1, lds, tw, nlines, lstride, es, f0 * f2, f2, 1, f2, M, 0,
+                                              line_map);
+    __syncthreads();
+    // adjoint of pass 1: twiddle W_n^{-r q} first, then DFT_f0 over stride M
+    line_pass2_rt<-1, 2, LINES_FAST, T, RMAX>(f0, lds, tw, nlines, lstride, es, M, M, 1, M, 0, 1, line_map);
+  }
 }
 
 }  // namespace gw

@@ -36,6 +36,9 @@ void set_error(const char* fmt, ...);
 
 // One factorisation n = a*b with a, b in the codelet set (<= 16).
 bool split_length(int n, int* a, int* b);
+// n = f[0]*f[1]*f[2] with every factor in the codelet set: two factors (f[2] = 1)
+// when split_length finds them, else three (7-smooth lengths up to 16^3).
+bool split_length3(int n, int f[3]);
 inline bool codelet_ok(int n) {
   return n == 1 || n == 2 || n == 3 || n == 4 || n == 5 || n == 6 || n == 7 || n == 8 ||
          n == 9 || n == 10 || n == 12 || n == 14 || n == 15 || n == 16;
@@ -44,7 +47,7 @@ inline bool codelet_ok(int n) {
 // Geometry + tables of one batched H|psi> operator (all device pointers owned).
 struct Plan {
   int n[3];       // box dims (n1, n2, n3); flat = (i0*n2 + i1)*n3 + i2
-  int f[3][2];    // per axis n = f[a][0] * f[a][1]
+  int f[3][3];    // per axis n = f[a][0] * f[a][1] * f[a][2] (f[a][2] = 1: two factors)
   int nk, npw_max, na, nst_max;
   int* planes;        // (na) i0 of each active plane slot (union over k)
   int* st_plane_ptr;  // (nk, na+1) stick range of each plane slot

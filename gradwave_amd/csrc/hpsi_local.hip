@@ -80,12 +80,57 @@ bool split_length(int n, int* a, int* b) {
   return true;
 }
 
-// Largest codelet the plan's axis splits use: kernels built for radices <= 10
-// need far fewer registers than the full set (up to 16).
-static int plan_rmax(const Plan* p) {
-  int m = 1;
-  for (int ax = 0; ax < 3; ++ax) m = std::max(m, std::max(p->f[ax][0], p->f[ax][1]));
-  return m <= 10 ? 10 : 16;
+bool split_length3(int n, int f[3]) {
+  if (split_length(n, &f[0], &f[1])) {
+    f[2] = 1;
+    return true;
+  }
+  // three factors: the smallest largest factor, then the most balanced
+  int best[3] = {0, 0, 0};
+  for (int a = 2; a <= 16; ++a) {
+    if (n % a || !codelet_ok(a)) continue;
+    for (int b = 2; b <= 16; ++b) {
+      if ((n / a) % b || !codelet_ok(b)) continue;
+      const int c = n / a / b;
+      if (!codelet_ok(c) || c < 2) continue;
+      const int mx = std::max(a, std::max(b, c)), mn = std::min(a, std::min(b, c));
+      const int bmx = std::max(best[0], std::max(best[1], best[2]));
+      const int bmn = std::min(best[0], std::min(best[1], best[2]));
+      if (best[0] == 0 || mx < bmx || (mx == bmx && mn > bmn)) {
+        best[0] = a;
+        best[1] = b;
+        best[2] = c;
+      }
+    }
+  }
+  if (best[0] == 0) return false;
+  f[0] = best[0];
+  f[1] = best[1];
+  f[2] = best[2];
+  return true;
+}
+
+// Three-factor box lengths are switched off: plans take two-codelet splits only.
+// The three-factor passes (gw_fft.h, kernels built with RMAX = kRmaxF3) are
+// kept, and their index maps are checked on the host, but their GPU runs on a
+// 125 x 24 x 30 box faulted and no cause has been shown.
+constexpr bool kThreeFactorEnabled = false;
+
+// The split a plan takes for box length n (f[2] = 1 for two factors).
+static bool plan_split(int n, int f[3]) {
+  if (kThreeFactorEnabled) return split_length3(n, f);
+  f[2] = 1;
+  return split_length(n, &f[0], &f[1]);
+}
+
+extern "C" int gw_box_split(int n, int* f) {
+  GW_REQUIRE(f && n > 0, GW_ERR_ARG, "bad argument");
+  if (!plan_split(n, f)) {
+    f[0] = f[1] = f[2] = 0;
+    set_error("box length %d has no split into two codelets <= 16 (three-factor splits are not enabled)", n);
+    return GW_ERR_UNSUPPORTED;
+  }
+  return GW_OK;
 }
 
 constexpr int kMaxLds = 160 * 1024;
@@ -96,6 +141,25 @@ constexpr int kXVpt = 16;      // V values cached per thread in the x pass
 constexpr int kXBandGroup = 4; // bands per x-pass workgroup (V reuse)
 constexpr int kStickChunk = 8; // sticks per workgroup in the unfused z pass
 constexpr int kColChunk = 32;  // columns per workgroup in the unfused y pass
+
+// Largest codelet the plan's axis splits use: kernels built for radices <= 10
+// need far fewer registers than the full set (up to 16).
+static int plan_rmax(const Plan* p) {
+  int m = 1;
+  for (int ax = 0; ax < 3; ++ax) {
+    if (p->f[ax][2] > 1) return kRmaxF3;  // a three-factor axis: the kernels that carry that split
+    m = std::max(m, std::max(p->f[ax][0], p->f[ax][1]));
+  }
+  return m <= 10 ? 10 : 16;
+}
+
+// Columns per workgroup of the split y passes: kColChunk, fewer when a
+// column chunk of n2 points would not fit in LDS.
+static int col_chunk(const Plan* p, size_t e) {
+  const size_t cap = (size_t)std::min(p->lds_max, kMaxLds);
+  return (int)std::max<size_t>(1, std::min<size_t>(kColChunk, cap / ((size_t)p->n[1] * e)));
+}
+
 
 static size_t csize(int precision) { return precision == GW_FP64 ? 16 : 8; }
 
@@ -149,6 +213,7 @@ struct Geom {
   int xlo, xhi;  // active x-planes = [0, xlo) U [xhi, n1), or xlo = -1
   int st_max_plane;
   int fx0, fx1, fy0, fy1, fz0, fz1;
+  int fx2, fy2, fz2;  // third factor of each axis split (1: two factors)
   const int* planes;
   const int* st_plane_ptr;
   const int* st_i1;
@@ -166,6 +231,7 @@ static Geom geom_of(const Plan* p, int nb) {
   g.fx0 = p->f[0][0]; g.fx1 = p->f[0][1];
   g.fy0 = p->f[1][0]; g.fy1 = p->f[1][1];
   g.fz0 = p->f[2][0]; g.fz1 = p->f[2][1];
+  g.fx2 = p->f[0][2]; g.fy2 = p->f[1][2]; g.fz2 = p->f[2][2];
   g.planes = p->planes; g.st_plane_ptr = p->st_plane_ptr; g.st_i1 = p->st_i1;
   g.st_pt_ptr = p->st_pt_ptr; g.pt_src = p->pt_src; g.pt_yz = p->pt_yz; g.npw = p->npw;
   return g;
@@ -236,9 +302,9 @@ __device__ void plane_inv_body(const Geom& g, int a, int bi, const cpx<T>* __res
   __syncthreads();
   if (s1 > s0) {
     line_fft_inverse<T, false, RMAX>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1,
-                               g.st_i1 + (long)k * g.nst_max + s0);
+                               g.st_i1 + (long)k * g.nst_max + s0, g.fz2);
     __syncthreads();
-    line_fft_inverse<T, true, RMAX>(tile, twy, g.fy0, g.fy1, g.n3, 1, g.n3, nullptr);
+    line_fft_inverse<T, true, RMAX>(tile, twy, g.fy0, g.fy1, g.n3, 1, g.n3, nullptr, g.fy2);
     __syncthreads();
   }
   cpx<T>* dst = box + ((long)bi * g.na + a) * n23;
@@ -307,7 +373,7 @@ __device__ void xpass_body(const Geom& g, int L, int nbt, int chunk, int bgrp, c
     }
     __syncthreads();
     if (bg + 1 < kXBandGroup && bi + 1 < nbt) prefetch(bi + 1);
-    line_fft_inverse<T, true, RMAX>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
+    line_fft_inverse<T, true, RMAX>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr, g.fx2);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kXVpt; ++i) {
@@ -315,7 +381,7 @@ __device__ void xpass_body(const Geom& g, int L, int nbt, int chunk, int bgrp, c
       if (e < ne) { tile[e].x *= vr[i]; tile[e].y *= vr[i]; }
     }
     __syncthreads();
-    line_fft_forward<T, true, RMAX>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
+    line_fft_forward<T, true, RMAX>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr, g.fx2);
     __syncthreads();
     for (int e = threadIdx.x; e < g.na * L; e += blockDim.x) {
       const int a = e / L, l = e - a * L;
@@ -563,10 +629,10 @@ __device__ void plane_fwd_body(const Geom& g, int a, int bi, const cpx<T>* __res
   const cpx<T>* srcb = box + ((long)bi * g.na + a) * n23;
   for (int e = threadIdx.x; e < n23; e += blockDim.x) tile[e] = srcb[e];
   __syncthreads();
-  line_fft_forward<T, true, RMAX>(tile, twy, g.fy0, g.fy1, g.n3, 1, g.n3, nullptr);
+  line_fft_forward<T, true, RMAX>(tile, twy, g.fy0, g.fy1, g.n3, 1, g.n3, nullptr, g.fy2);
   __syncthreads();
   line_fft_forward<T, false, RMAX>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1,
-                             g.st_i1 + (long)k * g.nst_max + s0);
+                             g.st_i1 + (long)k * g.nst_max + s0, g.fz2);
   __syncthreads();
   const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
   const int* src = g.pt_src + (long)k * g.npw_max;
@@ -598,7 +664,7 @@ k_plane_fwd(Geom g, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ c
 // ---- unfused path (plane tile larger than LDS) ----
 
 // z inverse on a chunk of sticks: scatter points into stick rows, transform, store.
-template <typename T>
+template <typename T, int RMAX = 16>
 __global__ void __launch_bounds__(256)
 k_sticks_inv(Geom g, const cpx<T>* __restrict__ c, long ldc, long sc, cpx<T>* __restrict__ sticks,
              const cpx<T>* __restrict__ twz) {
@@ -621,73 +687,73 @@ k_sticks_inv(Geom g, const cpx<T>* __restrict__ c, long ldc, long sc, cpx<T>* __
     for (int q = pp[s] + lane; q < pp[s + 1]; q += 64)
       tile[(s - s0) * g.n3 + yz[q] % g.n3] = cb[src[q]];
   __syncthreads();
-  line_fft_inverse<T, false>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1, nullptr);
+  line_fft_inverse<T, false, RMAX>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1, nullptr, g.fz2);
   __syncthreads();
   cpx<T>* dst = sticks + ((long)bi * g.nst_max + s0) * g.n3;
   for (int e = threadIdx.x; e < (s1 - s0) * g.n3; e += blockDim.x) dst[e] = tile[e];
 }
 
-// y inverse on a column chunk of one plane, reading z-transformed sticks.
-template <typename T>
+// y inverse on a chunk of CC columns of one plane, reading z-transformed sticks.
+template <typename T, int RMAX = 16>
 __global__ void __launch_bounds__(256)
-k_plane_inv_y(Geom g, const cpx<T>* __restrict__ sticks, cpx<T>* __restrict__ box,
+k_plane_inv_y(Geom g, int CC, const cpx<T>* __restrict__ sticks, cpx<T>* __restrict__ box,
               const cpx<T>* __restrict__ twy) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);  // [n2][kColChunk]
-  const int a = blockIdx.x, bi = blockIdx.y, c0 = blockIdx.z * kColChunk;
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);  // [n2][CC]
+  const int a = blockIdx.x, bi = blockIdx.y, c0 = blockIdx.z * CC;
   const int k = bi / g.nb;
-  const int cw = min(kColChunk, g.n3 - c0);
-  for (int e = threadIdx.x; e < g.n2 * kColChunk; e += blockDim.x) tile[e] = czero<T>();
+  const int cw = min(CC, g.n3 - c0);
+  for (int e = threadIdx.x; e < g.n2 * CC; e += blockDim.x) tile[e] = czero<T>();
   __syncthreads();
   const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
   const int s0 = sp[a], s1 = sp[a + 1];
   const int* i1s = g.st_i1 + (long)k * g.nst_max;
-  for (int e = threadIdx.x; e < (s1 - s0) * kColChunk; e += blockDim.x) {
-    const int s = s0 + e / kColChunk, cc = e % kColChunk;
-    if (cc < cw) tile[i1s[s] * kColChunk + cc] = sticks[((long)bi * g.nst_max + s) * g.n3 + c0 + cc];
+  for (int e = threadIdx.x; e < (s1 - s0) * CC; e += blockDim.x) {
+    const int s = s0 + e / CC, cc = e % CC;
+    if (cc < cw) tile[i1s[s] * CC + cc] = sticks[((long)bi * g.nst_max + s) * g.n3 + c0 + cc];
   }
   __syncthreads();
   if (s1 > s0) {
-    line_fft_inverse<T, true>(tile, twy, g.fy0, g.fy1, kColChunk, 1, kColChunk, nullptr);
+    line_fft_inverse<T, true, RMAX>(tile, twy, g.fy0, g.fy1, CC, 1, CC, nullptr, g.fy2);
     __syncthreads();
   }
   cpx<T>* dst = box + ((long)bi * g.na + a) * g.n2 * g.n3 + c0;
-  for (int e = threadIdx.x; e < g.n2 * kColChunk; e += blockDim.x) {
-    const int i1 = e / kColChunk, cc = e % kColChunk;
+  for (int e = threadIdx.x; e < g.n2 * CC; e += blockDim.x) {
+    const int i1 = e / CC, cc = e % CC;
     if (cc < cw) dst[(long)i1 * g.n3 + cc] = tile[e];
   }
 }
 
-// y forward on a column chunk of one plane, writing the sticks' columns.
-template <typename T>
+// y forward on a chunk of CC columns of one plane, writing the sticks' columns.
+template <typename T, int RMAX = 16>
 __global__ void __launch_bounds__(256)
-k_plane_fwd_y(Geom g, const cpx<T>* __restrict__ box, cpx<T>* __restrict__ sticks,
+k_plane_fwd_y(Geom g, int CC, const cpx<T>* __restrict__ box, cpx<T>* __restrict__ sticks,
               const cpx<T>* __restrict__ twy) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
-  const int a = blockIdx.x, bi = blockIdx.y, c0 = blockIdx.z * kColChunk;
+  const int a = blockIdx.x, bi = blockIdx.y, c0 = blockIdx.z * CC;
   const int k = bi / g.nb;
-  const int cw = min(kColChunk, g.n3 - c0);
+  const int cw = min(CC, g.n3 - c0);
   const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
   const int s0 = sp[a], s1 = sp[a + 1];
   if (s1 == s0) return;
   const cpx<T>* srcb = box + ((long)bi * g.na + a) * g.n2 * g.n3 + c0;
-  for (int e = threadIdx.x; e < g.n2 * kColChunk; e += blockDim.x) {
-    const int i1 = e / kColChunk, cc = e % kColChunk;
+  for (int e = threadIdx.x; e < g.n2 * CC; e += blockDim.x) {
+    const int i1 = e / CC, cc = e % CC;
     tile[e] = cc < cw ? srcb[(long)i1 * g.n3 + cc] : czero<T>();
   }
   __syncthreads();
-  line_fft_forward<T, true>(tile, twy, g.fy0, g.fy1, kColChunk, 1, kColChunk, nullptr);
+  line_fft_forward<T, true, RMAX>(tile, twy, g.fy0, g.fy1, CC, 1, CC, nullptr, g.fy2);
   __syncthreads();
   const int* i1s = g.st_i1 + (long)k * g.nst_max;
-  for (int e = threadIdx.x; e < (s1 - s0) * kColChunk; e += blockDim.x) {
-    const int s = s0 + e / kColChunk, cc = e % kColChunk;
-    if (cc < cw) sticks[((long)bi * g.nst_max + s) * g.n3 + c0 + cc] = tile[i1s[s] * kColChunk + cc];
+  for (int e = threadIdx.x; e < (s1 - s0) * CC; e += blockDim.x) {
+    const int s = s0 + e / CC, cc = e % CC;
+    if (cc < cw) sticks[((long)bi * g.nst_max + s) * g.n3 + c0 + cc] = tile[i1s[s] * CC + cc];
   }
 }
 
 // z forward on a chunk of sticks and gather of the sphere points (+ t c).
-template <typename T, bool KIN>
+template <typename T, bool KIN, int RMAX = 16>
 __global__ void __launch_bounds__(256)
 k_sticks_fwd(Geom g, const cpx<T>* __restrict__ sticks, const cpx<T>* __restrict__ c, long ldc,
              long sc, const T* __restrict__ t, cpx<T>* __restrict__ out, long ldo, long so,
@@ -707,7 +773,7 @@ k_sticks_fwd(Geom g, const cpx<T>* __restrict__ sticks, const cpx<T>* __restrict
   const cpx<T>* srcb = sticks + ((long)bi * g.nst_max + s0) * g.n3;
   for (int e = threadIdx.x; e < (s1 - s0) * g.n3; e += blockDim.x) tile[e] = srcb[e];
   __syncthreads();
-  line_fft_forward<T, false>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1, nullptr);
+  line_fft_forward<T, false, RMAX>(tile, twz, g.fz0, g.fz1, s1 - s0, g.n3, 1, nullptr, g.fz2);
   __syncthreads();
   const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
   const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
@@ -1415,7 +1481,7 @@ k_x_to_real(Geom g, int L, const cpx<T>* __restrict__ box, const cpx<T>* __restr
     if (l < nl) tile[g.planes[a] * L + l] = bb[(long)a * n23 + l];
   }
   __syncthreads();
-  line_fft_inverse<T, true>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr);
+  line_fft_inverse<T, true>(tile, twx, g.fx0, g.fx1, L, 1, L, nullptr, g.fx2);
   __syncthreads();
   const long nbox = (long)g.n1 * n23;
   for (int e = threadIdx.x; e < ne; e += blockDim.x) {
@@ -1507,6 +1573,8 @@ extern "C" int gw_tune_get(int key, int arg) {
 extern "C" int gw_version(void) { return 1; }
 
 static int noalloc_scratch_setup();
+template <typename T>
+static void set_generic_lds_attrs(int want);
 
 extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int npw_max,
                               const int64_t* flat_idx, const int64_t* npw) {
@@ -1517,9 +1585,10 @@ extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int
   P.n[0] = n1; P.n[1] = n2; P.n[2] = n3;
   P.nk = nk; P.npw_max = npw_max;
   for (int ax = 0; ax < 3; ++ax) {
-    if (!split_length(P.n[ax], &P.f[ax][0], &P.f[ax][1])) {
+    if (!plan_split(P.n[ax], P.f[ax])) {
+      const int n = P.n[ax];
       delete h;
-      set_error("box length %d cannot be split into two codelets <= 16", P.n[ax]);
+      set_error("box length %d has no split into two codelets <= 16 (three-factor splits are not enabled)", n);
       return GW_ERR_UNSUPPORTED;
     }
   }
@@ -1620,7 +1689,7 @@ extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int
   }
   // twiddles e^{+2 pi i t/n} and axis permutations
   for (int ax = 0; ax < 3; ++ax) {
-    const int n = P.n[ax], a = P.f[ax][0], b = P.f[ax][1];
+    const int n = P.n[ax], f0 = P.f[ax][0], f1 = P.f[ax][1], f2 = P.f[ax][2];
     std::vector<double2> t64(n);
     std::vector<float2> t32(n);
     for (int t = 0; t < n; ++t) {
@@ -1628,9 +1697,11 @@ extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int
       t64[t] = make_double2((double)cosl(ang), (double)sinl(ang));
       t32[t] = make_float2((float)cosl(ang), (float)sinl(ang));
     }
+    // position -> index after the inverse split: M*k0 + f2*k1 + k2 holds k0 + f0*k1 + f0*f1*k2
     std::vector<int> perm(n);
-    for (int k1 = 0; k1 < a; ++k1)
-      for (int k2 = 0; k2 < b; ++k2) perm[b * k1 + k2] = k1 + a * k2;
+    for (int k0 = 0; k0 < f0; ++k0)
+      for (int k1 = 0; k1 < f1; ++k1)
+        for (int k2 = 0; k2 < f2; ++k2) perm[f1 * f2 * k0 + f2 * k1 + k2] = k0 + f0 * k1 + f0 * f1 * k2;
     if (hipMalloc(&P.tw64[ax], n * sizeof(double2)) != hipSuccess ||
         hipMalloc(&P.tw32[ax], n * sizeof(float2)) != hipSuccess ||
         hipMalloc(&h->perm[ax], n * sizeof(int)) != hipSuccess) {
@@ -1695,6 +1766,8 @@ extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int
   (void)hipFuncSetAttribute((const void*)k_xpass<float, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_x_to_real<double>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_x_to_real<float>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  set_generic_lds_attrs<double>(want);
+  set_generic_lds_attrs<float>(want);
   set_wave_lds_attrs(want);
   (void)noalloc_scratch_setup();  // a failure is reported by gw_hpsi_apply_noalloc itself
   *out = h;
@@ -1720,7 +1793,8 @@ extern "C" int gw_plan_info(const gw_plan* h, int precision, int* info, int n) {
   GW_REQUIRE(h && info, GW_ERR_ARG, "bad argument");
   const Plan& P = h->p;
   const int v[] = {P.na, P.nst_max, wave_path_ok(P, precision) ? 2 : (plane_fits_lds(&P, precision) ? 1 : 0), P.lds_max, P.nproj,
-                   P.f[0][0], P.f[0][1], P.f[1][0], P.f[1][1], P.f[2][0], P.f[2][1]};
+                   P.f[0][0], P.f[0][1], P.f[1][0], P.f[1][1], P.f[2][0], P.f[2][1],
+                   P.f[0][2], P.f[1][2], P.f[2][2]};
   for (int i = 0; i < n && i < (int)(sizeof(v) / sizeof(v[0])); ++i) info[i] = v[i];
   return GW_OK;
 }
@@ -1814,6 +1888,85 @@ extern "C" size_t gw_hpsi_workspace_bytes(const gw_plan* h, int nb, int precisio
   return apply_layout(&h->p, nb, precision).total;
 }
 
+// The generic-size local term (any box length the plan splits): x-plane inverse
+// (fused LDS plane kernel, or stick and column-chunk kernels when a plane
+// exceeds LDS), x pass, x-plane forward; kernels built for codelets <= RMAX.
+template <typename T, int RMAX>
+static int generic_local(const Plan& P, const ApplyLayout& L, const Geom& g, int nbt, const cpx<T>* c, long ldc,
+                         long sc, cpx<T>* box, const T* vp, const T* t, cpx<T>* out, long ldo, long so, T scale,
+                         bool kin, unsigned char* ws, StageTimer& tm, hipStream_t st) {
+  const size_t plane_lds = (size_t)P.n[1] * P.n[2] * sizeof(cpx<T>);
+  const cpx<T>* twx = tw_of<T>(&P, 0);
+  const cpx<T>* twy = tw_of<T>(&P, 1);
+  const cpx<T>* twz = tw_of<T>(&P, 2);
+  const int CC = col_chunk(&P, sizeof(cpx<T>));
+  const dim3 gcol(P.na, nbt, (P.n[2] + CC - 1) / CC);
+  const size_t col_lds = (size_t)P.n[1] * CC * sizeof(cpx<T>);
+  const size_t stick_lds = (size_t)kStickChunk * P.n[2] * sizeof(cpx<T>);
+  hipEvent_t ev = tm.begin();
+  if (L.fused) {
+    hipLaunchKernelGGL((k_plane_inv<T, RMAX>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, c, ldc, sc,
+                       box, twy, twz);
+  } else {
+    cpx<T>* sticks = (cpx<T>*)(ws + L.stick_off);
+    hipLaunchKernelGGL((k_sticks_inv<T, RMAX>), dim3((P.nst_max + kStickChunk - 1) / kStickChunk, nbt), dim3(256),
+                       stick_lds, st, g, c, ldc, sc, sticks, twz);
+    hipLaunchKernelGGL((k_plane_inv_y<T, RMAX>), gcol, dim3(256), col_lds, st, g, CC, sticks, box, twy);
+  }
+  tm.end(ST_PLANE_INV, ev);
+  const int XL = x_lines(&P);
+  ev = tm.begin();
+  hipLaunchKernelGGL((k_xpass<T, RMAX>), dim3((P.n[1] * P.n[2] + XL - 1) / XL, (nbt + kXBandGroup - 1) / kXBandGroup),
+                     dim3(kXThreads), (size_t)P.n[0] * XL * sizeof(cpx<T>), st, g, XL, nbt, box, vp, twx);
+  tm.end(ST_XPASS, ev);
+  ev = tm.begin();
+  if (L.fused) {
+    if (kin)
+      hipLaunchKernelGGL((k_plane_fwd<T, true, RMAX>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, box,
+                         c, ldc, sc, t, out, ldo, so, scale, twy, twz);
+    else
+      hipLaunchKernelGGL((k_plane_fwd<T, false, RMAX>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, box,
+                         c, ldc, sc, t, out, ldo, so, scale, twy, twz);
+  } else {
+    cpx<T>* sticks = (cpx<T>*)(ws + L.stick_off);
+    hipLaunchKernelGGL((k_plane_fwd_y<T, RMAX>), gcol, dim3(256), col_lds, st, g, CC, box, sticks, twy);
+    const int nsc = std::max(1, (P.nst_max + kStickChunk - 1) / kStickChunk);
+    if (kin)
+      hipLaunchKernelGGL((k_sticks_fwd<T, true, RMAX>), dim3(nsc, nbt), dim3(256), stick_lds, st, g, sticks, c, ldc,
+                         sc, t, out, ldo, so, scale, twz);
+    else
+      hipLaunchKernelGGL((k_sticks_fwd<T, false, RMAX>), dim3(nsc, nbt), dim3(256), stick_lds, st, g, sticks, c,
+                         ldc, sc, t, out, ldo, so, scale, twz);
+  }
+  tm.end(ST_PLANE_FWD, ev);
+  GW_CHECK_HIP(hipGetLastError());
+  return GW_OK;
+}
+
+// Dynamic-LDS limits of the generic local-term kernels (every codelet class).
+template <typename T, int RMAX>
+static void set_generic_lds_attrs_r(int want) {
+  (void)hipFuncSetAttribute((const void*)k_plane_inv<T, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd<T, true, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd<T, false, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_xpass<T, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_sticks_inv<T, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_sticks_fwd<T, true, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_sticks_fwd<T, false, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_y<T, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_y<T, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+}
+template <typename T>
+static void set_generic_lds_attrs(int want) {
+  set_generic_lds_attrs_r<T, 10>(want);
+  set_generic_lds_attrs_r<T, 16>(want);
+  set_generic_lds_attrs_r<T, kRmaxF3>(want);
+}
+
 template <typename T>
 static int apply_t(gw_plan* h, int terms, const cpx<T>* c, long ldc, long sc, cpx<T>* out, long ldo,
                    long so, int nb, unsigned char* ws, hipStream_t st) {
@@ -1839,63 +1992,13 @@ static int apply_t(gw_plan* h, int terms, const cpx<T>* c, long ldc, long sc, cp
       int rc = launch_wave_path<T>(P, g, nbt, c, ldc, sc, box, vn, t, out, ldo, so, scale, kin, tm, st);
       if (rc) return rc;
     } else {
-      const size_t plane_lds = (size_t)P.n[1] * P.n[2] * sizeof(cpx<T>);
-      ev = tm.begin();
-      const bool r10 = plan_rmax(&P) <= 10;
-      if (L.fused) {
-        if (r10)
-          hipLaunchKernelGGL((k_plane_inv<T, 10>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, c, ldc,
-                             sc, box, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
-        else
-          hipLaunchKernelGGL(k_plane_inv<T>, dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, c, ldc, sc,
-                             box, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
-      } else {
-        cpx<T>* sticks = (cpx<T>*)(ws + L.stick_off);
-        hipLaunchKernelGGL(k_sticks_inv<T>, dim3((P.nst_max + kStickChunk - 1) / kStickChunk, nbt), dim3(256),
-                           kStickChunk * P.n[2] * sizeof(cpx<T>), st, g, c, ldc, sc, sticks, tw_of<T>(&P, 2));
-        hipLaunchKernelGGL(k_plane_inv_y<T>, dim3(P.na, nbt, (P.n[2] + kColChunk - 1) / kColChunk), dim3(256),
-                           P.n[1] * kColChunk * sizeof(cpx<T>), st, g, sticks, box, tw_of<T>(&P, 1));
-      }
-      tm.end(ST_PLANE_INV, ev);
-      const int XL = x_lines(&P);
-      ev = tm.begin();
-      if (r10)
-        hipLaunchKernelGGL((k_xpass<T, 10>),
-                           dim3((P.n[1] * P.n[2] + XL - 1) / XL, (nbt + kXBandGroup - 1) / kXBandGroup),
-                           dim3(kXThreads), (size_t)P.n[0] * XL * sizeof(cpx<T>), st, g, XL, nbt, box, vp,
-                           tw_of<T>(&P, 0));
-      else
-        hipLaunchKernelGGL(k_xpass<T>, dim3((P.n[1] * P.n[2] + XL - 1) / XL, (nbt + kXBandGroup - 1) / kXBandGroup),
-                           dim3(kXThreads), (size_t)P.n[0] * XL * sizeof(cpx<T>), st, g, XL, nbt, box, vp,
-                           tw_of<T>(&P, 0));
-      tm.end(ST_XPASS, ev);
-      ev = tm.begin();
-      if (L.fused) {
-        if (kin && r10)
-          hipLaunchKernelGGL((k_plane_fwd<T, true, 10>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g,
-                             box, c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
-        else if (kin)
-          hipLaunchKernelGGL((k_plane_fwd<T, true>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, box,
-                             c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
-        else if (r10)
-          hipLaunchKernelGGL((k_plane_fwd<T, false, 10>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g,
-                             box, c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
-        else
-          hipLaunchKernelGGL((k_plane_fwd<T, false>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, box,
-                             c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
-      } else {
-        cpx<T>* sticks = (cpx<T>*)(ws + L.stick_off);
-        hipLaunchKernelGGL(k_plane_fwd_y<T>, dim3(P.na, nbt, (P.n[2] + kColChunk - 1) / kColChunk), dim3(256),
-                           P.n[1] * kColChunk * sizeof(cpx<T>), st, g, box, sticks, tw_of<T>(&P, 1));
-        const int nsc = std::max(1, (P.nst_max + kStickChunk - 1) / kStickChunk);
-        if (kin)
-          hipLaunchKernelGGL((k_sticks_fwd<T, true>), dim3(nsc, nbt), dim3(256), kStickChunk * P.n[2] * sizeof(cpx<T>),
-                             st, g, sticks, c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 2));
-        else
-          hipLaunchKernelGGL((k_sticks_fwd<T, false>), dim3(nsc, nbt), dim3(256), kStickChunk * P.n[2] * sizeof(cpx<T>),
-                             st, g, sticks, c, ldc, sc, t, out, ldo, so, scale, tw_of<T>(&P, 2));
-      }
-      tm.end(ST_PLANE_FWD, ev);
+      const int rc = plan_rmax(&P) <= 10 ? generic_local<T, 10>(P, L, g, nbt, c, ldc, sc, box, vp, t, out, ldo, so,
+                                                                 scale, kin, ws, tm, st)
+                     : plan_rmax(&P) <= 16 ? generic_local<T, 16>(P, L, g, nbt, c, ldc, sc, box, vp, t, out, ldo,
+                                                                  so, scale, kin, ws, tm, st)
+                                           : generic_local<T, kRmaxF3>(P, L, g, nbt, c, ldc, sc, box, vp, t, out,
+                                                                       ldo, so, scale, kin, ws, tm, st);
+      if (rc) return rc;
 
     }
   } else if (kin) {
@@ -2030,10 +2133,11 @@ static int to_real_t(gw_plan* h, const cpx<T>* c, cpx<T>* psi, int nb, unsigned 
                        st, g, c, ld, sc, box, tw_of<T>(&P, 1), tw_of<T>(&P, 2));
   } else {
     cpx<T>* sticks = (cpx<T>*)(ws + L.stick_off);
+    const int CC = col_chunk(&P, sizeof(cpx<T>));
     hipLaunchKernelGGL(k_sticks_inv<T>, dim3((P.nst_max + kStickChunk - 1) / kStickChunk, nbt), dim3(256),
                        kStickChunk * P.n[2] * sizeof(cpx<T>), st, g, c, ld, sc, sticks, tw_of<T>(&P, 2));
-    hipLaunchKernelGGL(k_plane_inv_y<T>, dim3(P.na, nbt, (P.n[2] + kColChunk - 1) / kColChunk), dim3(256),
-                       P.n[1] * kColChunk * sizeof(cpx<T>), st, g, sticks, box, tw_of<T>(&P, 1));
+    hipLaunchKernelGGL(k_plane_inv_y<T>, dim3(P.na, nbt, (P.n[2] + CC - 1) / CC), dim3(256),
+                       (size_t)P.n[1] * CC * sizeof(cpx<T>), st, g, CC, sticks, box, tw_of<T>(&P, 1));
   }
   const int XL = x_lines(&P);
   hipLaunchKernelGGL(k_x_to_real<T>, dim3((P.n[1] * P.n[2] + XL - 1) / XL, nbt), dim3(kXThreads),
@@ -2046,6 +2150,9 @@ static int to_real_t(gw_plan* h, const cpx<T>* c, cpx<T>* psi, int nb, unsigned 
 extern "C" int gw_wfc_to_real(gw_plan* h, int precision, const void* c, void* psi, int nb, void* ws,
                               size_t ws_bytes, void* stream) {
   GW_REQUIRE(h && c && psi && nb > 0, GW_ERR_ARG, "bad argument");
+  GW_REQUIRE(plan_rmax(&h->p) < kRmaxF3, GW_ERR_UNSUPPORTED,
+             "psi(r) is not available on boxes with a three-factor length (%d x %d x %d)", h->p.n[0], h->p.n[1],
+             h->p.n[2]);
   const ApplyLayout L = apply_layout(&h->p, nb, precision);
   GW_REQUIRE(ws && ws_bytes >= L.total, GW_ERR_WORKSPACE, "workspace too small");
   if (precision == GW_FP64)
@@ -2312,6 +2419,16 @@ extern "C" int gw_batch_create(gw_batch** out, gw_plan* const* plans, int n) {
   (void)hipFuncSetAttribute((const void*)k_plane_fwd_rag<float, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_xpass_rag<double, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_xpass_rag<float, 10>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_rag<double, kRmaxF3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_rag<float, kRmaxF3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_rag<double, kRmaxF3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_rag<float, kRmaxF3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_xpass_rag<double, kRmaxF3>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_xpass_rag<float, kRmaxF3>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   *out = b;
   return GW_OK;
 }
@@ -2468,6 +2585,10 @@ static int batch_apply_t(gw_batch* b, const BatchBind& s, hipStream_t st) {
     hipLaunchKernelGGL((k_plane_inv_rag<T, 10>), g13, dim3(kPlaneThreads), s.lds13, st, R, n, s.tot13);
     hipLaunchKernelGGL((k_xpass_rag<T, 10>), g2, dim3(kXThreads), s.lds2, st, R, n, s.tot2);
     hipLaunchKernelGGL((k_plane_fwd_rag<T, 10>), g13, dim3(kPlaneThreads), s.lds13, st, R, n, s.tot13);
+  } else if (s.rmax >= kRmaxF3) {
+    hipLaunchKernelGGL((k_plane_inv_rag<T, kRmaxF3>), g13, dim3(kPlaneThreads), s.lds13, st, R, n, s.tot13);
+    hipLaunchKernelGGL((k_xpass_rag<T, kRmaxF3>), g2, dim3(kXThreads), s.lds2, st, R, n, s.tot2);
+    hipLaunchKernelGGL((k_plane_fwd_rag<T, kRmaxF3>), g13, dim3(kPlaneThreads), s.lds13, st, R, n, s.tot13);
   } else {
     hipLaunchKernelGGL(k_plane_inv_rag<T>, g13, dim3(kPlaneThreads), s.lds13, st, R, n, s.tot13);
     hipLaunchKernelGGL(k_xpass_rag<T>, g2, dim3(kXThreads), s.lds2, st, R, n, s.tot2);

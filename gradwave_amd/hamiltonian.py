@@ -265,11 +265,11 @@ class BatchedHamiltonian:
         return {n: (ms[i], calls[i]) for i, n in enumerate(L.STAGES)}
 
     def info(self, dtype: torch.dtype = torch.complex128) -> dict[str, int]:
-        v = (C.c_int * 11)()
-        L.check(L.lib().gw_plan_info(self.plan.handle, L.precision_of(dtype), v, 11), "gw_plan_info")
+        v = (C.c_int * 14)()
+        L.check(L.lib().gw_plan_info(self.plan.handle, L.precision_of(dtype), v, 14), "gw_plan_info")
+        split = [tuple(f for f in (v[5 + 2 * a], v[6 + 2 * a], v[11 + a]) if f != 1) or (1,) for a in range(3)]
         return {"active_planes": v[0], "max_sticks": v[1], "fused_planes": v[2], "lds_bytes": v[3],
-                "nproj": v[4], "split_x": (v[5], v[6]), "split_y": (v[7], v[8]),
-                "split_z": (v[9], v[10])}
+                "nproj": v[4], "split_x": split[0], "split_y": split[1], "split_z": split[2]}
 
     def wfc_to_real(self, c: torch.Tensor) -> torch.Tensor:
         """psi(r) = N ifftn(scatter c) on the natural box (nk, nb, n1, n2, n3)

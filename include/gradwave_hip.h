@@ -46,12 +46,17 @@ int gw_version(void);
 
 /* Geometry from the padded BatchedK maps (core/batch.py:126-205): flat_idx
  * (nk, npw_max) int64 host array, 0 in padding; npw (nk) int64 host array.
- * Box lengths must factor as a*b with a, b in {1..10,12,14,15,16}. */
+ * Box lengths must factor as a*b with a, b in {1..10,12,14,15,16} (the
+ * three-factor splits a*b*c are built but not enabled). */
 int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int npw_max,
                    const int64_t* flat_idx, const int64_t* npw);
+/* The split a plan takes for box length n: f[0..2] with f[0]*f[1]*f[2] = n and
+ * f[2] = 1 (two codelets); GW_ERR_UNSUPPORTED when there is none (e.g. 125,
+ * 162, 200, 216, 243, 250 and every length above 256). Host only, no GPU. */
+int gw_box_split(int n, int* f);
 int gw_plan_destroy(gw_plan* plan);
 /* plan options (no reference counterpart): GW_PLAN_LEGACY selects the generic
- * LDS two-pass kernels (any 7-smooth box) instead of the wave-line kernels of
+ * LDS kernels (any 7-smooth box) instead of the wave-line kernels of
  * cubic 24/36/90/120 boxes; GW_PLAN_FORCE_UNFUSED then splits each x-plane pass
  * into stick and column-chunk kernels (both for testing). */
 enum { GW_PLAN_FORCE_UNFUSED = 1, GW_PLAN_TIMING = 2, GW_PLAN_LEGACY = 4 };
@@ -62,9 +67,10 @@ int gw_plan_set_flags(gw_plan* plan, int flags);
  * 3 projection <p|c>, 4 D multiply, 5 back-projection. This returns the summed
  * milliseconds and call counts per stage (synchronising on pending events). */
 int gw_plan_stage_times(gw_plan* plan, double* ms, long* calls, int n, int reset);
-/* info[0..10] = active x-planes, max sticks per k, fused plane path (0/1) for
+/* info[0..13] = active x-planes, max sticks per k, fused plane path (0/1) for
  * `precision` (2 wave-line, 1 LDS plane tile, 0 split), LDS bytes per
- * workgroup, nproj, factor pairs of n1, n2, n3. */
+ * workgroup, nproj, the first two factors of n1, n2, n3 (pairs), then the
+ * third factor of n1, n2, n3 (1 for a two-factor split). */
 int gw_plan_info(const gw_plan* plan, int precision, int* info, int n);
 /* kinetic diagonal t = HBAR2_2M |k+G|^2 (nk, npw_max) float64, device (BatchedK.t) */
 int gw_plan_set_kinetic(gw_plan* plan, const double* t, void* stream);

@@ -119,6 +119,71 @@ template <int N, int M, int S> static void check_waveline() {
   report("waveline forward", N, err);
 }
 
+// Host emulation of the generic LDS line passes of gw_fft.h (line_fft_inverse /
+// line_fft_forward): every task of a pass runs gw::line_task, the same code the
+// device passes run, with the same (ntask, A, B, R, C, S) parameters.
+template <int SIGN, int TW>
+static void host_pass(int np, double2* a, const double2* tw, int ntask, int A, int B, int R, int C, int S) {
+  for (int t = 0; t < ntask; ++t) {
+    const int r = t % R, o = t / R;
+    double2* base = a + C * o + B * r;
+    switch (np) {
+#define GW_HP(NN) case NN: gw::line_task<NN, SIGN, TW, double>(base, A, tw, S * r); break;
+      GW_HP(1) GW_HP(2) GW_HP(3) GW_HP(4) GW_HP(5) GW_HP(6) GW_HP(7) GW_HP(8) GW_HP(9) GW_HP(10)
+      GW_HP(12) GW_HP(14) GW_HP(15) GW_HP(16)
+#undef GW_HP
+      default: ++g_fail; break;
+    }
+  }
+}
+
+// natural index held at position pos after the inverse split (f2 = 1: two factors)
+static int split_index(int pos, int f0, int f1, int f2) {
+  const int M = f1 * f2;
+  const int k0 = pos / M, k1 = (pos % M) / f2, k2 = pos % f2;
+  return k0 + f0 * k1 + f0 * f1 * k2;
+}
+
+static void check_split(int f0, int f1, int f2) {
+  const int n = f0 * f1 * f2, M = f1 * f2;
+  std::vector<double2> tw(n);
+  for (int m = 0; m < n; ++m) tw[m] = double2{std::cos(2 * M_PI * m / n), std::sin(2 * M_PI * m / n)};
+  std::vector<double2> a = random_line(n, 300 + n), ref;
+  naive(a, ref, +1);
+  if (f2 == 1) {
+    host_pass<+1, 1>(f0, a.data(), tw.data(), f1, f1, 1, f1, 0, 1);
+    host_pass<+1, 0>(f1, a.data(), tw.data(), f0, 1, f1, f0, 0, 1);
+  } else {
+    host_pass<+1, 1>(f0, a.data(), tw.data(), M, M, 1, M, 0, 1);
+    host_pass<+1, 1>(f1, a.data(), tw.data(), f0 * f2, f2, 1, f2, M, f0);
+    host_pass<+1, 0>(f2, a.data(), tw.data(), f0 * f1, 1, f2, f0 * f1, 0, 1);
+  }
+  double err = 0;
+  for (int pos = 0; pos < n; ++pos) {
+    const double2 r = ref[split_index(pos, f0, f1, f2)];
+    err = std::fmax(err, std::hypot(a[pos].x - r.x, a[pos].y - r.y));
+  }
+  char what[64];
+  snprintf(what, sizeof(what), "lds split %dx%dx%d inverse", f0, f1, f2);
+  report(what, n, err);
+  // forward: the permuted layout of a natural line back to natural order
+  std::vector<double2> bnat = random_line(n, 400 + n), b(n);
+  naive(bnat, ref, -1);
+  for (int pos = 0; pos < n; ++pos) b[pos] = bnat[split_index(pos, f0, f1, f2)];
+  if (f2 == 1) {
+    host_pass<-1, 1>(f1, b.data(), tw.data(), f0, 1, f1, f0, 0, 1);
+    host_pass<-1, 0>(f0, b.data(), tw.data(), f1, f1, 1, f1, 0, 1);
+  } else {
+    host_pass<-1, 1>(f2, b.data(), tw.data(), f0 * f1, 1, f2, f1, M, f0);
+    host_pass<-1, 0>(f1, b.data(), tw.data(), f0 * f2, f2, 1, f2, M, 0);
+    host_pass<-1, 2>(f0, b.data(), tw.data(), M, M, 1, M, 0, 1);
+  }
+  err = 0;
+  for (int k = 0; k < n; ++k) err = std::fmax(err, std::hypot(b[k].x - ref[k].x, b[k].y - ref[k].y));
+  snprintf(what, sizeof(what), "lds split %dx%dx%d forward", f0, f1, f2);
+  report(what, n, err);
+}
+
 int main() {
   check_dft<1>(); check_dft<2>(); check_dft<3>(); check_dft<4>(); check_dft<5>();
   check_dft<6>(); check_dft<7>(); check_dft<8>(); check_dft<9>(); check_dft<10>();
@@ -128,6 +193,12 @@ int main() {
   check_waveline<36, 6, 6>(); check_waveline<36, 4, 9>();
   check_waveline<90, 9, 10>();
   check_waveline<120, 10, 12>(); check_waveline<120, 8, 15>();
+  // the generic LDS kernels' two- and three-factor splits (three factors: every
+  // 7-smooth box length without a two-codelet split, e.g. 125, 162, 200, 216, 243, 250, > 256)
+  check_split(12, 10, 1); check_split(9, 10, 1); check_split(7, 7, 1);
+  check_split(5, 5, 5); check_split(2, 9, 9); check_split(8, 5, 5); check_split(6, 6, 6);
+  check_split(3, 9, 9); check_split(10, 5, 5); check_split(7, 7, 6); check_split(4, 3, 5);
+  check_split(16, 16, 16); check_split(15, 14, 12);
   printf("worst %.3e failures %d\n", g_worst, g_fail);
   return g_fail ? 1 : 0;
 }

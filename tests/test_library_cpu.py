@@ -73,4 +73,32 @@ def test_argument_errors_are_reported_without_a_gpu():
     flat = (C.c_int64 * 1)(0)
     npw = (C.c_int64 * 1)(1)
     h = C.c_void_p()
-    assert "cannot be split" in err(lib.gw_plan_create(C.byref(h), 11, 11, 11, 1, 1, flat, npw))
+    assert "box length 11 has no split" in err(lib.gw_plan_create(C.byref(h), 11, 11, 11, 1, 1, flat, npw))
+
+
+@pytest.mark.skipif(not os.path.exists(L.LIB_PATH), reason="library not built")
+def test_box_splits_of_good_fft_sizes():
+    """Box lengths the reference's good_fft_size returns (grids.py:32-41): every
+    one with a two-codelet split plans with the most balanced such split; the
+    others (125, 162, 200, 216, 243, 250, above 256) are refused, because the
+    three-factor splits are not enabled."""
+    import ctypes as C
+
+    from oracle.grids import good_fft_size
+
+    codelets = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16}
+    lib = L.lib()
+    f = (C.c_int * 3)()
+    lengths = sorted({good_fft_size(m) for m in range(1, 625)})
+    refused = []
+    for n in lengths:
+        two = [(x, n // x) for x in sorted(codelets) if n % x == 0 and n // x in codelets]
+        rc = lib.gw_box_split(n, f)
+        if two:
+            assert rc == 0 and f[2] == 1 and (f[0], f[1]) in two, n
+            assert abs(f[0] - f[1]) == min(abs(a - b) for a, b in two), n
+        else:
+            assert rc == 3, n  # GW_ERR_UNSUPPORTED
+            refused.append(n)
+    assert {125, 162, 200, 216, 243, 250} <= set(refused) and min(x for x in refused if x > 256) > 256
+    assert refused[:9] == [125, 147, 162, 175, 189, 200, 216, 243, 245], refused
