@@ -88,7 +88,8 @@ enum Stage { ST_PLANE_INV = 0, ST_XPASS, ST_PLANE_FWD, ST_BECP, ST_DMUL, ST_BACK
 struct StageTimer {
   bool on = false;
   hipStream_t st = nullptr;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  struct Pending { int stage; hipEvent_t b, e; bool count; };
+  std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
   double ms[ST_COUNT] = {0};
   long calls[ST_COUNT] = {0};
@@ -97,17 +98,20 @@ struct StageTimer {
     hipEvent_t e; (void)hipEventCreate(&e); return e;
   }
   hipEvent_t begin() { if (!on) return nullptr; hipEvent_t e = get(); (void)hipEventRecord(e, st); return e; }
+  // With count_calls false, end() adds the time to the stage without counting a
+  // call (the later band chunks of one apply).
+  bool count_calls = true;
   void end(int stage, hipEvent_t b) {
     if (!on) return;
     hipEvent_t e = get(); (void)hipEventRecord(e, st);
-    pending.push_back({stage, {b, e}});
+    pending.push_back({stage, b, e, count_calls});
   }
   void collect() {
     for (auto& p : pending) {
-      float t = 0; (void)hipEventSynchronize(p.second.second);
-      (void)hipEventElapsedTime(&t, p.second.first, p.second.second);
-      ms[p.first] += t; calls[p.first] += 1;
-      pool.push_back(p.second.first); pool.push_back(p.second.second);
+      float t = 0; (void)hipEventSynchronize(p.e);
+      (void)hipEventElapsedTime(&t, p.b, p.e);
+      ms[p.stage] += t; calls[p.stage] += p.count ? 1 : 0;
+      pool.push_back(p.b); pool.push_back(p.e);
     }
     pending.clear();
   }
@@ -127,7 +131,8 @@ struct ApplyLayout {
   size_t total;
   bool fused;
 };
-ApplyLayout apply_layout(const Plan* p, int nb, int precision);
+// chunked: the layout of gw_hpsi_apply, whose box holds one band chunk (GW_TUNE_BAND_CHUNK)
+ApplyLayout apply_layout(const Plan* p, int nb, int precision, bool chunked = false);
 bool plane_fits_lds(const Plan* p, int precision);
 void set_gemm_target_wgs(int v);
 int gemm_target_wgs();

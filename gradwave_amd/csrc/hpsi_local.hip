@@ -50,6 +50,11 @@ struct XpassShape { int mode, waves, bgroup; };
 // Defaults from timing Si64 (90^3) and Si216 (120^3) Gamma applies: FP32 gains from
 // staging (-16% / -12%), FP64 (twice the bytes per lane, half the tiles per CU) does not.
 static XpassShape g_xpass[2] = {{0, 4, 8}, {1, 4, 16}};
+// Bands per chunk of the local term, per precision (0 = the whole block in one
+// pass sequence). A chunk's x-plane box is written by P1, read and rewritten by
+// the x pass and read by P3 back to back, so a chunk small enough stays in the
+// memory-side cache between the passes; the box workspace shrinks with it.
+static int g_band_chunk[2] = {0, 0};
 static int xp_index(int precision) { return precision == GW_FP64 ? 0 : 1; }
 static int xpass_encode(const XpassShape& x) { return x.mode * 10000 + x.waves * 100 + x.bgroup; }
 
@@ -176,17 +181,19 @@ static int x_lines(const Plan* p) {
 
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-ApplyLayout apply_layout(const Plan* p, int nb, int precision) {
+ApplyLayout apply_layout(const Plan* p, int nb, int precision, bool chunked) {
   ApplyLayout L{};
   const size_t e = csize(precision);
   const size_t nbt = (size_t)p->nk * nb;
   L.fused = plane_fits_lds(p, precision);
   size_t off = 0;
   L.box_off = off;
-  L.box_bytes = nbt * p->na * p->n[1] * p->n[2] * e;
+  const int chunk = chunked ? g_band_chunk[precision == GW_FP64 ? 0 : 1] : 0;
+  const size_t nbt_box = (size_t)p->nk * (chunk > 0 && chunk < nb ? chunk : nb);
+  L.box_bytes = nbt_box * p->na * p->n[1] * p->n[2] * e;
   off = align256(off + L.box_bytes);
   L.stick_off = off;
-  L.stick_bytes = L.fused ? 0 : nbt * p->nst_max * p->n[2] * e;
+  L.stick_bytes = L.fused ? 0 : nbt_box * p->nst_max * p->n[2] * e;
   off = align256(off + L.stick_bytes);
   L.becp_off = off;
   L.becp_bytes = (size_t)p->nk * p->nproj * nb * e;
@@ -1541,6 +1548,12 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
     set_combine_lib(arg, value);
     return GW_OK;
   }
+  if (key == GW_TUNE_BAND_CHUNK) {
+    GW_REQUIRE(arg == GW_FP64 || arg == GW_FP32, GW_ERR_ARG, "band chunk: bad precision %d", arg);
+    GW_REQUIRE(value >= 0, GW_ERR_ARG, "band chunk %d < 0", value);
+    g_band_chunk[arg == GW_FP64 ? 0 : 1] = value;
+    return GW_OK;
+  }
   if (key == GW_TUNE_INNER_LIB) {
     GW_REQUIRE(arg == GW_FP64 || arg == GW_FP32, GW_ERR_ARG, "bad precision %d", arg);
     GW_REQUIRE(value == 0 || value == 1, GW_ERR_ARG, "bad inner-product choice %d", value);
@@ -1568,6 +1581,7 @@ extern "C" int gw_tune_get(int key, int arg) {
   if (key == GW_TUNE_XPASS) return xpass_encode(g_xpass[xp_index(arg == GW_FP64 ? GW_FP64 : GW_FP32)]);
   if (key == GW_TUNE_COMBINE_LIB) return (arg == GW_FP64 || arg == GW_FP32) ? combine_lib(arg) : -1;
   if (key == GW_TUNE_INNER_LIB) return (arg == GW_FP64 || arg == GW_FP32) ? inner_lib(arg) : -1;
+  if (key == GW_TUNE_BAND_CHUNK) return (arg == GW_FP64 || arg == GW_FP32) ? g_band_chunk[arg == GW_FP64 ? 0 : 1] : -1;
   return -1;
 }
 extern "C" int gw_version(void) { return 1; }
@@ -1885,7 +1899,7 @@ extern "C" int gw_plan_set_projectors(gw_plan* h, int nproj, const void* p, cons
 
 extern "C" size_t gw_hpsi_workspace_bytes(const gw_plan* h, int nb, int precision) {
   if (!h || nb <= 0) return 0;
-  return apply_layout(&h->p, nb, precision).total;
+  return apply_layout(&h->p, nb, precision, true).total;
 }
 
 // The generic-size local term (any box length the plan splits): x-plane inverse
@@ -1972,7 +1986,7 @@ static int apply_t(gw_plan* h, int terms, const cpx<T>* c, long ldc, long sc, cp
                    long so, int nb, unsigned char* ws, hipStream_t st) {
   Plan& P = h->p;
   const int prec = sizeof(T) == 8 ? GW_FP64 : GW_FP32;
-  const ApplyLayout L = apply_layout(&P, nb, prec);
+  const ApplyLayout L = apply_layout(&P, nb, prec, true);
   const Geom g = geom_of(&P, nb);
   const int nbt = P.nk * nb;
   const T* t = prec == GW_FP64 ? (const T*)P.t64 : (const T*)P.t32;
@@ -1987,19 +2001,28 @@ static int apply_t(gw_plan* h, int terms, const cpx<T>* c, long ldc, long sc, cp
   hipEvent_t ev;
   if (terms & GW_TERM_LOCAL) {
     GW_REQUIRE(P.have_v, GW_ERR_ARG, "local term requested but no potential set");
-    if (wave_path_ok(P, prec)) {
-      const T* vn = prec == GW_FP64 ? (const T*)P.vx64 : (const T*)P.vx32;
-      int rc = launch_wave_path<T>(P, g, nbt, c, ldc, sc, box, vn, t, out, ldo, so, scale, kin, tm, st);
+    const int chunk0 = g_band_chunk[prec == GW_FP64 ? 0 : 1];
+    const int chunk = chunk0 > 0 && chunk0 < nb ? chunk0 : nb;
+    for (int b0 = 0; b0 < nb; b0 += chunk) {  // band chunks (one pass sequence each)
+      const int nbc = std::min(chunk, nb - b0);
+      const Geom gc = geom_of(&P, nbc);
+      const cpx<T>* cc = c + (long)b0 * ldc;
+      cpx<T>* oc = out + (long)b0 * ldo;
+      tm.count_calls = b0 == 0;
+      int rc;
+      if (wave_path_ok(P, prec)) {
+        const T* vn = prec == GW_FP64 ? (const T*)P.vx64 : (const T*)P.vx32;
+        rc = launch_wave_path<T>(P, gc, P.nk * nbc, cc, ldc, sc, box, vn, t, oc, ldo, so, scale, kin, tm, st);
+      } else {
+        rc = plan_rmax(&P) <= 10 ? generic_local<T, 10>(P, L, gc, P.nk * nbc, cc, ldc, sc, box, vp, t, oc, ldo, so,
+                                                        scale, kin, ws, tm, st)
+             : plan_rmax(&P) <= 16 ? generic_local<T, 16>(P, L, gc, P.nk * nbc, cc, ldc, sc, box, vp, t, oc, ldo,
+                                                          so, scale, kin, ws, tm, st)
+                                   : generic_local<T, kRmaxF3>(P, L, gc, P.nk * nbc, cc, ldc, sc, box, vp, t, oc,
+                                                               ldo, so, scale, kin, ws, tm, st);
+      }
+      tm.count_calls = true;
       if (rc) return rc;
-    } else {
-      const int rc = plan_rmax(&P) <= 10 ? generic_local<T, 10>(P, L, g, nbt, c, ldc, sc, box, vp, t, out, ldo, so,
-                                                                 scale, kin, ws, tm, st)
-                     : plan_rmax(&P) <= 16 ? generic_local<T, 16>(P, L, g, nbt, c, ldc, sc, box, vp, t, out, ldo,
-                                                                  so, scale, kin, ws, tm, st)
-                                           : generic_local<T, kRmaxF3>(P, L, g, nbt, c, ldc, sc, box, vp, t, out,
-                                                                       ldo, so, scale, kin, ws, tm, st);
-      if (rc) return rc;
-
     }
   } else if (kin) {
     hipLaunchKernelGGL(k_kinetic<T>, dim3(grid_blocks(P.npw_max, 256), nbt), dim3(256), 0, st, nb, P.npw_max,
@@ -2040,7 +2063,7 @@ extern "C" int gw_hpsi_apply_strided(gw_plan* h, int precision, int terms, const
                                      size_t ws_bytes, void* stream) {
   GW_REQUIRE(h && c && out && nb > 0, GW_ERR_ARG, "bad argument");
   GW_REQUIRE(precision == GW_FP64 || precision == GW_FP32, GW_ERR_ARG, "bad precision");
-  const ApplyLayout L = apply_layout(&h->p, nb, precision);
+  const ApplyLayout L = apply_layout(&h->p, nb, precision, true);
   GW_REQUIRE(ws && ws_bytes >= L.total, GW_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, L.total);
   GW_REQUIRE(ldc >= h->p.npw_max && ldo >= h->p.npw_max, GW_ERR_ARG, "row stride < npw_max");
   if (precision == GW_FP64)
@@ -2085,7 +2108,7 @@ extern "C" int gw_hpsi_apply_noalloc(gw_plan* h, int precision, int terms, const
   GW_REQUIRE(precision == GW_FP64 || precision == GW_FP32, GW_ERR_ARG, "bad precision");
   // contractions on the kernels here only (rocBLAS handles and workspaces allocate)
   const int prev = set_kernels_only(1);
-  const size_t need = apply_layout(&h->p, nb, precision).total;
+  const size_t need = apply_layout(&h->p, nb, precision, true).total;
   int rc = GW_OK;
   if (need > kNoAllocScratchBytes) {
     set_error("no-allocation apply: %d bands need %zu bytes of scratch, above the fixed maximum of %zu "

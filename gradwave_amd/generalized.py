@@ -12,8 +12,9 @@ not positive definite (a 1e-10 shift as the last resort), L^-1 H L^-H and its
 eigensolve, the back-solve with L^H, the Ritz rotation of the cached H and S
 applies, residuals H x - e S x, the same worst-residual expansion count on
 every k, the Teter-preconditioned unit-normalised directions (one kernel), and
-the QR restart in the standard metric. Norm-conserving operator pieces only:
-DFT+U projectors, the dual grid and meta-GGA terms are refused.
+the QR restart in the standard metric. H carries the USPP/PAW dual grid and a
+DFT+U term with a real D when given (as BatchedHamiltonian does); the
+meta-GGA tau term is refused.
 """
 
 from __future__ import annotations
@@ -35,8 +36,8 @@ class BatchedHS:
 
     def __init__(self, bk, shape, v_eff_r: torch.Tensor, p: torch.Tensor, dscr: torch.Tensor,
                  q_full: torch.Tensor, hub_sphi=None, hub_d=None, smooth=None, metagga_v=None) -> None:
-        if hub_sphi is not None or hub_d is not None or smooth is not None or metagga_v is not None:
-            raise NotImplementedError("DFT+U, the dual grid and meta-GGA terms are not part of this operator")
+        if metagga_v is not None:
+            raise NotImplementedError("the meta-GGA tau term is not part of this operator")
         if torch.is_complex(q_full):
             if float(q_full.imag.abs().max()) != 0.0:
                 raise ValueError("q_full must be real symmetric")
@@ -47,10 +48,13 @@ class BatchedHS:
         self.t = bk.t
         hb = copy.copy(bk)
         hb.dij_full = dscr
-        self.ham = BatchedHamiltonian(hb, self.shape, v_eff_r, p)
+        # H with the dual grid and the DFT+U term as the reference builds it (uspp_batch.py:37-41)
+        self.ham = BatchedHamiltonian(hb, self.shape, v_eff_r, p, hub_q=hub_sphi, hub_dij=hub_d, smooth=smooth)
+        self.shape = self.ham.shape
         sb = copy.copy(bk)
         sb.dij_full = q_full
-        self._aug = BatchedHamiltonian(sb, self.shape, v_eff_r, p)
+        # S needs only the projector contractions (its local-term geometry is H's, unused)
+        self._aug = BatchedHamiltonian(sb, shape, v_eff_r, p, smooth=smooth)
 
     def h(self, c: torch.Tensor) -> torch.Tensor:
         return self.ham.apply(c)

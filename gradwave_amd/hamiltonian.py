@@ -103,32 +103,53 @@ class BatchedHamiltonian:
 
     ``bk`` needs the BatchedK fields npw, mask, flat_idx, t and dij_full
     (core/batch.py:126-171); ``v_eff_r`` (n1,n2,n3) real and ``p``
-    (nk, nproj, npw_max) complex live on the GPU. DFT+U projectors and the
-    USPP dual grid (batch.py:256-266) are outside this operator.
+    (nk, nproj, npw_max) complex live on the GPU.
+
+    ``smooth = (shape_s, flat_idx_s, v_s)`` runs the local term on that smooth
+    box with its own sphere map (the USPP/PAW dual grid, batch.py:258-266; the
+    kinetic and nonlocal terms are sphere-based and unchanged), and then
+    ``shape`` / ``v_eff_r`` are the smooth box's, as in the reference.
+    ``hub_q`` (nk, nproj_U, npw_max) with a real ``hub_dij`` (nproj_U, nproj_U)
+    adds the DFT+U term (batch.py:272-274, 501-504): the plan carries the KB and
+    Hubbard projectors stacked, with D block-diagonal, so both run in the same
+    MFMA contractions. A complex hub_dij is refused.
     """
 
     def __init__(self, bk, shape, v_eff_r: torch.Tensor, p: torch.Tensor, hub_q=None,
                  hub_dij=None, smooth=None, toeplitz: str | None = None) -> None:
-        if hub_q is not None or hub_dij is not None or smooth is not None:
-            raise NotImplementedError("DFT+U and the USPP dual grid are not part of this operator")
         mode = (toeplitz or _TOEPLITZ_MODE).strip().lower()
         if mode not in ("auto", "on", "off"):
             raise ValueError(f"toeplitz mode must be auto, on or off, got {mode!r}")
         L.require_gpu(bk.mask, "bk.mask")
         self.bk = bk
+        flat_idx = bk.flat_idx
+        if smooth is not None:
+            shape, flat_idx, v_eff_r = smooth
         self.shape = tuple(int(s) for s in shape)
         self.n = self.shape[0] * self.shape[1] * self.shape[2]
         self.v_eff_r = v_eff_r
         self.p = p
-        self.plan = Plan(self.shape, bk.flat_idx, bk.npw)
+        self.hub_q, self.hub_dij = hub_q, hub_dij
+        proj, dij = p, bk.dij_full
+        if hub_q is not None and hub_dij is not None:
+            hd = hub_dij
+            if torch.is_complex(hd):
+                if float(hd.imag.abs().max()) != 0.0:
+                    raise NotImplementedError("a complex DFT+U D matrix is not supported (real D only)")
+                hd = hd.real
+            proj = torch.cat([p, hub_q.to(p.dtype)], dim=1)
+            dij = torch.block_diag(bk.dij_full.to(torch.float64), hd.to(bk.dij_full.device, torch.float64))
+        self.plan = Plan(self.shape, flat_idx, bk.npw)
         self.plan.set_kinetic(bk.t)
         self.plan.set_potential(v_eff_r)
-        self.plan.set_projectors(p, bk.dij_full)
+        self.plan.set_projectors(proj, dij)
         self._ws: dict[int, torch.Tensor] = {}
         self.device = bk.mask.device
         nk, npw_max = self.plan.nk, self.plan.npw_max
         self._toep_mode = mode
-        self._toep_eligible = mode != "off" and nk * npw_max * npw_max * 16 <= _TOEPLITZ_M_BUDGET_BYTES
+        # the Toeplitz path reads the dense sphere map (ineligible with a dual grid, batch.py:288-296)
+        self._toep_eligible = (mode != "off" and smooth is None
+                               and nk * npw_max * npw_max * 16 <= _TOEPLITZ_M_BUDGET_BYTES)
         self._toep_M: dict[torch.dtype, torch.Tensor] = {}
 
     def _workspace(self, nb: int, prec: int) -> torch.Tensor:
@@ -279,7 +300,8 @@ class BatchedHamiltonian:
         c = c.contiguous()
         nk, nb, _ = c.shape
         psi = torch.empty(nk, nb, *self.shape, dtype=c.dtype, device=c.device)
-        ws = self._workspace(nb, prec)
+        need = int(L.lib().gw_wfc_to_real_workspace_bytes(self.plan.handle, nb, prec))
+        ws = torch.empty(max(need, 256), dtype=torch.uint8, device=c.device)
         L.check(L.lib().gw_wfc_to_real(self.plan.handle, prec, c.data_ptr(), psi.data_ptr(), nb,
                                        ws.data_ptr(), ws.numel(), L.stream_ptr(c.device)),
                 "gw_wfc_to_real")

@@ -256,11 +256,15 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
  *   10416 for FP32.
  * GW_TUNE_COMBINE_LIB: for precision `arg`, 1 runs gw_zgemm_combine (and the
  *   back-projection) on rocBLAS strided-batched GEMM, 0 on the kernels here (default).
+ * GW_TUNE_BAND_CHUNK: for precision `arg`, bands per chunk of the local term of
+ *   gw_hpsi_apply (0, the default: the whole block); each chunk runs the three
+ *   passes back to back on a box slice of that many bands, which also sets the
+ *   box part of gw_hpsi_workspace_bytes.
  * GW_TUNE_INNER_LIB: for precision `arg`, 1 runs the K-chunks of gw_zgemm_inner (and
  *   of the projection <beta|psi>) as one rocBLAS strided-batched GEMM whose partials the
  *   FP64 split-K reduction here sums, 0 the MFMA kernels here (default). */
 enum { GW_TUNE_RADIX = 1, GW_TUNE_GEMM_WGS = 2, GW_TUNE_GEMM_TILE = 3, GW_TUNE_XPASS = 4,
-       GW_TUNE_COMBINE_LIB = 5, GW_TUNE_INNER_LIB = 6 };
+       GW_TUNE_COMBINE_LIB = 5, GW_TUNE_INNER_LIB = 6, GW_TUNE_BAND_CHUNK = 7 };
 int gw_tune_set(int key, int arg, int value);
 int gw_tune_get(int key, int arg);
 

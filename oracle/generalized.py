@@ -29,14 +29,16 @@ from oracle.hamiltonian import BatchedHamiltonian, becp_b
 
 
 class BatchedHS:
-    """uspp_batch.py:27-72 (no DFT+U, dual grid or meta-GGA terms)."""
+    """uspp_batch.py:27-72 (the dual grid and DFT+U through BatchedHamiltonian;
+    no meta-GGA term)."""
 
-    def __init__(self, bk, shape, v_eff_r, p, dscr, q_full) -> None:
+    def __init__(self, bk, shape, v_eff_r, p, dscr, q_full, hub_sphi=None, hub_d=None, smooth=None) -> None:
         self.bk = bk
         self.shape = shape
         self.p = p
         self.q = q_full.to(p.dtype)
-        self.ham = BatchedHamiltonian(dataclasses.replace(bk, dij_full=dscr), shape, v_eff_r, p)
+        self.ham = BatchedHamiltonian(dataclasses.replace(bk, dij_full=dscr), shape, v_eff_r, p,
+                                      hub_q=hub_sphi, hub_dij=hub_d, smooth=smooth)
         self.t = bk.t
 
     def h(self, c: torch.Tensor) -> torch.Tensor:

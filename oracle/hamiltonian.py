@@ -206,14 +206,25 @@ class BatchedHamiltonian:
     Padded plane-wave slots scatter into a trash slot one past the box
     (batch.py:277-279); fftn(ifftn(.)) is norm-neutral (batch.py:422-424)."""
 
-    def __init__(self, bk: BatchedK, shape, v_eff_r: torch.Tensor, p: torch.Tensor) -> None:
+    def __init__(self, bk: BatchedK, shape, v_eff_r: torch.Tensor, p: torch.Tensor, hub_q=None, hub_dij=None,
+                 smooth=None) -> None:
+        """batch.py:249-279: ``smooth = (shape, flat_idx, v_eff_r)`` runs the local
+        term on that (smooth) box with its own sphere map (the USPP/PAW dual
+        grid, 258-266); ``hub_q`` (nk, nproj_U, npw_max) and ``hub_dij``
+        (nproj_U, nproj_U) add the DFT+U term hub_q^H-contraction, as the KB term
+        (272-274, 501-504)."""
         self.bk = bk
+        if smooth is not None:
+            shape, flat_idx, v_eff_r = smooth
+        else:
+            flat_idx = bk.flat_idx
         self.shape = tuple(shape)
         self.n = shape[0] * shape[1] * shape[2]
         self.v_eff_r = v_eff_r
-        self.gather_idx = bk.flat_idx
+        self.gather_idx = flat_idx
         self.p = p
-        self.idx_scatter = torch.where(bk.mask, bk.flat_idx, torch.full_like(bk.flat_idx, self.n))
+        self.hub_q, self.hub_dij = hub_q, hub_dij
+        self.idx_scatter = torch.where(bk.mask, flat_idx, torch.full_like(flat_idx, self.n))
         self._tab_cache: dict = {}
 
     def _tables(self, cdtype):
@@ -246,6 +257,10 @@ class BatchedHamiltonian:
         if p.shape[1]:
             b = becp_b(p, c, p_conj)
             out = out + torch.einsum("kbp,pq,kqg->kbg", b, dij, p)
+        if self.hub_q is not None and self.hub_dij is not None:  # batch.py:501-504
+            hq = self.hub_q.to(c.dtype)
+            bh = torch.einsum("kpg,kbg->kbp", hq.conj(), c)
+            out = out + torch.einsum("kbp,pq,kqg->kbg", bh, self.hub_dij.to(c.dtype), hq)
         return out * self.bk.mask[:, None, :]
 
     def apply_cols(self, c: torch.Tensor, kcol: torch.Tensor) -> torch.Tensor:
@@ -262,6 +277,10 @@ class BatchedHamiltonian:
             ps, ps_conj = p[kcol], p_conj[kcol]
             b = torch.einsum("cpg,cg->cp", ps_conj, c)
             out = out + torch.einsum("cp,pq,cqg->cg", b, dij, ps)
+        if self.hub_q is not None and self.hub_dij is not None:  # batch.py:543-546
+            hq = self.hub_q.to(c.dtype)[kcol]
+            bh = torch.einsum("cpg,cg->cp", hq.conj(), c)
+            out = out + torch.einsum("cp,pq,cqg->cg", bh, self.hub_dij.to(c.dtype), hq)
         return out * self.bk.mask[kcol]
 
 

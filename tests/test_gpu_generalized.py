@@ -80,6 +80,72 @@ def test_refuses_unsupported_terms():
 
     s, cpu, gpu, p, q, *_ = _cases()
     with pytest.raises(NotImplementedError):
-        BatchedHS(gpu, s.shape, s.v_r.to(dev()), p.to(dev()), cpu.dij_full, q, hub_d=torch.zeros(1))
+        BatchedHS(gpu, s.shape, s.v_r.to(dev()), p.to(dev()), cpu.dij_full, q, metagga_v=torch.zeros(1))
+    hq = torch.zeros(len(SEL), 2, cpu.mask.shape[1], dtype=torch.complex128, device=dev())
+    with pytest.raises(NotImplementedError):
+        BatchedHS(gpu, s.shape, s.v_r.to(dev()), p.to(dev()), cpu.dij_full, q, hub_sphi=hq,
+                  hub_d=torch.eye(2, dtype=torch.complex128) * 1j)
     with pytest.raises(ValueError):
         BatchedHS(gpu, s.shape, s.v_r.to(dev()), p.to(dev()), cpu.dij_full, q.to(torch.complex128) * 1j)
+
+
+def _dual_grid_case():
+    """Si2 at two k-points in a 36^3 dense box with the local term on a 24^3
+    smooth box (its own sphere map and a smooth V), plus a DFT+U term of five
+    seeded unit projectors with a real symmetric D."""
+    from oracle import systems as S
+
+    cell = S.fcc_primitive(S.A_SI)
+    pos = np.array([[0.0, 0.0, 0.0], [0.25 * S.A_SI] * 3])
+    k = np.array([[0.125, 0.25, -0.125], [0.375, 0.0, 0.125]])
+    small = S.build_system("s24", cell, pos, [S.SI, S.SI], 30.0, k, np.full(2, 0.5), 8, shape=(24, 24, 24))
+    dense = S.build_system("s36", cell, pos, [S.SI, S.SI], 30.0, k, np.full(2, 0.5), 8, shape=(36, 36, 36))
+    g = torch.Generator().manual_seed(31)
+    hub_q = torch.randn(2, 5, dense.bk.npw_max, dtype=torch.complex128, generator=g) * dense.bk.mask[:, None, :]
+    hub_q = hub_q / torch.linalg.norm(hub_q, dim=-1, keepdim=True)  # unit projectors: a term of a few eV
+    a = torch.randn(5, 5, dtype=torch.float64, generator=g)
+    hub_d = 0.3 * (a + a.T)
+    return small, dense, hub_q, hub_d
+
+
+def test_dual_grid_and_hubbard_term_match_oracle():
+    """BatchedHamiltonian(..., hub_q, hub_dij, smooth) (core/batch.py:249-279,
+    501-504) against the oracle's, FP64 1e-12 / FP32 2e-5, and the generalized
+    Davidson on that H (scf/uspp_batch.py:37-41) against the oracle's solve."""
+    from types import SimpleNamespace
+
+    from gradwave_amd.generalized import BatchedHS, davidson_gen_batched
+    from gradwave_amd.hamiltonian import BatchedHamiltonian
+    from oracle.generalized import BatchedHS as RefHS
+    from oracle.generalized import davidson_gen_batched as ref_gen
+    from oracle.hamiltonian import BatchedHamiltonian as RefH
+
+    say("dual grid + DFT+U term vs oracle")
+    small, dense, hub_q, hub_d = _dual_grid_case()
+    d = dev()
+    bk = dense.bk
+    gpu = SimpleNamespace(npw=bk.npw.to(d), mask=bk.mask.to(d), flat_idx=bk.flat_idx.to(d), t=bk.t.to(d),
+                          dij_full=bk.dij_full.to(d))
+    smooth_cpu = ((24, 24, 24), small.bk.flat_idx, small.v_r)
+    smooth_gpu = ((24, 24, 24), small.bk.flat_idx.to(d), small.v_r.to(d))
+    ref = RefH(bk, dense.shape, dense.v_r, dense.p, hub_q=hub_q, hub_dij=hub_d, smooth=smooth_cpu)
+    h = BatchedHamiltonian(gpu, dense.shape, dense.v_r.to(d), dense.p.to(d), hub_q=hub_q.to(d),
+                           hub_dij=hub_d.to(d), smooth=smooth_gpu)
+    assert h.shape == (24, 24, 24)
+    c = _start(bk.mask, 4, seed=7)
+    want = ref.apply(c)
+    assert float((h.apply(c.to(d)).cpu() - want).abs().max()) < 1e-12 * float(want.abs().max())
+    got32 = h.apply(c.to(torch.complex64).to(d)).cpu().to(torch.complex128)
+    assert float((got32 - want).abs().max()) < 2e-5 * float(want.abs().max())
+    kcol = torch.tensor([1, 0, 1])
+    cols = h.apply_cols(c[kcol, torch.tensor([0, 2, 3])].to(d), kcol.to(d)).cpu()
+    assert float((cols - want[kcol, torch.tensor([0, 2, 3])]).abs().max()) < 1e-12 * float(want.abs().max())
+    # the generalized solve on the same H
+    q = q_blocks(bk.proj_atom_index.numpy(), seed=2)
+    hs = BatchedHS(gpu, dense.shape, dense.v_r.to(d), dense.p.to(d), bk.dij_full, q, hub_sphi=hub_q.to(d),
+                   hub_d=hub_d.to(d), smooth=smooth_gpu)
+    rhs = RefHS(bk, dense.shape, dense.v_r, dense.p, bk.dij_full, q, hub_sphi=hub_q, hub_d=hub_d, smooth=smooth_cpu)
+    x0 = _start(bk.mask, 8)
+    eig, _ = davidson_gen_batched(hs, x0.to(d), 8, tol=1e-9, max_iter=200)
+    want_e, _ = ref_gen(rhs, x0, 8, tol=1e-9, max_iter=200)
+    assert float((eig.cpu() - want_e).abs().max()) / HARTREE_EV < 1e-10

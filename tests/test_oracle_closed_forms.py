@@ -228,3 +228,77 @@ def test_density_integrates_to_weighted_norms(si2):
     want = float((kw[:, None] * occ * (c.abs() ** 2).sum(-1)).sum())
     assert abs(float(rho.sum()) * si2.grid.volume / n - want) < 1e-12 * want
     assert float(rho.min()) >= 0.0
+
+
+def _fourier_interp(v_small, shape_big):
+    """Real v on a small box -> the same band-limited field on a bigger box (zero-padded G)."""
+    import numpy as np
+
+    vg = np.fft.fftn(v_small.numpy()) / v_small.numel()
+    n, m = v_small.shape, shape_big
+    big = np.zeros(m, dtype=complex)
+    freqs = [np.fft.fftfreq(k, 1.0 / k).astype(int) for k in n]
+    for i, fi in enumerate(freqs[0]):
+        for j, fj in enumerate(freqs[1]):
+            for k, fk in enumerate(freqs[2]):
+                big[fi % m[0], fj % m[1], fk % m[2]] = vg[i, j, k]
+    return torch.as_tensor(np.fft.ifftn(big).real * (m[0] * m[1] * m[2]))
+
+
+def test_dual_grid_local_term_equals_dense_box():
+    """core/batch.py:258-266: the local term on the smooth box (its own sphere
+    map) equals the dense-box term for a V band-limited to the smooth box, since
+    V-hat(G_i - G_j) never aliases when the smooth box holds 2m+1 points per axis."""
+    cell = S.fcc_primitive(S.A_SI)
+    pos = np.array([[0.0, 0.0, 0.0], [0.25 * S.A_SI] * 3])
+    k = np.array([[0.125, 0.25, -0.125], [0.375, 0.0, 0.125]])
+    small = S.build_system("s24", cell, pos, [SI, SI], 30.0, k, np.full(2, 0.5), 4, shape=(24, 24, 24))
+    dense = S.build_system("s36", cell, pos, [SI, SI], 30.0, k, np.full(2, 0.5), 4, shape=(36, 36, 36))
+    assert torch.equal(small.bk.npw, dense.bk.npw)
+    g = torch.Generator().manual_seed(9)
+    v_s = torch.randn(24, 24, 24, dtype=torch.float64, generator=g)
+    vg = torch.fft.fftn(v_s)  # drop the Nyquist planes: the field is then band-limited below 12
+    vg[12], vg[:, 12], vg[:, :, 12] = 0, 0, 0
+    v_s = torch.fft.ifftn(vg).real
+    v_d = _fourier_interp(v_s, (36, 36, 36))
+    c = torch.randn(2, 4, dense.bk.npw_max, dtype=torch.complex128, generator=g) * dense.bk.mask[:, None, :]
+    h_dense = BatchedHamiltonian(dense.bk, dense.shape, v_d, dense.p)
+    h_smooth = BatchedHamiltonian(dense.bk, dense.shape, None, dense.p,
+                                  smooth=((24, 24, 24), small.bk.flat_idx, v_s))
+    want = h_dense.apply(c)
+    assert h_smooth.shape == (24, 24, 24)
+    assert float((h_smooth.apply(c) - want).abs().max()) < 1e-11 * float(want.abs().max())
+    # the smooth box with another k-point's map scrambles the term
+    h_bad = BatchedHamiltonian(dense.bk, dense.shape, None, dense.p,
+                               smooth=((24, 24, 24), torch.roll(small.bk.flat_idx, 1, dims=0), v_s))
+    assert float((h_bad.apply(c) - want).abs().max()) > 1e-5 * float(want.abs().max())
+
+
+def test_hubbard_term_is_a_second_projector_contraction():
+    """core/batch.py:501-504, 543-546: the DFT+U term sum_pq <q_p|c> U_pq q_q equals
+    the KB contraction with the projector sets stacked and D block-diagonal, and
+    matches the dense matrix q^T U conj(q) added to H."""
+    s = S.si2_c1()
+    sel = [0, 5]
+    bk = BatchedK(npw=s.bk.npw[sel], mask=s.bk.mask[sel], flat_idx=s.bk.flat_idx[sel], kpg=s.bk.kpg[sel],
+                  t=s.bk.t[sel], proj_phase_free=s.bk.proj_phase_free[sel], proj_atom_index=s.bk.proj_atom_index,
+                  dij_full=s.bk.dij_full)
+    g = torch.Generator().manual_seed(12)
+    nq = 5
+    hub_q = torch.randn(2, nq, bk.npw_max, dtype=torch.complex128, generator=g) * bk.mask[:, None, :]
+    a = torch.randn(nq, nq, dtype=torch.float64, generator=g)
+    hub_d = a + a.T
+    h = BatchedHamiltonian(bk, s.shape, s.v_r, s.p[sel], hub_q=hub_q, hub_dij=hub_d)
+    c = torch.randn(2, 3, bk.npw_max, dtype=torch.complex128, generator=g) * bk.mask[:, None, :]
+    out = h.apply(c)
+    stacked = BatchedK(**{**bk.__dict__, "dij_full": torch.block_diag(bk.dij_full, hub_d)})
+    h2 = BatchedHamiltonian(stacked, s.shape, s.v_r, torch.cat([s.p[sel], hub_q], dim=1))
+    assert float((out - h2.apply(c)).abs().max()) < 1e-12 * float(out.abs().max())
+    for j, ik in enumerate(sel):
+        n = int(bk.npw[j])
+        hq = hub_q[j, :, :n]
+        hd = dense_h(s, ik) + hq.T @ hub_d.to(torch.complex128).T @ hq.conj()
+        want = c[j, :, :n] @ hd.T
+        assert float((out[j, :, :n] - want).abs().max()) < 1e-10 * float(want.abs().max())
+    cols = h.apply_cols(c[[1, 0], [2, 1]], torch.tensor([1, 0]))
+    assert float((cols - out[[1, 0], [2, 1]]).abs().max()) < 1e-12 * float(out.abs().max())
