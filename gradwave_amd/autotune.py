@@ -147,7 +147,10 @@ def tune_radix(workloads, nb: int = 8, reps: int = 5) -> dict[int, int]:
     return chosen
 
 
-GEMM_TILES = (0, 1, 2, 3, 4, 5)  # MFMA blocks per wave: 2x2, 4x2, 2x4, 4x4, band-sized 2x5/5x2, 4x5/5x4
+# 0-5: expanded-operand tiles (MFMA blocks per wave 2x2, 4x2, 2x4, 4x4, band-sized
+# 2x5/5x2, 4x5/5x4); 6-9: complex-accumulator tiles, four real products per
+# complex one; 10-13: the same with three (Gauss)
+GEMM_TILES = tuple(range(14))
 
 
 def tune_gemm(workloads, nb: int = 32, reps: int = 5, allow_library: bool = True) -> tuple[dict, int]:

@@ -612,6 +612,278 @@ k_gemm_combine_tile(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu,
 }
 
 // ---------------------------------------------------------------------------
+// Complex-accumulator kernels (tile variants 6-13). Each 16 x 16 complex block of
+// the result keeps its own accumulators, and every complex k feeds the MFMAs
+// straight from the (re, im) storage of both operands: nothing is expanded on
+// the fly. A workgroup (2 x 2 waves of BM x BN blocks) covers 32 BM x 32 BN
+// complex results; both operands are staged through two padded LDS buffers in
+// chunks of KC complex along K, with one barrier per chunk and the next chunk's
+// global loads in flight during the current chunk's MFMAs. With a = a' + i a''
+// (a'' = -Im a for conj(a), +Im a otherwise) and b = b' + i b'':
+//   four real products (G3 = false): re += a'b' - a''b'', im += a'b'' + a''b'
+//   three real products (G3 = true, Gauss): t1 += a'b', t2 += a''b'',
+//     t3 += (a' + a'')(b' + b''); re = t1 - t2, im = t3 - t1 - t2
+// The three-product form does 3/4 of the MFMA work for two VALU adds per
+// operand value; its rounding error stays within a small multiple of the
+// four-product form's (both sum |a||b| terms in the accumulation type).
+template <typename T> struct CTileCfg;
+// FP64: 8 complex per chunk row (128 B); odd row stride -> conflict-free column reads
+template <> struct CTileCfg<double> { static constexpr int KC = 8, PADK = 1, PADM = 0; };
+// FP32: 16 complex per chunk row (128 B); row stride KC + 2 / M-rows padded by 16 complex
+template <> struct CTileCfg<float> { static constexpr int KC = 16, PADK = 2, PADM = 16; };
+
+template <bool G3> constexpr int kNacc = G3 ? 3 : 2;
+
+template <typename T, int BM, int BN, bool CONJ_A, bool G3>
+__device__ __forceinline__ void cmma_step(const cpx<T> (&a)[BM], const cpx<T> (&b)[BN],
+                                          typename Mfma<T>::acc (&acc)[kNacc<G3>][BM][BN]) {
+  if constexpr (G3) {
+    T bs[BN];
+#pragma unroll
+    for (int j = 0; j < BN; ++j) bs[j] = b[j].x + b[j].y;
+#pragma unroll
+    for (int i = 0; i < BM; ++i) {
+      const T a2 = CONJ_A ? -a[i].y : a[i].y, as = a[i].x + a2;
+#pragma unroll
+      for (int j = 0; j < BN; ++j) {
+        acc[0][i][j] = Mfma<T>::op(a[i].x, b[j].x, acc[0][i][j]);
+        acc[1][i][j] = Mfma<T>::op(a2, b[j].y, acc[1][i][j]);
+        acc[2][i][j] = Mfma<T>::op(as, bs[j], acc[2][i][j]);
+      }
+    }
+  } else {
+    // the a' products of every block first, then the a'' ones: 2 BM BN
+    // independent MFMAs separate the two updates of one accumulator
+#pragma unroll
+    for (int i = 0; i < BM; ++i)
+#pragma unroll
+      for (int j = 0; j < BN; ++j) {
+        acc[0][i][j] = Mfma<T>::op(a[i].x, b[j].x, acc[0][i][j]);
+        acc[1][i][j] = Mfma<T>::op(a[i].x, b[j].y, acc[1][i][j]);
+      }
+#pragma unroll
+    for (int i = 0; i < BM; ++i) {
+      const T ai = a[i].y, nai = -a[i].y;
+#pragma unroll
+      for (int j = 0; j < BN; ++j) {
+        acc[0][i][j] = Mfma<T>::op(CONJ_A ? ai : nai, b[j].y, acc[0][i][j]);
+        acc[1][i][j] = Mfma<T>::op(CONJ_A ? nai : ai, b[j].x, acc[1][i][j]);
+      }
+    }
+  }
+}
+
+// (re, im) of register r of block (i, j) from the accumulators
+template <typename T, int BM, int BN, bool G3>
+__device__ __forceinline__ cpx<T> cmma_result(const typename Mfma<T>::acc (&acc)[kNacc<G3>][BM][BN], int i, int j,
+                                              int r) {
+  if constexpr (G3) {
+    const T t1 = acc[0][i][j][r], t2 = acc[1][i][j][r];
+    return cpx<T>{t1 - t2, acc[2][i][j][r] - t1 - t2};
+  } else {
+    return cpx<T>{acc[0][i][j][r], acc[1][i][j][r]};
+  }
+}
+
+// Inner product C = conj(X) Y^T over split s of K (both operands K-contiguous,
+// LDS rows [m][k]); partials [k][split][M][N] complex in the compute type.
+template <typename T, int BM, int BN, bool G3>
+__global__ void __launch_bounds__(256)
+k_cgemm_inner(int M, int N, int K, const cpx<T>* __restrict__ X, long ldx, long sx,
+              const cpx<T>* __restrict__ Y, long ldy, long sy, int splits, int kchunk,
+              T* __restrict__ part, int ntx, int nty, long total) {
+  using A4 = typename Mfma<T>::acc;
+  constexpr int KC = CTileCfg<T>::KC, LD = KC + CTileCfg<T>::PADK;
+  constexpr int TM = 32 * BM, TN = 32 * BN;
+  constexpr int NA = TM * KC / 256, NB = TN * KC / 256;
+  static_assert(NA * 256 == TM * KC && NB * 256 == TN * KC, "tile / chunk shape");
+  __shared__ cpx<T> As[2][TM * LD];
+  __shared__ cpx<T> Bs[2][TN * LD];
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const int bx = (int)(wid % ntx), by = (int)((wid / ntx) % nty), bz = (int)(wid / ((long)ntx * nty));
+  const int k = bz / splits, sp = bz - k * splits;
+  const int gb = sp * kchunk, ge = min(K, gb + kchunk);
+  const int row0 = by * TM, col0 = bx * TN;
+  const cpx<T>* Xk = X + (long)k * sx;
+  const cpx<T>* Yk = Y + (long)k * sy;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = w >> 1, wn = w & 1;
+  cpx<T> ra[NA], rb[NB];
+  // every lane loads from a clamped (valid) address and zeroes what lies outside,
+  // so the loop body has no branches
+  auto gload = [&](int g0) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int e = tid + 256 * i, g = g0 + e % KC, row = row0 + e / KC;
+      const cpx<T> v = Xk[(long)min(row, M - 1) * ldx + min(g, ge - 1)];
+      ra[i] = (row < M && g < ge) ? v : cpx<T>{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int e = tid + 256 * i, g = g0 + e % KC, row = col0 + e / KC;
+      const cpx<T> v = Yk[(long)min(row, N - 1) * ldy + min(g, ge - 1)];
+      rb[i] = (row < N && g < ge) ? v : cpx<T>{0, 0};
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) { const int e = tid + 256 * i; As[buf][(e / KC) * LD + e % KC] = ra[i]; }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) { const int e = tid + 256 * i; Bs[buf][(e / KC) * LD + e % KC] = rb[i]; }
+  };
+  A4 acc[kNacc<G3>][BM][BN];
+#pragma unroll
+  for (int q = 0; q < kNacc<G3>; ++q)
+#pragma unroll
+    for (int i = 0; i < BM; ++i)
+#pragma unroll
+      for (int j = 0; j < BN; ++j) acc[q][i][j] = A4{0, 0, 0, 0};
+  const int nch = ge > gb ? (ge - gb + KC - 1) / KC : 0;
+  auto compute = [&](int buf) {
+    const cpx<T>* as = As[buf] + (wm * 16 * BM + li) * LD + lk;
+    const cpx<T>* bs = Bs[buf] + (wn * 16 * BN + li) * LD + lk;
+#pragma unroll
+    for (int s = 0; s < KC / 4; ++s) {
+      cpx<T> a[BM], b[BN];
+#pragma unroll
+      for (int i = 0; i < BM; ++i) a[i] = as[i * 16 * LD + 4 * s];
+#pragma unroll
+      for (int j = 0; j < BN; ++j) b[j] = bs[j * 16 * LD + 4 * s];
+      cmma_step<T, BM, BN, true, G3>(a, b, acc);
+    }
+  };
+  if (nch > 0) {
+    gload(gb);
+    sstore(0);
+    __syncthreads();
+    // steady state without conditionals; the last chunk is peeled
+    for (int c = 0; c + 1 < nch; ++c) {
+      gload(gb + (c + 1) * KC);
+      compute(c & 1);
+      sstore((c + 1) & 1);
+      __syncthreads();
+    }
+    compute((nch - 1) & 1);
+  }
+  cpx<T>* pk = reinterpret_cast<cpx<T>*>(part) + ((long)k * splits + sp) * M * N;
+#pragma unroll
+  for (int i = 0; i < BM; ++i)
+#pragma unroll
+    for (int j = 0; j < BN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = row0 + wm * 16 * BM + i * 16 + Mfma<T>::row(lk, r);
+        const int col = col0 + wn * 16 * BN + j * 16 + li;
+        if (row < M && col < N) pk[(long)row * N + col] = cmma_result<T, BM, BN, G3>(acc, i, j, r);
+      }
+}
+
+// Combine Z = beta Z + U'^T Y (U' = U or conj U, both operands stored [j][.],
+// LDS rows [j][a] and [j][g]); the whole J per workgroup.
+template <typename T, int BM, int BN, bool G3, bool CONJ>
+__global__ void __launch_bounds__(256)
+k_cgemm_combine(int A, int G, int J, const cpx<T>* __restrict__ U, long ldu, long su,
+                const cpx<T>* __restrict__ Y, long ldy, long sy, cpx<T>* __restrict__ Z, long ldz, long sz,
+                T beta, int ntx, int nty, long total) {
+  using A4 = typename Mfma<T>::acc;
+  constexpr int KC = CTileCfg<T>::KC;
+  constexpr int TA = 32 * BM, TG = 32 * BN;
+  constexpr int LDU = TA + CTileCfg<T>::PADM, LDY = TG + CTileCfg<T>::PADM;
+  constexpr int NU = TA * KC / 256, NY = TG * KC / 256;
+  static_assert(NU * 256 == TA * KC && NY * 256 == TG * KC, "tile / chunk shape");
+  __shared__ cpx<T> Us[2][KC * LDU];
+  __shared__ cpx<T> Ys[2][KC * LDY];
+  const long wid = gemm_work_id(total);
+  if (wid < 0) return;
+  const int by = (int)(wid % nty), bx = (int)((wid / nty) % ntx), k = (int)(wid / ((long)ntx * nty));
+  const int a0 = by * TA, g0 = bx * TG;
+  const cpx<T>* Uk = U + (long)k * su;
+  const cpx<T>* Yk = Y + (long)k * sy;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int wm = w >> 1, wn = w & 1;
+  cpx<T> ru[NU], ry[NY];
+  // clamped addresses, zeroed outside: no branches in the loop body
+  auto gload = [&](int j0) {
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+      const int e = tid + 256 * i, j = j0 + e / TA, a = a0 + e % TA;
+      const cpx<T> v = Uk[(long)min(j, J - 1) * ldu + min(a, A - 1)];
+      ru[i] = (j < J && a < A) ? v : cpx<T>{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < NY; ++i) {
+      const int e = tid + 256 * i, j = j0 + e / TG, g = g0 + e % TG;
+      const cpx<T> v = Yk[(long)min(j, J - 1) * ldy + min(g, G - 1)];
+      ry[i] = (j < J && g < G) ? v : cpx<T>{0, 0};
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NU; ++i) { const int e = tid + 256 * i; Us[buf][(e / TA) * LDU + e % TA] = ru[i]; }
+#pragma unroll
+    for (int i = 0; i < NY; ++i) { const int e = tid + 256 * i; Ys[buf][(e / TG) * LDY + e % TG] = ry[i]; }
+  };
+  A4 acc[kNacc<G3>][BM][BN];
+#pragma unroll
+  for (int q = 0; q < kNacc<G3>; ++q)
+#pragma unroll
+    for (int i = 0; i < BM; ++i)
+#pragma unroll
+      for (int j = 0; j < BN; ++j) acc[q][i][j] = A4{0, 0, 0, 0};
+  const int nch = (J + KC - 1) / KC;
+  auto compute = [&](int buf) {
+    const cpx<T>* us = Us[buf] + lk * LDU + wm * 16 * BM + li;
+    const cpx<T>* ys = Ys[buf] + lk * LDY + wn * 16 * BN + li;
+#pragma unroll
+    for (int s = 0; s < KC / 4; ++s) {
+      cpx<T> a[BM], b[BN];
+#pragma unroll
+      for (int i = 0; i < BM; ++i) a[i] = us[4 * s * LDU + i * 16];
+#pragma unroll
+      for (int j = 0; j < BN; ++j) b[j] = ys[4 * s * LDY + j * 16];
+      cmma_step<T, BM, BN, CONJ, G3>(a, b, acc);
+    }
+  };
+  if (nch > 0) {
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int c = 0; c + 1 < nch; ++c) {  // the last chunk is peeled
+      gload((c + 1) * KC);
+      compute(c & 1);
+      sstore((c + 1) & 1);
+      __syncthreads();
+    }
+    compute((nch - 1) & 1);
+  }
+  cpx<T>* Zk = Z + (long)k * sz;
+#pragma unroll
+  for (int i = 0; i < BM; ++i)
+#pragma unroll
+    for (int j = 0; j < BN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = a0 + wm * 16 * BM + i * 16 + Mfma<T>::row(lk, r);
+        const int col = g0 + wn * 16 * BN + j * 16 + li;
+        if (row < A && col < G) {
+          cpx<T>* zp = Zk + (long)row * ldz + col;
+          cpx<T> v = cmma_result<T, BM, BN, G3>(acc, i, j, r);
+          if (beta != T(0)) {
+            const cpx<T> o = *zp;
+            v.x += beta * o.x;
+            v.y += beta * o.y;
+          }
+          *zp = v;
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
 // Grouped launches: the jobs' work items are concatenated; a workgroup finds its
 // job by binary search over the jobs' first items and runs the single-job body.
 template <typename Job>
@@ -695,7 +967,7 @@ template int gemm_group_combine<float>(const GroupCombine*, int, long, float, hi
 // tile variant per [precision (0 FP64, 1 FP32)][contraction (0 inner, 1 combine)]
 static int g_gemm_tile[2][2] = {{0, 0}, {0, 0}};
 void set_gemm_tile(int precision, int op, int v) {
-  const int t = (v >= 0 && v <= 5) ? v : 0;
+  const int t = (v >= 0 && v <= 13) ? v : 0;
   if (op != 1) g_gemm_tile[precision == GW_FP32][0] = t;
   if (op != 0) g_gemm_tile[precision == GW_FP32][1] = t;
 }
@@ -717,6 +989,27 @@ static int inner_splits(int nk, int M, int N, int K) {
   const long smax = std::max<long>(1, (K + 4 * kStepC - 1) / (4 * kStepC));  // >= 4 groups per split
   s = std::max<long>(1, std::min<long>(s, smax));
   return (int)std::min<long>(s, 4096);
+}
+
+// Workgroup tile (complex rows x complex columns, as 32 BM x 32 BN) of the
+// complex-accumulator variants 6-9 of each contraction.
+// Variants 6-9 use four real products per complex one, 10-13 three.
+static void ctile_shape(int variant, int op, int* bm, int* bn) {
+  static const int inner[8][2] = {{2, 2}, {3, 3}, {2, 5}, {4, 3}, {2, 2}, {3, 2}, {2, 3}, {3, 3}};
+  static const int comb[8][2] = {{2, 2}, {3, 2}, {5, 2}, {3, 4}, {2, 2}, {3, 2}, {2, 3}, {5, 2}};
+  const int v = std::min(std::max(variant, 6), 13) - 6;
+  *bm = op == 0 ? inner[v][0] : comb[v][0];
+  *bn = op == 0 ? inner[v][1] : comb[v][1];
+}
+
+// Split-K count of the complex-accumulator inner product: the workgroup target
+// over the tiles, each split at least 4 chunks long, at most 256 splits.
+template <typename T>
+static int ctile_splits(int nk, int M, int N, int K, int bm, int bn) {
+  const long tiles = (long)((M + 32 * bm - 1) / (32 * bm)) * ((N + 32 * bn - 1) / (32 * bn)) * nk;
+  long s = (g_target_wgs + tiles - 1) / tiles;
+  const long smax = std::max<long>(1, K / (4 * CTileCfg<T>::KC));
+  return (int)std::max<long>(1, std::min<long>(std::min<long>(s, smax), 256));
 }
 
 template <typename T>
@@ -763,6 +1056,11 @@ size_t gemm_inner_partials_bytes(int nk, int M, int N, int K) {
   if (M <= 0 || N <= 0 || K <= 0) return 0;
   int s = inner_splits(nk, M, N, K);
   if (g_inner_lib[sizeof(T) == 4] && !g_kernels_only) s = std::max(s, lib_splits(M, N, K));
+  if (tile_of<T>(0) >= 6) {
+    int bm, bn;
+    ctile_shape(tile_of<T>(0), 0, &bm, &bn);
+    s = std::max(s, ctile_splits<T>(nk, M, N, K, bm, bn));
+  }
   return (size_t)nk * s * M * 2 * N * sizeof(T);
 }
 template size_t gemm_inner_partials_bytes<double>(int, int, int, int);
@@ -836,6 +1134,36 @@ int gemm_inner(int nk, int M, int N, int K, const cpx<T>* X, long ldx, long sx, 
     const long nreal = (long)M * 2 * N;
     hipLaunchKernelGGL(k_gemm_inner_reduce<T>, dim3((unsigned)((nreal + 63) / 64), nk), dim3(256), 0, st, M, N,
                        used, (const T*)partials, C, ldc, sc, 1);
+    GW_CHECK_HIP(hipGetLastError());
+    return GW_OK;
+  }
+  if (tile_of<T>(0) >= 6 && !gemm_simple()) {
+    int bm, bn;
+    ctile_shape(tile_of<T>(0), 0, &bm, &bn);
+    const int cs = ctile_splits<T>(nk, M, N, K, bm, bn);
+    const size_t cneed = (size_t)nk * cs * M * 2 * N * sizeof(T);
+    GW_REQUIRE(part_bytes >= cneed, GW_ERR_WORKSPACE, "gemm_inner workspace %zu < %zu", part_bytes, cneed);
+    int cchunk = (K + cs - 1) / cs;
+    cchunk = ((cchunk + CTileCfg<T>::KC - 1) / CTileCfg<T>::KC) * CTileCfg<T>::KC;
+    const int tx = (N + 32 * bn - 1) / (32 * bn), ty = (M + 32 * bm - 1) / (32 * bm);
+    const long tt = (long)tx * ty * nk * cs;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3((unsigned)(8 * ((tt + 7) / 8))), dim3(256), 0, st, M, N, K, X, ldx, sx, Y, ldy,
+                         sy, cs, cchunk, (T*)partials, tx, ty, tt);
+    };
+    switch (tile_of<T>(0)) {
+      case 6: go(k_cgemm_inner<T, 2, 2, false>); break;
+      case 7: go(k_cgemm_inner<T, 3, 3, false>); break;
+      case 8: go(k_cgemm_inner<T, 2, 5, false>); break;
+      case 9: go(k_cgemm_inner<T, 4, 3, false>); break;
+      case 10: go(k_cgemm_inner<T, 2, 2, true>); break;
+      case 11: go(k_cgemm_inner<T, 3, 2, true>); break;
+      case 12: go(k_cgemm_inner<T, 2, 3, true>); break;
+      default: go(k_cgemm_inner<T, 3, 3, true>); break;
+    }
+    const long nreal = (long)M * 2 * N;
+    hipLaunchKernelGGL(k_gemm_inner_reduce<T>, dim3((unsigned)((nreal + 63) / 64), nk), dim3(256), 0, st, M, N, cs,
+                       (const T*)partials, C, ldc, sc, 0);
     GW_CHECK_HIP(hipGetLastError());
     return GW_OK;
   }
@@ -1026,6 +1354,42 @@ int gemm_combine(int nk, int A, int G, int J, const cpx<T>* U, long ldu, long su
   if (g_combine_lib[sizeof(T) == 4] && !g_kernels_only && J > 0 && ldu < (1L << 31) && ldy < (1L << 31) &&
       ldz < (1L << 31))
     return combine_rocblas<T>(nk, A, G, J, U, ldu, su, conj_u, Y, ldy, sy, Z, ldz, sz, beta, st);
+  if (tile_of<T>(1) >= 6 && !gemm_simple()) {
+    int bm, bn;
+    ctile_shape(tile_of<T>(1), 1, &bm, &bn);
+    const int tx = (G + 32 * bn - 1) / (32 * bn), ty = (A + 32 * bm - 1) / (32 * bm);
+    const long tt = (long)tx * ty * nk;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3((unsigned)(8 * ((tt + 7) / 8))), dim3(256), 0, st, A, G, J, U, ldu, su, Y, ldy,
+                         sy, Z, ldz, sz, beta, tx, ty, tt);
+    };
+    const int v = tile_of<T>(1);
+    if (conj_u) {
+      switch (v) {
+        case 6: go(k_cgemm_combine<T, 2, 2, false, true>); break;
+        case 7: go(k_cgemm_combine<T, 3, 2, false, true>); break;
+        case 8: go(k_cgemm_combine<T, 5, 2, false, true>); break;
+        case 9: go(k_cgemm_combine<T, 3, 4, false, true>); break;
+        case 10: go(k_cgemm_combine<T, 2, 2, true, true>); break;
+        case 11: go(k_cgemm_combine<T, 3, 2, true, true>); break;
+        case 12: go(k_cgemm_combine<T, 2, 3, true, true>); break;
+        default: go(k_cgemm_combine<T, 5, 2, true, true>); break;
+      }
+    } else {
+      switch (v) {
+        case 6: go(k_cgemm_combine<T, 2, 2, false, false>); break;
+        case 7: go(k_cgemm_combine<T, 3, 2, false, false>); break;
+        case 8: go(k_cgemm_combine<T, 5, 2, false, false>); break;
+        case 9: go(k_cgemm_combine<T, 3, 4, false, false>); break;
+        case 10: go(k_cgemm_combine<T, 2, 2, true, false>); break;
+        case 11: go(k_cgemm_combine<T, 3, 2, true, false>); break;
+        case 12: go(k_cgemm_combine<T, 2, 3, true, false>); break;
+        default: go(k_cgemm_combine<T, 5, 2, true, false>); break;
+      }
+    }
+    GW_CHECK_HIP(hipGetLastError());
+    return GW_OK;
+  }
   const int ntx = (2 * G + kTile - 1) / kTile, nty = (A + kTile - 1) / kTile;
   const long total = (long)ntx * nty * nk;
   auto launch_tile = [&](auto kern, int wm, int wn) {

@@ -1562,7 +1562,7 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
   }
   if (key == GW_TUNE_GEMM_TILE) {  // arg = precision + 256 * (0 both, 1 inner, 2 combine)
     const int prec = arg & 255, op = arg >> 8;
-    GW_REQUIRE(value >= 0 && value <= 5, GW_ERR_ARG, "bad tile variant %d", value);
+    GW_REQUIRE(value >= 0 && value <= 13, GW_ERR_ARG, "bad tile variant %d", value);
     GW_REQUIRE(prec == GW_FP64 || prec == GW_FP32, GW_ERR_ARG, "bad precision %d", prec);
     GW_REQUIRE(op >= 0 && op <= 2, GW_ERR_ARG, "bad contraction selector %d", op);
     set_gemm_tile(prec, op == 0 ? -1 : op - 1, value);

@@ -247,7 +247,11 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
  *   2 = 2x4, 3 = 4x4; 4 and 5 cover 160
  *   bands (80 complex columns of the inner product, 160 rows of the
  *   back-projection) per workgroup: 4 = inner 2x5 / combine 5x2, 5 = inner 4x5 /
- *   combine 5x4.
+ *   combine 5x4. 6-13 are the complex-accumulator kernels (workgroup tile
+ *   32 BM x 32 BN complex, double-buffered LDS): 6-9 with four real MFMA
+ *   products per complex one (inner BM x BN = 2x2, 3x3, 2x5, 4x3; combine 2x2,
+ *   3x2, 5x2, 3x4), 10-13 with three (Gauss: inner 2x2, 3x2, 2x3, 3x3; combine
+ *   2x2, 3x2, 2x3, 5x2).
  * GW_TUNE_XPASS: launch shape of the wave-line x pass for precision `arg` (GW_FP64 /
  *   GW_FP32; any other `arg` sets both), value = mode * 10000 + waves per workgroup
  *   (2, 4, 8) * 100 + bands per workgroup (1..64); mode 0 reads the x-lines straight

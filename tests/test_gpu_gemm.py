@@ -124,10 +124,12 @@ def test_real_combine(dtype, nk, A, G, J, alpha, beta):
     assert float((z.cpu().double() - want).abs().max()) < RTOL[dtype] * scale
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", list(range(1, 14)))
 @pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
 def test_tile_variants_match(tile, dtype):
-    """The autotuner's MFMA tile shapes give the same contractions."""
+    """The autotuner's MFMA tile shapes give the same contractions: 1-5 the
+    expanded-operand tiles, 6-9 the complex-accumulator tiles with four real
+    products, 10-13 with three (Gauss); edges, batches, K tails, both combine forms."""
     from gradwave_amd import _lib as L
 
     say(f"gemm tile variant {tile} {dtype}")
@@ -135,20 +137,23 @@ def test_tile_variants_match(tile, dtype):
     prec = L.precision_of(dtype)
     try:
         assert lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, tile) == 0
-        for (nk, M, N, K) in [(1, 256, 160, 37073), (3, 17, 33, 1000), (2, 70, 5, 4097)]:
+        for (nk, M, N, K) in [(1, 256, 160, 37073), (3, 17, 33, 1000), (2, 70, 5, 4097), (1, 97, 130, 9)]:
             x, y = _rand(nk, M, K, seed=1, dtype=dtype), _rand(nk, N, K, seed=2, dtype=dtype)
             want = torch.matmul(x.to(torch.complex128).conj(), y.to(torch.complex128).mT)
             got = LA.inner(x.to(dev()), y.to(dev())).cpu().to(torch.complex128)
             scale = float((x.abs() ** 2).sum(-1).max().sqrt() * (y.abs() ** 2).sum(-1).max().sqrt())
             assert float((got - want).abs().max()) < TOL[dtype] * scale
-        for (nk, A, G, J) in [(1, 160, 37073, 256), (3, 33, 1000, 17), (2, 5, 4097, 70)]:
+        for (nk, A, G, J) in [(1, 160, 37073, 256), (3, 33, 1000, 17), (2, 5, 4097, 70), (1, 97, 130, 9)]:
             u, y = _rand(nk, J, A, seed=3, dtype=dtype), _rand(nk, J, G, seed=4, dtype=dtype)
             z0 = _rand(nk, A, G, seed=5, dtype=dtype)
-            want = torch.matmul(u.to(torch.complex128).conj().mT, y.to(torch.complex128)) + z0.to(torch.complex128)
-            z = z0.to(dev())
-            LA.combine(u.to(dev()), y.to(dev()), out=z, beta=1.0, conj_u=True)
             scale = float(u.abs().pow(2).sum(-2).max().sqrt() * y.abs().pow(2).sum(-2).max().sqrt()) + 1.0
-            assert float((z.cpu().to(torch.complex128) - want).abs().max()) < TOL[dtype] * scale
+            for conj_u, beta in ((True, 1.0), (False, 0.0)):
+                uu = u.to(torch.complex128)
+                uu = uu.conj() if conj_u else uu
+                want = torch.matmul(uu.mT, y.to(torch.complex128)) + beta * z0.to(torch.complex128)
+                z = z0.to(dev())
+                LA.combine(u.to(dev()), y.to(dev()), out=z, beta=beta, conj_u=conj_u)
+                assert float((z.cpu().to(torch.complex128) - want).abs().max()) < TOL[dtype] * scale
     finally:
         lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, prec, 0)
 

@@ -53,7 +53,7 @@ def test_argument_errors_are_reported_without_a_gpu():
 
     assert "unknown tuning key" in err(lib.gw_tune_set(99, 0, 0))
     assert "does not split" in err(lib.gw_tune_set(L.GW_TUNE_RADIX, 24, 5))
-    assert "bad tile" in err(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32, 7))
+    assert "bad tile" in err(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32, 14))
     assert "contraction selector" in err(lib.gw_tune_set(L.GW_TUNE_GEMM_TILE, L.GW_FP32 + 768, 0))
     assert "inner-product choice" in err(lib.gw_tune_set(L.GW_TUNE_INNER_LIB, L.GW_FP32, 2))
     assert lib.gw_tune_get(L.GW_TUNE_INNER_LIB, L.GW_FP64) == 0
