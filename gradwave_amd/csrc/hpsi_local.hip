@@ -905,15 +905,31 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
   const int* src = g.pt_src + (long)k * g.npw_max;
   const int* yz = g.pt_yz + (long)k * g.npw_max;
   const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
-  // The plane's points, one per thread and pass over all threads; the first two
-  // passes are loaded before the LDS set-up so their latency overlaps it.
+  // The plane's points, one per thread and pass over all threads, in batches of
+  // KB passes whose loads are all in flight together (indices first, then the
+  // coefficients they point at, from clamped in-range slots); the first batch is
+  // loaded before the LDS set-up so its latency overlaps it.
   constexpr int NT = 64 * NW;
+  constexpr int KB = NW >= 16 ? 2 : (40 + NW - 1) / NW;
   const int p0 = pp[s0], p1 = pp[s1];
-  const int qa = p0 + threadIdx.x, qb = qa + NT;
-  int va = -1, vb = -1;
-  cpx<T> ca = czero<T>(), cbv = czero<T>();
-  if (qa < p1) { va = yz[qa]; ca = cb[src[qa]]; }
-  if (qb < p1) { vb = yz[qb]; cbv = cb[src[qb]]; }
+  const int q0 = p0 + threadIdx.x;
+  // (an empty plane reads slot 0 of the point tables, always in range, and places nothing)
+  // (values kept as separate re / im arrays: an array of FP64 complex was
+  // placed in scratch memory by the compiler)
+  int pv[KB], si[KB];
+  T pr[KB], pim[KB];
+#pragma unroll
+  for (int i = 0; i < KB; ++i) {
+    const int q = p1 > p0 ? min(q0 + i * NT, p1 - 1) : 0;
+    pv[i] = yz[q];
+    si[i] = src[q];
+  }
+#pragma unroll
+  for (int i = 0; i < KB; ++i) {
+    const cpx<T> v = cb[si[i]];
+    pr[i] = v.x;
+    pim[i] = v.y;
+  }
   for (int i = threadIdx.x; i < N; i += blockDim.x) { tw[i] = twg[i]; map[i] = -1; }
   for (int e = threadIdx.x; e < ns * N; e += blockDim.x) rows[e] = czero<T>();
   __syncthreads();
@@ -921,11 +937,27 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
   for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) map[i1s[s]] = s - s0;
   __syncthreads();
   // point (y, z) of the plane -> stick row map[y], column z
-  if (va >= 0) rows[map[va / N] * N + va % N] = ca;
-  if (vb >= 0) rows[map[vb / N] * N + vb % N] = cbv;
-  for (int q = qb + NT; q < p1; q += NT) {
-    const int v = yz[q];
-    rows[map[v / N] * N + v % N] = cb[src[q]];
+#pragma unroll
+  for (int i = 0; i < KB; ++i)
+    if (q0 + i * NT < p1) rows[map[pv[i] / N] * N + pv[i] % N] = cpx<T>{pr[i], pim[i]};
+  for (int qq = q0 + KB * NT; qq < p1; qq += KB * NT) {  // planes of more than KB * NT points
+    int v2[KB], s2[KB];
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const int q = min(qq + i * NT, p1 - 1);
+      v2[i] = yz[q];
+      s2[i] = src[q];
+    }
+    T r2[KB], i2[KB];
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const cpx<T> v = cb[s2[i]];
+      r2[i] = v.x;
+      i2[i] = v.y;
+    }
+#pragma unroll
+    for (int i = 0; i < KB; ++i)
+      if (qq + i * NT < p1) rows[map[v2[i] / N] * N + v2[i] % N] = cpx<T>{r2[i], i2[i]};
   }
   __syncthreads();
   const int t = lane / LW, l = lane - t * LW;
@@ -945,16 +977,17 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
     }
   }
   __syncthreads();
-  // y inverse on all columns; column input is non-zero only at stick rows
+  // y inverse on all columns; column input is non-zero only at stick rows. The
+  // stick row of each of the lane's M y-positions is the same for every column.
   cpx<T>* dst = box + ((long)bi * g.na + a) * N * N;
+  int st[M];
+#pragma unroll
+  for (int q = 0; q < M; ++q) st[q] = part ? map[W::gpos(t, q)] : -1;
   for (int c0 = wave * LW; c0 < N; c0 += NW * LW) {
     const int col = c0 + l;
-    const bool ok = part && col < N;
+    const bool ok = col < N;
 #pragma unroll
-    for (int q = 0; q < M; ++q) {
-      const int st = ok ? map[W::gpos(t, q)] : -1;
-      x[q] = st >= 0 ? rows[st * N + col] : czero<T>();
-    }
+    for (int q = 0; q < M; ++q) x[q] = (ok && st[q] >= 0) ? rows[st[q] * N + col] : czero<T>();
     W::inverse(x, y, t, l, part, Ew, tw);
     if (t < M && col < N) {
 #pragma unroll
@@ -998,16 +1031,49 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
   // overlaps the transforms.
   constexpr int NT = 64 * NW;
   const int p0 = pp[s0], p1 = pp[s1];
-  const int qa = p0 + threadIdx.x, qb = qa + NT;
-  int sa = -1, sb = -1, va = 0, vb = 0;
-  cpx<T> ka = czero<T>(), kb = czero<T>();
-  if (qa < p1) {
-    sa = src[qa]; va = yz[qa];
-    if constexpr (KIN) { const cpx<T> cv = cb[sa]; const T tt = tk[sa]; ka.x = tt * cv.x; ka.y = tt * cv.y; }
-  }
-  if (qb < p1) {
-    sb = src[qb]; vb = yz[qb];
-    if constexpr (KIN) { const cpx<T> cv = cb[sb]; const T tt = tk[sb]; kb.x = tt * cv.x; kb.y = tt * cv.y; }
+  constexpr int KB = NW >= 16 ? 2 : (40 + NW - 1) / NW;  // passes per batch, all loads in flight together
+  // indices and t*c of the first batch of points are loaded now, so their latency
+  // overlaps the transforms (p1 > p0 here: the plane has sticks, each with points)
+  const int q0 = p0 + threadIdx.x;
+  int gs[KB], gv[KB];
+  T gr[KB], gi[KB];  // t*c, as separate re / im arrays (kept in registers)
+  if constexpr (NW >= 16) {  // guarded loads: fewer live registers under the 16-wave budget
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const int q = q0 + i * NT;
+      gs[i] = 0;
+      gv[i] = 0;
+      gr[i] = T(0);
+      gi[i] = T(0);
+      if (q < p1) {
+        gs[i] = src[q];
+        gv[i] = yz[q];
+        if constexpr (KIN) {
+          const cpx<T> cv = cb[gs[i]];
+          const T tt = tk[gs[i]];
+          gr[i] = tt * cv.x;
+          gi[i] = tt * cv.y;
+        }
+      }
+    }
+  } else {  // clamped loads: every load of the batch issued together, no branches
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const int q = min(q0 + i * NT, p1 - 1);
+      gs[i] = src[q];
+      gv[i] = yz[q];
+    }
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      gr[i] = T(0);
+      gi[i] = T(0);
+      if constexpr (KIN) {
+        const cpx<T> cv = cb[gs[i]];
+        const T tt = tk[gs[i]];
+        gr[i] = tt * cv.x;
+        gi[i] = tt * cv.y;
+      }
+    }
   }
   for (int i = threadIdx.x; i < N; i += blockDim.x) { tw[i] = twg[i]; map[i] = -1; }
   __syncthreads();
@@ -1024,6 +1090,14 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
 #pragma unroll
     for (int u = 0; u < S; ++u) v[u] = (col < N && t < M) ? srcp[(long)W::rpos(t, u) * N + col] : czero<T>();
   };
+  // the stick row of each of the lane's M y-positions (the same for every column;
+  // kept in registers unless the 16-wave register budget is too tight for them)
+  constexpr bool HOIST = NW < 16;
+  int st[HOIST ? M : 1];
+  if constexpr (HOIST) {
+#pragma unroll
+    for (int q = 0; q < M; ++q) st[q] = part ? map[W::gpos(t, q)] : -1;
+  }
   if (wave * LW < N) load_cols(wave * LW, yn);
   for (int c0 = wave * LW; c0 < N; c0 += NW * LW) {
     const int col = c0 + l;
@@ -1035,8 +1109,8 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
     if (ok && part) {
 #pragma unroll
       for (int q = 0; q < M; ++q) {
-        const int st = map[W::gpos(t, q)];
-        if (st >= 0) rows[st * N + col] = x[q];
+        const int sq = HOIST ? st[HOIST ? q : 0] : map[W::gpos(t, q)];
+        if (sq >= 0) rows[sq * N + col] = x[q];
       }
     }
   }
@@ -1060,13 +1134,41 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
     r.y = r.y * scale + kin.y;
     ob[si] = r;
   };
-  if (sa >= 0) put(sa, va, ka);
-  if (sb >= 0) put(sb, vb, kb);
-  for (int q = qb + NT; q < p1; q += NT) {
-    const int si = src[q];
-    cpx<T> kin = czero<T>();
-    if constexpr (KIN) { const cpx<T> cv = cb[si]; const T tt = tk[si]; kin.x = tt * cv.x; kin.y = tt * cv.y; }
-    put(si, yz[q], kin);
+#pragma unroll
+  for (int i = 0; i < KB; ++i)
+    if (q0 + i * NT < p1) put(gs[i], gv[i], cpx<T>{gr[i], gi[i]});
+  if constexpr (NW >= 16) {  // one point at a time: the batch registers would spill here
+    for (int q = q0 + KB * NT; q < p1; q += NT) {
+      const int si = src[q];
+      cpx<T> kin = czero<T>();
+      if constexpr (KIN) { const cpx<T> cv = cb[si]; const T tt = tk[si]; kin.x = tt * cv.x; kin.y = tt * cv.y; }
+      put(si, yz[q], kin);
+    }
+    return;
+  }
+  for (int qq = q0 + KB * NT; qq < p1; qq += KB * NT) {  // planes of more than KB * NT points
+    int s2[KB], v2[KB];
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const int q = min(qq + i * NT, p1 - 1);
+      s2[i] = src[q];
+      v2[i] = yz[q];
+    }
+    T r2[KB], i2[KB];
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      r2[i] = T(0);
+      i2[i] = T(0);
+      if constexpr (KIN) {
+        const cpx<T> cv = cb[s2[i]];
+        const T tt = tk[s2[i]];
+        r2[i] = tt * cv.x;
+        i2[i] = tt * cv.y;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < KB; ++i)
+      if (qq + i * NT < p1) put(s2[i], v2[i], cpx<T>{r2[i], i2[i]});
   }
 }
 
