@@ -19,6 +19,7 @@ GW_TERM_KINETIC, GW_TERM_LOCAL, GW_TERM_NONLOCAL, GW_TERM_ALL = 1, 2, 4, 7
 GW_PLAN_FORCE_UNFUSED, GW_PLAN_TIMING, GW_PLAN_LEGACY = 1, 2, 4
 GW_TUNE_RADIX, GW_TUNE_GEMM_WGS, GW_TUNE_GEMM_TILE, GW_TUNE_XPASS, GW_TUNE_COMBINE_LIB, GW_TUNE_INNER_LIB = 1, 2, 3, 4, 5, 6
 GW_TUNE_BAND_CHUNK = 7
+GW_TUNE_PLANE_BANDS = 8
 STAGES = ("plane_inv", "xpass", "plane_fwd", "becp", "dmul", "backproj")
 
 _P = C.c_void_p

@@ -35,6 +35,61 @@ CODELETS = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 12, 14, 15, 16}
 GEMM_TARGETS = (512, 1024, 2048, 4096, 8192)
 
 
+PLANE_BANDS = (1, 2, 4, 8)
+
+
+def tune_plane_bands(workloads, nb: int = 64, reps: int = 5) -> dict:
+    """For each precision, time P1 and P3 (the plan's stage timer) of the local
+    term of ``workloads`` whose box is a wave-line cube with every bands-per-item
+    count in PLANE_BANDS and keep the fastest per pass. Returns {"fp64": [p1, p3],
+    "fp32": [p1, p3]} (the current values if no workload qualifies)."""
+    from gradwave_amd.hamiltonian import BatchedHamiltonian
+
+    lib = L.lib()
+    users = [w for w in workloads if len(set(w.shape)) == 1 and w.shape[0] in WAVE_LENGTHS]
+    chosen = {name: [lib.gw_tune_get(L.GW_TUNE_PLANE_BANDS, prec + 256), lib.gw_tune_get(L.GW_TUNE_PLANE_BANDS,
+                                                                                        prec + 512)]
+              for name, prec in (("fp64", L.GW_FP64), ("fp32", L.GW_FP32))}
+    if not users:
+        return chosen
+    hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in users]
+    for h in hams:
+        h.set_flags(L.GW_PLAN_TIMING)
+    g = torch.Generator().manual_seed(6)
+    b64 = [(torch.randn(w.bk.nk, nb, w.bk.npw_max, dtype=torch.complex128, generator=g).to(w.v_r.device)
+            * w.bk.mask[:, None, :]) for w in users]
+    try:
+        for name, prec, blocks in (("fp64", L.GW_FP64, b64),
+                                   ("fp32", L.GW_FP32, [b.to(torch.complex64) for b in b64])):
+            best = {"plane_inv": (0, float("inf")), "plane_fwd": (0, float("inf"))}
+            for bpw in PLANE_BANDS:
+                L.check(lib.gw_tune_set(L.GW_TUNE_PLANE_BANDS, prec, bpw), "gw_tune_set")
+                for h, x in zip(hams, blocks):
+                    h.apply_terms(x, L.GW_TERM_LOCAL)
+                torch.cuda.synchronize()
+                for h in hams:
+                    h.stage_times(reset=True)
+                for _ in range(reps):
+                    for h, x in zip(hams, blocks):
+                        h.apply_terms(x, L.GW_TERM_LOCAL)
+                torch.cuda.synchronize()
+                tot = {"plane_inv": 0.0, "plane_fwd": 0.0}
+                for h in hams:
+                    st = h.stage_times(reset=True)
+                    for k in tot:
+                        tot[k] += st[k][0] / max(st[k][1], 1)
+                for k in best:
+                    if tot[k] < best[k][1]:
+                        best[k] = (bpw, tot[k])
+            L.check(lib.gw_tune_set(L.GW_TUNE_PLANE_BANDS, prec + 256, best["plane_inv"][0]), "gw_tune_set")
+            L.check(lib.gw_tune_set(L.GW_TUNE_PLANE_BANDS, prec + 512, best["plane_fwd"][0]), "gw_tune_set")
+            chosen[name] = [best["plane_inv"][0], best["plane_fwd"][0]]
+    finally:
+        for h in hams:
+            h.set_flags(0)
+    return chosen
+
+
 def tune_file() -> str:
     return os.environ.get("GW_TUNE_FILE", os.path.join(os.path.expanduser("~"), ".cache", "gradwave_amd",
                                                        "tune.json"))
@@ -63,7 +118,10 @@ def current() -> dict:
             "inner_lib": {"fp64": lib.gw_tune_get(L.GW_TUNE_INNER_LIB, L.GW_FP64),
                           "fp32": lib.gw_tune_get(L.GW_TUNE_INNER_LIB, L.GW_FP32)},
             "xpass": {"fp64": lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP64),
-                      "fp32": lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP32)}}
+                      "fp32": lib.gw_tune_get(L.GW_TUNE_XPASS, L.GW_FP32)},
+            "plane_bands": {name: [lib.gw_tune_get(L.GW_TUNE_PLANE_BANDS, prec + 256),
+                                   lib.gw_tune_get(L.GW_TUNE_PLANE_BANDS, prec + 512)]
+                            for name, prec in (("fp64", L.GW_FP64), ("fp32", L.GW_FP32))}}
 
 
 def apply(choices: dict) -> None:
@@ -84,6 +142,10 @@ def apply(choices: dict) -> None:
             L.check(lib.gw_tune_set(L.GW_TUNE_INNER_LIB, prec, int(choices["inner_lib"][name])), "gw_tune_set")
         if name in choices.get("xpass", {}):
             L.check(lib.gw_tune_set(L.GW_TUNE_XPASS, prec, int(choices["xpass"][name])), "gw_tune_set")
+        if name in choices.get("plane_bands", {}):
+            p1, p3 = choices["plane_bands"][name]
+            L.check(lib.gw_tune_set(L.GW_TUNE_PLANE_BANDS, prec + 256, int(p1)), "gw_tune_set")
+            L.check(lib.gw_tune_set(L.GW_TUNE_PLANE_BANDS, prec + 512, int(p3)), "gw_tune_set")
 
 
 def load(path: str | None = None) -> dict | None:

@@ -883,9 +883,40 @@ constexpr int kPW = 8;
 constexpr size_t kLdsPerCu = 160 * 1024;
 constexpr int kAdjBandGroup = 16;  // bands reduced per x-pass workgroup in the V adjoint
 
-template <typename T, int N, int M, int S, int NW>
+// Coefficients c[si[i]] of one batch of a plane's points, as separate re / im
+// registers (an array of FP64 complex was placed in scratch by the compiler).
+template <typename T, int KB>
+__device__ __forceinline__ void gather_points(const cpx<T>* __restrict__ cb, const int (&si)[KB], T (&re)[KB],
+                                              T (&im)[KB]) {
+#pragma unroll
+  for (int i = 0; i < KB; ++i) {
+    const cpx<T> v = cb[si[i]];
+    re[i] = v.x;
+    im[i] = v.y;
+  }
+}
+
+// A plane work item is x-plane a of the bands [b0, b1) of one k-point: the
+// plane's geometry (stick map, point indices, twiddles) is set up once, and
+// while one band's transforms run the next band's points are already loaded
+// and the previous band's stores drain.
+struct PlaneItem {
+  int a, k, b0, b1;
+};
+__device__ __forceinline__ PlaneItem plane_item(long wid, int na, int nb, int bpw) {
+  const int ngrp = (nb + bpw - 1) / bpw;
+  const long rest = wid / na;
+  PlaneItem it;
+  it.a = (int)(wid % na);
+  it.k = (int)(rest / ngrp);
+  it.b0 = (int)(rest % ngrp) * bpw;
+  it.b1 = min(nb, it.b0 + bpw);
+  return it;
+}
+
+template <typename T, int N, int M, int S, int NW, bool LOOP>
 __global__ void __launch_bounds__(64 * NW)
-k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long sc,
+k_plane_inv_w(Geom g, long total, int bpw, const cpx<T>* __restrict__ c, long ldc, long sc,
               cpx<T>* __restrict__ box, const cpx<T>* __restrict__ twg) {
   using W = WaveLine<T, N, M, S>;
   constexpr int LW = W::LW;
@@ -896,110 +927,104 @@ k_plane_inv_w(Geom g, long total, const cpx<T>* __restrict__ c, long ldc, long s
   int* map = reinterpret_cast<int*>(rows + (long)g.st_max_plane * N);
   const long wid = xcd_work_id(total);
   if (wid < 0) return;
-  const int a = (int)(wid % g.na), bi = (int)(wid / g.na);
-  const int k = bi / g.nb, b = bi - k * g.nb;
+  const PlaneItem it = plane_item(wid, g.na, g.nb, bpw);
+  const int a = it.a, k = it.k;
   const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
   const int s0 = sp[a], s1 = sp[a + 1], ns = s1 - s0;
   const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
   const int* i1s = g.st_i1 + (long)k * g.nst_max;
   const int* src = g.pt_src + (long)k * g.npw_max;
   const int* yz = g.pt_yz + (long)k * g.npw_max;
-  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  const cpx<T>* ck = c + (long)k * sc;
   // The plane's points, one per thread and pass over all threads, in batches of
-  // KB passes whose loads are all in flight together (indices first, then the
-  // coefficients they point at, from clamped in-range slots); the first batch is
-  // loaded before the LDS set-up so its latency overlaps it.
+  // KB passes whose loads are all in flight together (from clamped in-range
+  // slots; an empty plane reads slot 0 of the point tables and places nothing).
+  // The first batch's indices are the same for every band of the item.
   constexpr int NT = 64 * NW;
   constexpr int KB = NW >= 16 ? 2 : (40 + NW - 1) / NW;
   const int p0 = pp[s0], p1 = pp[s1];
   const int q0 = p0 + threadIdx.x;
-  // (an empty plane reads slot 0 of the point tables, always in range, and places nothing)
-  // (values kept as separate re / im arrays: an array of FP64 complex was
-  // placed in scratch memory by the compiler)
   int pv[KB], si[KB];
-  T pr[KB], pim[KB];
 #pragma unroll
   for (int i = 0; i < KB; ++i) {
     const int q = p1 > p0 ? min(q0 + i * NT, p1 - 1) : 0;
     pv[i] = yz[q];
     si[i] = src[q];
   }
-#pragma unroll
-  for (int i = 0; i < KB; ++i) {
-    const cpx<T> v = cb[si[i]];
-    pr[i] = v.x;
-    pim[i] = v.y;
-  }
+  T pr[KB], pim[KB];
+  gather_points<T, KB>(ck + (long)it.b0 * ldc, si, pr, pim);
   for (int i = threadIdx.x; i < N; i += blockDim.x) { tw[i] = twg[i]; map[i] = -1; }
-  for (int e = threadIdx.x; e < ns * N; e += blockDim.x) rows[e] = czero<T>();
   __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int s = s0 + threadIdx.x; s < s1; s += blockDim.x) map[i1s[s]] = s - s0;
   __syncthreads();
-  // point (y, z) of the plane -> stick row map[y], column z
-#pragma unroll
-  for (int i = 0; i < KB; ++i)
-    if (q0 + i * NT < p1) rows[map[pv[i] / N] * N + pv[i] % N] = cpx<T>{pr[i], pim[i]};
-  for (int qq = q0 + KB * NT; qq < p1; qq += KB * NT) {  // planes of more than KB * NT points
-    int v2[KB], s2[KB];
-#pragma unroll
-    for (int i = 0; i < KB; ++i) {
-      const int q = min(qq + i * NT, p1 - 1);
-      v2[i] = yz[q];
-      s2[i] = src[q];
-    }
-    T r2[KB], i2[KB];
-#pragma unroll
-    for (int i = 0; i < KB; ++i) {
-      const cpx<T> v = cb[s2[i]];
-      r2[i] = v.x;
-      i2[i] = v.y;
-    }
-#pragma unroll
-    for (int i = 0; i < KB; ++i)
-      if (qq + i * NT < p1) rows[map[v2[i] / N] * N + v2[i] % N] = cpx<T>{r2[i], i2[i]};
-  }
-  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int t = lane / LW, l = lane - t * LW;
   const bool part = t < S;
   cpx<T>* Ew = E + wave * W::E_PER_WAVE;
-  cpx<T> x[M], y[S];
-  // z inverse on the plane's sticks (rows of `rows`)
-  for (int r0 = wave * LW; r0 < ns; r0 += NW * LW) {
-    const int row = r0 + l;
-    const bool ok = part && row < ns;
-#pragma unroll
-    for (int q = 0; q < M; ++q) x[q] = ok ? rows[row * N + W::gpos(t, q)] : czero<T>();
-    W::inverse(x, y, t, l, part, Ew, tw);
-    if (t < M && row < ns) {
-#pragma unroll
-      for (int u = 0; u < S; ++u) rows[row * N + W::rpos(t, u)] = y[u];
-    }
-  }
-  __syncthreads();
-  // y inverse on all columns; column input is non-zero only at stick rows. The
-  // stick row of each of the lane's M y-positions is the same for every column.
-  cpx<T>* dst = box + ((long)bi * g.na + a) * N * N;
+  // stick row of each of the lane's M y-positions (the same for every column)
   int st[M];
 #pragma unroll
   for (int q = 0; q < M; ++q) st[q] = part ? map[W::gpos(t, q)] : -1;
-  for (int c0 = wave * LW; c0 < N; c0 += NW * LW) {
-    const int col = c0 + l;
-    const bool ok = col < N;
+  cpx<T> x[M], y[S];
+  const int b_end = LOOP ? it.b1 : it.b0 + 1;  // one band per item unless LOOP
+  for (int b = it.b0; b < b_end; ++b) {
+    const cpx<T>* cb = ck + (long)b * ldc;
+    for (int e = threadIdx.x; e < ns * N; e += blockDim.x) rows[e] = czero<T>();
+    __syncthreads();
+    // point (y, z) of the plane -> stick row map[y], column z
 #pragma unroll
-    for (int q = 0; q < M; ++q) x[q] = (ok && st[q] >= 0) ? rows[st[q] * N + col] : czero<T>();
-    W::inverse(x, y, t, l, part, Ew, tw);
-    if (t < M && col < N) {
+    for (int i = 0; i < KB; ++i)
+      if (q0 + i * NT < p1) rows[map[pv[i] / N] * N + pv[i] % N] = cpx<T>{pr[i], pim[i]};
+    for (int qq = q0 + KB * NT; qq < p1; qq += KB * NT) {  // planes of more than KB * NT points
+      int v2[KB], s2[KB];
 #pragma unroll
-      for (int u = 0; u < S; ++u) dst[(long)W::rpos(t, u) * N + col] = y[u];
+      for (int i = 0; i < KB; ++i) {
+        const int q = min(qq + i * NT, p1 - 1);
+        v2[i] = yz[q];
+        s2[i] = src[q];
+      }
+      T r2[KB], i2[KB];
+      gather_points<T, KB>(cb, s2, r2, i2);
+#pragma unroll
+      for (int i = 0; i < KB; ++i)
+        if (qq + i * NT < p1) rows[map[v2[i] / N] * N + v2[i] % N] = cpx<T>{r2[i], i2[i]};
     }
+    __syncthreads();
+    if (LOOP && b + 1 < b_end) gather_points<T, KB>(cb + ldc, si, pr, pim);  // the next band's first batch
+    // z inverse on the plane's sticks (rows of `rows`)
+    for (int r0 = wave * LW; r0 < ns; r0 += NW * LW) {
+      const int row = r0 + l;
+      const bool ok = part && row < ns;
+#pragma unroll
+      for (int q = 0; q < M; ++q) x[q] = ok ? rows[row * N + W::gpos(t, q)] : czero<T>();
+      W::inverse(x, y, t, l, part, Ew, tw);
+      if (t < M && row < ns) {
+#pragma unroll
+        for (int u = 0; u < S; ++u) rows[row * N + W::rpos(t, u)] = y[u];
+      }
+    }
+    __syncthreads();
+    // y inverse on all columns; column input is non-zero only at stick rows
+    cpx<T>* dst = box + (((long)k * g.nb + b) * g.na + a) * N * N;
+    for (int c0 = wave * LW; c0 < N; c0 += NW * LW) {
+      const int col = c0 + l;
+      const bool ok = col < N;
+#pragma unroll
+      for (int q = 0; q < M; ++q) x[q] = (ok && st[q] >= 0) ? rows[st[q] * N + col] : czero<T>();
+      W::inverse(x, y, t, l, part, Ew, tw);
+      if (t < M && col < N) {
+#pragma unroll
+        for (int u = 0; u < S; ++u) dst[(long)W::rpos(t, u) * N + col] = y[u];
+      }
+    }
+    __syncthreads();  // the rows are rewritten for the next band
   }
 }
 
 // P3: y forward on columns from the box -> stick rows (LDS) -> z forward -> gather (+ t c).
-template <typename T, int N, int M, int S, bool KIN, int NW>
+template <typename T, int N, int M, int S, bool KIN, int NW, bool LOOP>
 __global__ void __launch_bounds__(64 * NW)
-k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ c, long ldc,
+k_plane_fwd_w(Geom g, long total, int bpw, const cpx<T>* __restrict__ box, const cpx<T>* __restrict__ c, long ldc,
               long sc, const T* __restrict__ tk_all, cpx<T>* __restrict__ out, long ldo, long so, T scale,
               const cpx<T>* __restrict__ twg) {
   using W = WaveLine<T, N, M, S>;
@@ -1011,69 +1036,43 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
   int* map = reinterpret_cast<int*>(rows + (long)g.st_max_plane * N);
   const long wid = xcd_work_id(total);
   if (wid < 0) return;
-  const int a = (int)(wid % g.na), bi = (int)(wid / g.na);
-  const int k = bi / g.nb, b = bi - k * g.nb;
-  cpx<T>* ob = out + (long)k * so + (long)b * ldo;
-  if (a == 0) {  // padded slots of this band are zero (reference: out * mask)
-    for (int q = g.npw[k] + threadIdx.x; q < g.npw_max; q += blockDim.x) ob[q] = czero<T>();
+  const PlaneItem it = plane_item(wid, g.na, g.nb, bpw);
+  const int a = it.a, k = it.k;
+  if (a == 0) {  // padded slots of these bands are zero (reference: out * mask)
+    for (int b = it.b0; b < it.b1; ++b) {
+      cpx<T>* ob = out + (long)k * so + (long)b * ldo;
+      for (int q = g.npw[k] + threadIdx.x; q < g.npw_max; q += blockDim.x) ob[q] = czero<T>();
+    }
   }
   const int* sp = g.st_plane_ptr + (long)k * (g.na + 1);
   const int s0 = sp[a], s1 = sp[a + 1], ns = s1 - s0;
   if (ns == 0) return;
   const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
   const int* i1s = g.st_i1 + (long)k * g.nst_max;
-  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  const cpx<T>* ck = c + (long)k * sc;
   const int* src = g.pt_src + (long)k * g.npw_max;
   const int* yz = g.pt_yz + (long)k * g.npw_max;
   const T* tk = tk_all + (long)k * g.npw_max;
-  // The plane's points, one per thread and pass over all threads: the indices and
-  // the kinetic term t*c of the first two passes are loaded now, so their latency
-  // overlaps the transforms.
   constexpr int NT = 64 * NW;
-  const int p0 = pp[s0], p1 = pp[s1];
-  constexpr int KB = NW >= 16 ? 2 : (40 + NW - 1) / NW;  // passes per batch, all loads in flight together
-  // indices and t*c of the first batch of points are loaded now, so their latency
-  // overlaps the transforms (p1 > p0 here: the plane has sticks, each with points)
+  const int p0 = pp[s0], p1 = pp[s1];  // p1 > p0: the plane has sticks, each with points
+  // passes per batch, all loads in flight together; with 8 or more waves the
+  // workgroup's registers, not its LDS, set the occupancy, so the batch stays short
+  constexpr int KB = NW >= 8 ? 2 : (40 + NW - 1) / NW;
+  // the first batch's point indices (the same for every band of the item) and
+  // the first band's t*c, loaded now so their latency overlaps the transforms
   const int q0 = p0 + threadIdx.x;
   int gs[KB], gv[KB];
-  T gr[KB], gi[KB];  // t*c, as separate re / im arrays (kept in registers)
-  if constexpr (NW >= 16) {  // guarded loads: fewer live registers under the 16-wave budget
 #pragma unroll
-    for (int i = 0; i < KB; ++i) {
-      const int q = q0 + i * NT;
-      gs[i] = 0;
-      gv[i] = 0;
-      gr[i] = T(0);
-      gi[i] = T(0);
-      if (q < p1) {
-        gs[i] = src[q];
-        gv[i] = yz[q];
-        if constexpr (KIN) {
-          const cpx<T> cv = cb[gs[i]];
-          const T tt = tk[gs[i]];
-          gr[i] = tt * cv.x;
-          gi[i] = tt * cv.y;
-        }
-      }
-    }
-  } else {  // clamped loads: every load of the batch issued together, no branches
+  for (int i = 0; i < KB; ++i) {
+    const int q = min(q0 + i * NT, p1 - 1);
+    gs[i] = src[q];
+    gv[i] = yz[q];
+  }
+  T gr[KB], gi[KB], tt[KB];
+  if constexpr (KIN) {
 #pragma unroll
-    for (int i = 0; i < KB; ++i) {
-      const int q = min(q0 + i * NT, p1 - 1);
-      gs[i] = src[q];
-      gv[i] = yz[q];
-    }
-#pragma unroll
-    for (int i = 0; i < KB; ++i) {
-      gr[i] = T(0);
-      gi[i] = T(0);
-      if constexpr (KIN) {
-        const cpx<T> cv = cb[gs[i]];
-        const T tt = tk[gs[i]];
-        gr[i] = tt * cv.x;
-        gi[i] = tt * cv.y;
-      }
-    }
+    for (int i = 0; i < KB; ++i) tt[i] = tk[gs[i]];
+    gather_points<T, KB>(ck + (long)it.b0 * ldc, gs, gr, gi);
   }
   for (int i = threadIdx.x; i < N; i += blockDim.x) { tw[i] = twg[i]; map[i] = -1; }
   __syncthreads();
@@ -1083,92 +1082,82 @@ k_plane_fwd_w(Geom g, long total, const cpx<T>* __restrict__ box, const cpx<T>* 
   const int t = lane / LW, l = lane - t * LW;
   const bool part = t < S;
   cpx<T>* Ew = E + wave * W::E_PER_WAVE;
-  cpx<T> x[M], y[S], yn[S];
-  const cpx<T>* srcp = box + ((long)bi * g.na + a) * N * N;
-  auto load_cols = [&](int c0, cpx<T> (&v)[S]) {
-    const int col = c0 + l;
-#pragma unroll
-    for (int u = 0; u < S; ++u) v[u] = (col < N && t < M) ? srcp[(long)W::rpos(t, u) * N + col] : czero<T>();
-  };
   // the stick row of each of the lane's M y-positions (the same for every column;
-  // kept in registers unless the 16-wave register budget is too tight for them)
-  constexpr bool HOIST = NW < 16;
+  // kept in registers by the 4-wave workgroups only, see KB)
+  constexpr bool HOIST = NW < 8;
   int st[HOIST ? M : 1];
   if constexpr (HOIST) {
 #pragma unroll
     for (int q = 0; q < M; ++q) st[q] = part ? map[W::gpos(t, q)] : -1;
   }
-  if (wave * LW < N) load_cols(wave * LW, yn);
-  for (int c0 = wave * LW; c0 < N; c0 += NW * LW) {
-    const int col = c0 + l;
-    const bool ok = col < N;
+  cpx<T> x[M], y[S], yn[S];
+  const int b_end = LOOP ? it.b1 : it.b0 + 1;  // one band per item unless LOOP
+  for (int b = it.b0; b < b_end; ++b) {
+    const cpx<T>* cb = ck + (long)b * ldc;
+    cpx<T>* ob = out + (long)k * so + (long)b * ldo;
+    const cpx<T>* srcp = box + (((long)k * g.nb + b) * g.na + a) * N * N;
+    auto load_cols = [&](int c0, cpx<T> (&v)[S]) {
+      const int col = c0 + l;
 #pragma unroll
-    for (int u = 0; u < S; ++u) y[u] = yn[u];
-    if (c0 + NW * LW < N) load_cols(c0 + NW * LW, yn);  // next columns' loads overlap this transform
-    W::forward(y, x, t, l, part, Ew, tw);
-    if (ok && part) {
+      for (int u = 0; u < S; ++u) v[u] = (col < N && t < M) ? srcp[(long)W::rpos(t, u) * N + col] : czero<T>();
+    };
+    if (wave * LW < N) load_cols(wave * LW, yn);
+    for (int c0 = wave * LW; c0 < N; c0 += NW * LW) {
+      const int col = c0 + l;
+      const bool ok = col < N;
 #pragma unroll
-      for (int q = 0; q < M; ++q) {
-        const int sq = HOIST ? st[HOIST ? q : 0] : map[W::gpos(t, q)];
-        if (sq >= 0) rows[sq * N + col] = x[q];
+      for (int u = 0; u < S; ++u) y[u] = yn[u];
+      if (c0 + NW * LW < N) load_cols(c0 + NW * LW, yn);  // next columns' loads overlap this transform
+      W::forward(y, x, t, l, part, Ew, tw);
+      if (ok && part) {
+#pragma unroll
+        for (int q = 0; q < M; ++q) {
+          const int sq = HOIST ? st[HOIST ? q : 0] : map[W::gpos(t, q)];
+          if (sq >= 0) rows[sq * N + col] = x[q];
+        }
       }
     }
-  }
-  __syncthreads();
-  for (int r0 = wave * LW; r0 < ns; r0 += NW * LW) {
-    const int row = r0 + l;
-    const bool ok = row < ns;
+    __syncthreads();
+    for (int r0 = wave * LW; r0 < ns; r0 += NW * LW) {
+      const int row = r0 + l;
+      const bool ok = row < ns;
 #pragma unroll
-    for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? rows[row * N + W::rpos(t, u)] : czero<T>();
-    W::forward(y, x, t, l, part, Ew, tw);
-    if (ok && part) {
+      for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? rows[row * N + W::rpos(t, u)] : czero<T>();
+      W::forward(y, x, t, l, part, Ew, tw);
+      if (ok && part) {
 #pragma unroll
-      for (int q = 0; q < M; ++q) rows[row * N + W::gpos(t, q)] = x[q];
+        for (int q = 0; q < M; ++q) rows[row * N + W::gpos(t, q)] = x[q];
+      }
     }
-  }
-  __syncthreads();
-  // gather: point (y, z) of the plane sits in stick row map[y], column z
-  auto put = [&](int si, int v, cpx<T> kin) {
-    cpx<T> r = rows[map[v / N] * N + v % N];
-    r.x = r.x * scale + kin.x;
-    r.y = r.y * scale + kin.y;
-    ob[si] = r;
-  };
+    __syncthreads();
+    // gather: point (y, z) of the plane sits in stick row map[y], column z
+    auto put = [&](int si, int v, cpx<T> kin) {
+      cpx<T> r = rows[map[v / N] * N + v % N];
+      r.x = r.x * scale + kin.x;
+      r.y = r.y * scale + kin.y;
+      ob[si] = r;
+    };
 #pragma unroll
-  for (int i = 0; i < KB; ++i)
-    if (q0 + i * NT < p1) put(gs[i], gv[i], cpx<T>{gr[i], gi[i]});
-  if constexpr (NW >= 16) {  // one point at a time: the batch registers would spill here
-    for (int q = q0 + KB * NT; q < p1; q += NT) {
+    for (int i = 0; i < KB; ++i) {
+      if (q0 + i * NT < p1) {
+        cpx<T> kin = czero<T>();
+        if constexpr (KIN) {
+          kin.x = tt[i] * gr[i];
+          kin.y = tt[i] * gi[i];
+        }
+        put(gs[i], gv[i], kin);
+      }
+    }
+    if constexpr (KIN) {
+      if (LOOP && b + 1 < b_end) gather_points<T, KB>(cb + ldc, gs, gr, gi);  // the next band's first batch
+    }
+    for (int q = q0 + KB * NT; q < p1; q += NT) {  // planes of more than KB * NT points
       const int si = src[q];
       cpx<T> kin = czero<T>();
       if constexpr (KIN) { const cpx<T> cv = cb[si]; const T tt = tk[si]; kin.x = tt * cv.x; kin.y = tt * cv.y; }
       put(si, yz[q], kin);
     }
-    return;
-  }
-  for (int qq = q0 + KB * NT; qq < p1; qq += KB * NT) {  // planes of more than KB * NT points
-    int s2[KB], v2[KB];
-#pragma unroll
-    for (int i = 0; i < KB; ++i) {
-      const int q = min(qq + i * NT, p1 - 1);
-      s2[i] = src[q];
-      v2[i] = yz[q];
-    }
-    T r2[KB], i2[KB];
-#pragma unroll
-    for (int i = 0; i < KB; ++i) {
-      r2[i] = T(0);
-      i2[i] = T(0);
-      if constexpr (KIN) {
-        const cpx<T> cv = cb[s2[i]];
-        const T tt = tk[s2[i]];
-        r2[i] = tt * cv.x;
-        i2[i] = tt * cv.y;
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < KB; ++i)
-      if (qq + i * NT < p1) put(s2[i], v2[i], cpx<T>{r2[i], i2[i]});
+    __syncthreads();  // the rows are rewritten for the next band
   }
 }
 
@@ -1204,22 +1193,45 @@ static int plane_waves(const Plan& P) {
 }
 
 // Launches P1/P2/P3 of the wave-line path; false if the box is not handled.
+// Bands per plane work item, per precision and pass (0 P1, 1 P3), from
+// GW_TUNE_PLANE_BANDS; 0 picks 4 where one workgroup per CU is all the plane's
+// LDS allows (P1 of both precisions, P3 in FP64: one band's stores drain and the
+// next band's points load while a band transforms), else 1. Items of one band
+// run the kernels built without the band loop (fewer live registers).
+static int g_plane_bpw[2][2] = {{0, 0}, {0, 0}};
+template <typename T, int N, int NW>
+static int plane_bpw(const Plan& P, bool inv) {
+  const int v = g_plane_bpw[sizeof(T) == 4][inv ? 0 : 1];
+  if (v > 0) return v;
+  const bool one_wg = kLdsPerCu / plane_w_lds<T, N, NW>(P) <= 1;
+  return one_wg && (inv || sizeof(T) == 8) ? 4 : 1;
+}
+
+// `total` = active planes x bands (P.na * nbt, nbt = nk * g.nb)
 template <typename T, int N, int NW>
 static void launch_planes(bool inv, Plan& P, const Geom& g, long total, const cpx<T>* c, long ldc, long sc,
                           cpx<T>* box, const T* t, cpx<T>* out, long ldo, long so, T scale, bool kin, hipStream_t st) {
   using C = WaveCfg<N>;
-  const int nblk = (int)(8 * ((total + 7) / 8));
+  const int bpw = std::min(plane_bpw<T, N, NW>(P, inv), std::max(1, g.nb));
+  const long nk = total / ((long)P.na * std::max(1, g.nb));
+  const long items = (long)P.na * nk * ((g.nb + bpw - 1) / bpw);
+  const int nblk = (int)(8 * ((items + 7) / 8));
   const size_t lds = plane_w_lds<T, N, NW>(P);
   const cpx<T>* tw = tw_of<T>(&P, 1);
-  if (inv)
-    hipLaunchKernelGGL((k_plane_inv_w<T, N, C::M, C::S, NW>), dim3(nblk), dim3(64 * NW), lds, st, g, total, c, ldc,
-                       sc, box, tw);
-  else if (kin)
-    hipLaunchKernelGGL((k_plane_fwd_w<T, N, C::M, C::S, true, NW>), dim3(nblk), dim3(64 * NW), lds, st, g, total,
-                       box, c, ldc, sc, t, out, ldo, so, scale, tw);
-  else
-    hipLaunchKernelGGL((k_plane_fwd_w<T, N, C::M, C::S, false, NW>), dim3(nblk), dim3(64 * NW), lds, st, g, total,
-                       box, c, ldc, sc, t, out, ldo, so, scale, tw);
+  auto go = [&](auto loop) {
+    constexpr bool L = decltype(loop)::value;
+    if (inv)
+      hipLaunchKernelGGL((k_plane_inv_w<T, N, C::M, C::S, NW, L>), dim3(nblk), dim3(64 * NW), lds, st, g, items, bpw,
+                         c, ldc, sc, box, tw);
+    else if (kin)
+      hipLaunchKernelGGL((k_plane_fwd_w<T, N, C::M, C::S, true, NW, L>), dim3(nblk), dim3(64 * NW), lds, st, g,
+                         items, bpw, box, c, ldc, sc, t, out, ldo, so, scale, tw);
+    else
+      hipLaunchKernelGGL((k_plane_fwd_w<T, N, C::M, C::S, false, NW, L>), dim3(nblk), dim3(64 * NW), lds, st, g,
+                         items, bpw, box, c, ldc, sc, t, out, ldo, so, scale, tw);
+  };
+  if (bpw > 1) go(std::true_type{});
+  else go(std::false_type{});
 }
 
 // Launches P1/P2/P3 of the wave-line path.
@@ -1493,9 +1505,13 @@ static int launch_wave_path(Plan& P, const Geom& g, int nbt, const cpx<T>* c, lo
 template <typename T, int N, int NW>
 static void set_wave_attr_nw(int want) {
   using C = WaveCfg<N>;
-  (void)hipFuncSetAttribute((const void*)k_plane_inv_w<T, N, C::M, C::S, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
-  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, true, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
-  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, false, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  constexpr auto A = hipFuncAttributeMaxDynamicSharedMemorySize;
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_w<T, N, C::M, C::S, NW, false>, A, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_inv_w<T, N, C::M, C::S, NW, true>, A, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, true, NW, false>, A, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, true, NW, true>, A, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, false, NW, false>, A, want);
+  (void)hipFuncSetAttribute((const void*)k_plane_fwd_w<T, N, C::M, C::S, false, NW, true>, A, want);
 }
 template <typename T, int N>
 static void set_wave_attr_n(int want) {
@@ -1650,6 +1666,15 @@ extern "C" int gw_tune_set(int key, int arg, int value) {
     set_combine_lib(arg, value);
     return GW_OK;
   }
+  if (key == GW_TUNE_PLANE_BANDS) {  // arg = precision + 256 * (0 both passes, 1 P1, 2 P3)
+    const int prec = arg & 255, pass = arg >> 8;
+    GW_REQUIRE(prec == GW_FP64 || prec == GW_FP32, GW_ERR_ARG, "plane bands: bad precision %d", prec);
+    GW_REQUIRE(pass >= 0 && pass <= 2, GW_ERR_ARG, "plane bands: bad pass selector %d", pass);
+    GW_REQUIRE(value >= 0 && value <= 64, GW_ERR_ARG, "plane bands %d outside 0..64", value);
+    if (pass != 2) g_plane_bpw[prec == GW_FP32][0] = value;
+    if (pass != 1) g_plane_bpw[prec == GW_FP32][1] = value;
+    return GW_OK;
+  }
   if (key == GW_TUNE_BAND_CHUNK) {
     GW_REQUIRE(arg == GW_FP64 || arg == GW_FP32, GW_ERR_ARG, "band chunk: bad precision %d", arg);
     GW_REQUIRE(value >= 0, GW_ERR_ARG, "band chunk %d < 0", value);
@@ -1684,6 +1709,11 @@ extern "C" int gw_tune_get(int key, int arg) {
   if (key == GW_TUNE_COMBINE_LIB) return (arg == GW_FP64 || arg == GW_FP32) ? combine_lib(arg) : -1;
   if (key == GW_TUNE_INNER_LIB) return (arg == GW_FP64 || arg == GW_FP32) ? inner_lib(arg) : -1;
   if (key == GW_TUNE_BAND_CHUNK) return (arg == GW_FP64 || arg == GW_FP32) ? g_band_chunk[arg == GW_FP64 ? 0 : 1] : -1;
+  if (key == GW_TUNE_PLANE_BANDS) {
+    const int prec = arg & 255, pass = arg >> 8;
+    return (prec == GW_FP64 || prec == GW_FP32) && pass >= 0 && pass <= 2
+               ? g_plane_bpw[prec == GW_FP32][pass == 2 ? 1 : 0] : -1;
+  }
   return -1;
 }
 extern "C" int gw_version(void) { return 1; }

@@ -260,6 +260,12 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
  *   10416 for FP32.
  * GW_TUNE_COMBINE_LIB: for precision `arg`, 1 runs gw_zgemm_combine (and the
  *   back-projection) on rocBLAS strided-batched GEMM, 0 on the kernels here (default).
+ * GW_TUNE_PLANE_BANDS: bands one workgroup of the wave-line plane passes transforms in
+ *   turn for one x-plane, 1..64; `arg` = precision + 256 * (0: both passes, 1: P1 only,
+ *   2: P3 only; gw_tune_get with 0 reports P1). 0, the default: 4 where the plane's
+ *   LDS allows one workgroup per CU (P1 of both precisions, P3 in FP64), else 1. The
+ *   geometry is set up once per workgroup and the next band's points load during a
+ *   band's transforms.
  * GW_TUNE_BAND_CHUNK: for precision `arg`, bands per chunk of the local term of
  *   gw_hpsi_apply (0, the default: the whole block); each chunk runs the three
  *   passes back to back on a box slice of that many bands, which also sets the
@@ -268,7 +274,7 @@ int gw_vloc_g_grad(int na, long npts, const double* positions, const int64_t* sp
  *   of the projection <beta|psi>) as one rocBLAS strided-batched GEMM whose partials the
  *   FP64 split-K reduction here sums, 0 the MFMA kernels here (default). */
 enum { GW_TUNE_RADIX = 1, GW_TUNE_GEMM_WGS = 2, GW_TUNE_GEMM_TILE = 3, GW_TUNE_XPASS = 4,
-       GW_TUNE_COMBINE_LIB = 5, GW_TUNE_INNER_LIB = 6, GW_TUNE_BAND_CHUNK = 7 };
+       GW_TUNE_COMBINE_LIB = 5, GW_TUNE_INNER_LIB = 6, GW_TUNE_BAND_CHUNK = 7, GW_TUNE_PLANE_BANDS = 8 };
 int gw_tune_set(int key, int arg, int value);
 int gw_tune_get(int key, int arg);
 
