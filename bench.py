@@ -99,10 +99,12 @@ def main() -> None:
         t1 = time.perf_counter()
         tiles, wgs = autotune.tune_gemm([wl], nb=nb)
         plane_bands = autotune.tune_plane_bands([wl], nb=nb)
+        xpass = autotune.tune_xpass([wl], nb=nb)
         tuned = {"gemm_tile_variant": tiles, "gemm_split_k_target_wgs": wgs,
-                 "plane_bands_per_item": plane_bands, "seconds": time.perf_counter() - t1,
+                 "plane_bands_per_item": plane_bands, "xpass_shape": xpass, "seconds": time.perf_counter() - t1,
                  "saved_to": autotune.save()}  # later runs with --no-autotune load these choices
-        print(f"[bench] autotune GEMM: tiles {tiles}, split-K target {wgs}; plane bands {plane_bands} "
+        print(f"[bench] autotune GEMM: tiles {tiles}, split-K target {wgs}; plane bands {plane_bands}, "
+              f"x-pass {xpass} "
               f"({tuned['seconds']:.1f}s)",
               file=sys.stderr, flush=True)
     if args.only is not None:
