@@ -397,10 +397,21 @@ def _refine_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, sweeps:
     return BatchedDavidsonResult(eig, x, sweeps, rn.to(x0.real.dtype), 0, counts["full"])
 
 
+BUFFER_SEED = 4241
+
+
+def buffer_block(nk: int, nbuf: int, m: int, mask: torch.Tensor | None, device) -> torch.Tensor:
+    """``nbuf`` seeded complex128 start vectors per k (generator seed BUFFER_SEED,
+    padded slots zero) for the buffer bands of the precision form."""
+    g = torch.Generator().manual_seed(BUFFER_SEED)
+    x = torch.randn(nk, nbuf, m, dtype=torch.complex128, generator=g).to(device)
+    return x * mask[:, None, :].to(x.dtype) if mask is not None else x
+
+
 @torch.no_grad()
 def davidson_mixed(h_apply, x0, t, mask, tol: float = 1e-9, draft_tol: float = 1e-4,
                    max_iter: int = 100, max_dim_factor: int = 4, sweeps: int | None = None,
-                   profile_out: dict | None = None) -> BatchedDavidsonResult:
+                   profile_out: dict | None = None, buffer: int = 0) -> BatchedDavidsonResult:
     """Precision form: the draft solve runs on complex64 blocks -- FP32 FFTs, V(r)
     multiply and MFMA contractions with FP32 accumulation -- to ``draft_tol``,
     keeping the subspace matrix, its eigensolve, the Cholesky factors and every
@@ -410,7 +421,15 @@ def davidson_mixed(h_apply, x0, t, mask, tol: float = 1e-9, draft_tol: float = 1
     is refined by n FP64 Davidson sweeps instead (no gate; the result carries
     their FP64 residual norms, and ``tol`` is unused). The reference's
     draft/polish scaffold is solvers/_ms.py:26-52 (davidson_batched_ms gates it
-    on crossover < tol)."""
+    on crossover < tol). ``buffer`` > 0 carries that many extra bands (seeded
+    start vectors, ``buffer_block``) through draft and sweeps and returns only
+    the requested ones: the last bands of a block then converge like interior
+    bands (in the sweep form they otherwise carry the largest eigenvalue error)."""
+    if buffer < 0:
+        raise ValueError(f"buffer must be >= 0, got {buffer}")
+    nb = x0.shape[1]
+    if buffer:
+        x0 = torch.cat([x0, buffer_block(x0.shape[0], buffer, x0.shape[2], mask, x0.device).to(x0.dtype)], dim=1)
     hi = x0.dtype
     d = davidson_batched(h_apply, x0.to(LOW), t.to(torch.float32), mask, tol=draft_tol,
                          max_iter=max_iter, max_dim_factor=max_dim_factor, profile_out=profile_out)
@@ -423,6 +442,9 @@ def davidson_mixed(h_apply, x0, t, mask, tol: float = 1e-9, draft_tol: float = 1
                              max_dim_factor=max_dim_factor, profile_out=profile_out)
     p.n_apply_low += d.n_apply_full
     p.n_draft_iter = d.n_iter
+    if buffer:
+        p.eigenvalues, p.eigenvectors = p.eigenvalues[:, :nb], p.eigenvectors[:, :nb]
+        p.residual_norms = p.residual_norms[:, :nb]
     return p
 
 

@@ -26,6 +26,7 @@ import torch
 
 from gradwave_amd import _lib as L
 from gradwave_amd import linalg as LA
+from gradwave_amd.davidson import buffer_block
 from gradwave_amd.hamiltonian import BatchedHamiltonian
 
 HBAR2_2M = 27.211386245988 / 2.0 * 0.529177210903**2
@@ -434,7 +435,7 @@ def refine_real(h_apply, x0: torch.Tensor, t_g: torch.Tensor, sweeps: int = 1):
 
 def davidson_gamma_mixed(ham: GammaHamiltonian, x0: torch.Tensor, tol: float = 1e-9, draft_tol: float = 1e-4,
                          max_iter: int = 100, max_dim_factor: int = 4, sweeps: int | None = None,
-                         profile_out: dict | None = None) -> GammaDavidsonResult:
+                         profile_out: dict | None = None, buffer: int = 0) -> GammaDavidsonResult:
     """Precision form at Gamma: the draft solve keeps its basis in FP32 and
     applies H in FP32 (FFTs, V multiply, MFMA projector and subspace GEMMs with
     FP32 accumulation and FP64 split-K sums) to ``draft_tol``, with the subspace
@@ -442,8 +443,17 @@ def davidson_gamma_mixed(ham: GammaHamiltonian, x0: torch.Tensor, tol: float = 1
     ``sweeps=None`` the polish solve restarts from the draft Ritz vectors in
     FP64 and converges on FP64 residuals of the FP64 operator to ``tol``; with
     ``sweeps=n`` n FP64 refinement sweeps (``refine_real``) replace it and
-    ``tol`` is unused."""
+    ``tol`` is unused. ``buffer`` > 0 carries that many extra bands (seeded
+    half-sphere start vectors, G = 0 real) and returns the requested ones only,
+    as in ``davidson_mixed``."""
+    if buffer < 0:
+        raise ValueError(f"buffer must be >= 0, got {buffer}")
     gb = ham.gb
+    nb = x0.shape[0]
+    if buffer:
+        xb = buffer_block(1, buffer, x0.shape[1], None, x0.device)[0]
+        xb[:, 0] = xb[:, 0].real.to(xb.dtype)
+        x0 = torch.cat([x0.to(torch.complex128), xb])
     t_phi = kinetic_phi(gb)
     x32 = embed_real(gb, x0.to(torch.complex128)).to(torch.float32)
     _, xd, itd, _ = davidson_real(ham.apply_real, x32, t_phi, tol=draft_tol, max_iter=max_iter,
@@ -455,4 +465,6 @@ def davidson_gamma_mixed(ham: GammaHamiltonian, x0: torch.Tensor, tol: float = 1
     else:
         eig, x, it, rn = davidson_real(ham.apply_real, xd.to(torch.float64), t_phi, tol=tol, max_iter=max_iter,
                                        max_dim_factor=max_dim_factor, profile_out=profile_out)
+    if buffer:
+        eig, x, rn = eig[:nb], x[:nb], rn[:nb]
     return GammaDavidsonResult(eig, unembed_real(gb, x), it, rn, n_draft_iter=itd)

@@ -200,6 +200,16 @@ def test_si8_mixed_refinement_sweeps():
     rms = davidson_mixed(h.apply, x0, gpu.t, gpu.mask, draft_tol=1e-3, max_iter=200, sweeps=1)
     assert rms.n_iter == 1 and rms.n_draft_iter == draft.n_iter and rms.n_apply_low > 0
     assert float((rms.eigenvalues - r64.eigenvalues).abs().max()) < 1e-6 * HARTREE_EV
+    # buffer bands: the requested 24 come back, within the bound, as upper bounds
+    rb = davidson_mixed(h.apply, x0, gpu.t, gpu.mask, draft_tol=5e-3, max_iter=200, sweeps=1, buffer=4)
+    assert rb.eigenvalues.shape == (3, 24) and rb.eigenvectors.shape == x0.shape and rb.residual_norms.shape == (3, 24)
+    assert float((rb.eigenvalues - r64.eigenvalues).abs().max()) < 1e-6 * HARTREE_EV
+    assert float((rb.eigenvalues - r64.eigenvalues).min()) > -eps
+    hx = h.apply(rb.eigenvectors)
+    rn = torch.linalg.norm(hx - rb.eigenvalues[..., None] * rb.eigenvectors, dim=-1)
+    assert float((rn - rb.residual_norms).abs().max()) < 1e-10
+    with pytest.raises(ValueError):
+        davidson_mixed(h.apply, x0, gpu.t, gpu.mask, sweeps=1, buffer=-1)
     with pytest.raises(ValueError):
         davidson_mixed(h.apply, x0, gpu.t, gpu.mask, draft_tol=1e-3, max_iter=200, sweeps=-1)
 

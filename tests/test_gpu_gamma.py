@@ -105,6 +105,11 @@ def test_gamma_davidson_matches_dense_and_complex():
     one = G.davidson_gamma_mixed(ours, x0.to(dev()), draft_tol=1e-3, max_iter=300, sweeps=1)
     assert one.n_iter == 1
     assert float((one.eigenvalues.cpu() - dense).abs().max()) / HARTREE_EV < 1e-6
+    # buffer bands: the requested bands only, within the bound, upper bounds
+    buf = G.davidson_gamma_mixed(ours, x0.to(dev()), draft_tol=5e-3, max_iter=300, sweeps=1, buffer=4)
+    assert buf.eigenvalues.shape == (x0.shape[0],) and buf.eigenvectors.shape == x0.shape
+    assert float((buf.eigenvalues.cpu() - dense).abs().max()) / HARTREE_EV < 1e-6
+    assert float((buf.eigenvalues.cpu() - dense).min()) > -1e-12 * HARTREE_EV
     # the complex batched solver on the same frozen potential
     d = dev()
     bk = SimpleNamespace(npw=s.bk.npw.to(d), mask=s.bk.mask.to(d), flat_idx=s.bk.flat_idx.to(d), t=s.bk.t.to(d),

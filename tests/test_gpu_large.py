@@ -9,7 +9,8 @@
 * Si216: the GPU FP64 eigenpairs are certified with the oracle's operator in
   the test (480 applies on the host): Rayleigh quotients equal the eigenvalues
   to 1e-10 Ha and residual norms stay at the 1e-10 Ha tolerance. The sweep
-  form's eigenvalues are within 1e-6 Ha of FP64, the mixed-orbital forces on
+  form's eigenvalues are within 1e-6 Ha of FP64 (also with the looser draft
+  and buffer bands the bench times), the mixed-orbital forces on
   all 648 coordinates within 1e-5 Ha/bohr of the FP64-orbital forces, and the
   GPU forces match the oracle's forces (postscf/forces.py:168-213 route) on a
   subset of atoms and the oracle's band energy.
@@ -96,6 +97,13 @@ def test_si216_fp64_certified_by_oracle_and_precision_form():
     to_ha_bohr = BOHR_ANG / HARTREE_EV
     assert f64.shape == (216, 3)
     assert float((fsw - f64).abs().max()) * to_ha_bohr < 1e-5
+    say("si216 sweep form at the metric bound: draft 5e-3 eV, 32 buffer bands, one FP64 sweep")
+    rb = davidson_mixed(h.apply, x0, wl.bk.t, wl.bk.mask, draft_tol=5e-3, max_iter=400, sweeps=1, buffer=32)
+    assert rb.eigenvalues.shape == r64.eigenvalues.shape and rb.eigenvectors.shape == r64.eigenvectors.shape
+    assert float((rb.eigenvalues - r64.eigenvalues).abs().max()) < 1e-6 * HARTREE_EV
+    assert float((rb.eigenvalues - r64.eigenvalues).min()) > -1e-12 * HARTREE_EV  # Ritz values: upper bounds
+    _, fb = band_energy_forces(*args, rb.eigenvectors, w)
+    assert float((fb - f64).abs().max()) * to_ha_bohr < 1e-5
 
     say("si216 oracle system and certification of the fp64 eigenpairs (host)")
     torch.set_num_threads(min(16, os.cpu_count() or 1))
