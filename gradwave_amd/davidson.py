@@ -245,26 +245,29 @@ def davidson_batched(h_apply: Callable[[torch.Tensor], torch.Tensor], x0: torch.
                      t: torch.Tensor, mask: torch.Tensor, tol: float = 1e-9, max_iter: int = 40,
                      max_dim_factor: int = 4, sync_free: bool = False,
                      history_out: list | None = None,
-                     profile_out: dict | None = None) -> BatchedDavidsonResult:
+                     profile_out: dict | None = None, nconv: int | None = None) -> BatchedDavidsonResult:
     """Batched block Davidson over all k at once -- solvers/davidson.py:527-774.
 
     ``sync_free`` is accepted for signature parity; the synchronous gate is
     always used (the reference measured the delayed gate as a loss,
     davidson.py:545-554). ``profile_out`` (not in the reference) collects wall
-    seconds per phase."""
+    seconds per phase. ``nconv`` (not in the reference; default: every band)
+    stops the solve once the first ``nconv`` bands are converged, the rest
+    being buffer bands that are still expanded but do not gate."""
     return drive(davidson_steps(x0, t, mask, tol=tol, max_iter=max_iter, max_dim_factor=max_dim_factor,
-                                history_out=history_out, profile_out=profile_out), h_apply)
+                                history_out=history_out, profile_out=profile_out, nconv=nconv), h_apply)
 
 
 @torch.no_grad()
 def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: float = 1e-9,
                    max_iter: int = 40, max_dim_factor: int = 4, history_out: list | None = None,
-                   profile_out: dict | None = None):
+                   profile_out: dict | None = None, nconv: int | None = None):
     """The solver of ``davidson_batched`` as a step generator: it yields
     ("apply", block) and ("eigh", subspace matrix) requests and returns the
     BatchedDavidsonResult, so a driver can serve many independent solves in
     lockstep with one batched apply and one batched eigensolve per round."""
     nk, nb, m = x0.shape
+    ng = nb if nconv is None else max(1, min(int(nconv), nb))  # bands that gate convergence
     tm = PhaseTimer(profile_out)
     device = x0.device
     max_dim = min(max_dim_factor * nb, int(mask.sum(dim=1).min()))
@@ -299,17 +302,17 @@ def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: f
         tm("residual")
         r, rn = LA.residual(x, hx, eig)
         if use_fp32_exp:
-            rnmax = float(rn.max())
+            rnmax = float(rn[:, :ng].max())
             stalled = it > 1 and rnmax > _FP32_CERT_STALL * prev_rnmax
             prev_rnmax = rnmax
             if rnmax < max(tol, cert_floor) or stalled:
                 eig, x, hx, r, rn = yield from _certify_steps(x, counts)
-                if float(rn.max()) >= tol:
+                if float(rn[:, :ng].max()) >= tol:
                     basis.set(x, hx)
                     use_fp32_exp = False
         if history_out is not None:
             history_out.append((it, rn.detach().to("cpu").clone(), eig.detach().to("cpu").clone()))
-        gate = yield ("host", torch.stack([rn.max(), (rn > tol).sum(dim=1).max().to(rn.dtype)]))
+        gate = yield ("host", torch.stack([rn[:, :ng].max(), (rn > tol).sum(dim=1).max().to(rn.dtype)]))
         if float(gate[0]) < tol:
             tm(None)
             return BatchedDavidsonResult(eig, x, it, rn.to(rdtype), counts["low"], counts["full"])
@@ -424,22 +427,24 @@ def davidson_mixed(h_apply, x0, t, mask, tol: float = 1e-9, draft_tol: float = 1
     on crossover < tol). ``buffer`` > 0 carries that many extra bands (seeded
     start vectors, ``buffer_block``) through draft and sweeps and returns only
     the requested ones: the last bands of a block then converge like interior
-    bands (in the sweep form they otherwise carry the largest eigenvalue error)."""
+    bands (in the sweep form they otherwise carry the largest eigenvalue error);
+    only the requested bands gate the draft's (and the polish's) convergence."""
     if buffer < 0:
         raise ValueError(f"buffer must be >= 0, got {buffer}")
     nb = x0.shape[1]
     if buffer:
         x0 = torch.cat([x0, buffer_block(x0.shape[0], buffer, x0.shape[2], mask, x0.device).to(x0.dtype)], dim=1)
     hi = x0.dtype
+    nconv = nb if buffer else None  # buffer bands are expanded but do not gate
     d = davidson_batched(h_apply, x0.to(LOW), t.to(torch.float32), mask, tol=draft_tol,
-                         max_iter=max_iter, max_dim_factor=max_dim_factor, profile_out=profile_out)
+                         max_iter=max_iter, max_dim_factor=max_dim_factor, profile_out=profile_out, nconv=nconv)
     if sweeps is not None:
         if sweeps < 0:
             raise ValueError(f"sweeps must be >= 0, got {sweeps}")
         p = drive(_refine_steps(d.eigenvectors.to(hi), t, mask, int(sweeps)), h_apply)
     else:
         p = davidson_batched(h_apply, d.eigenvectors.to(hi), t, mask, tol=tol, max_iter=max_iter,
-                             max_dim_factor=max_dim_factor, profile_out=profile_out)
+                             max_dim_factor=max_dim_factor, profile_out=profile_out, nconv=nconv)
     p.n_apply_low += d.n_apply_full
     p.n_draft_iter = d.n_iter
     if buffer:

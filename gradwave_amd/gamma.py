@@ -313,13 +313,16 @@ class _Timer:
 
 @torch.no_grad()
 def davidson_real(h_apply, x0: torch.Tensor, t_g: torch.Tensor, tol: float = 1e-9, max_iter: int = 40,
-                  max_dim_factor: int = 4, profile_out: dict | None = None):
+                  max_dim_factor: int = 4, profile_out: dict | None = None, nconv: int | None = None):
     """The reference's single-k block Davidson (solvers/davidson.py:329-395) for a
     real symmetric operator on the GPU. The basis lives in preallocated buffers
     of x0's precision, every contraction over the long axis is a real MFMA
     kernel, the subspace matrix (FP64) grows by its new rows and columns
-    instead of being rebuilt, and its eigensolve is FP64."""
+    instead of being rebuilt, and its eigensolve is FP64. ``nconv`` (default:
+    every band) stops once the first ``nconv`` bands are converged; the others
+    are buffer bands, still expanded while unconverged."""
     nb, n = x0.shape
+    ng = nb if nconv is None else max(1, min(int(nconv), nb))
     dev = x0.device
     dt = x0.dtype
     f64 = torch.float64
@@ -362,7 +365,7 @@ def davidson_real(h_apply, x0: torch.Tensor, t_g: torch.Tensor, tol: float = 1e-
         r = hx - eig[:, None].to(dt) * x
         rn = torch.linalg.norm(r.to(f64), dim=1)
         bad = rn > tol
-        nbad = int(bad.sum())
+        nbad = int(bad[:ng].sum())
         if nbad == 0:
             tm(None)
             return eig, x, it, rn
@@ -456,15 +459,16 @@ def davidson_gamma_mixed(ham: GammaHamiltonian, x0: torch.Tensor, tol: float = 1
         x0 = torch.cat([x0.to(torch.complex128), xb])
     t_phi = kinetic_phi(gb)
     x32 = embed_real(gb, x0.to(torch.complex128)).to(torch.float32)
+    nconv = nb if buffer else None  # buffer bands are expanded but do not gate
     _, xd, itd, _ = davidson_real(ham.apply_real, x32, t_phi, tol=draft_tol, max_iter=max_iter,
-                                  max_dim_factor=max_dim_factor, profile_out=profile_out)
+                                  max_dim_factor=max_dim_factor, profile_out=profile_out, nconv=nconv)
     if sweeps is not None:
         if sweeps < 0:
             raise ValueError(f"sweeps must be >= 0, got {sweeps}")
         eig, x, it, rn = refine_real(ham.apply_real, xd.to(torch.float64), t_phi, int(sweeps))
     else:
         eig, x, it, rn = davidson_real(ham.apply_real, xd.to(torch.float64), t_phi, tol=tol, max_iter=max_iter,
-                                       max_dim_factor=max_dim_factor, profile_out=profile_out)
+                                       max_dim_factor=max_dim_factor, profile_out=profile_out, nconv=nconv)
     if buffer:
         eig, x, rn = eig[:nb], x[:nb], rn[:nb]
     return GammaDavidsonResult(eig, unembed_real(gb, x), it, rn, n_draft_iter=itd)

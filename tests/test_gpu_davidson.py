@@ -210,6 +210,10 @@ def test_si8_mixed_refinement_sweeps():
     assert float((rn - rb.residual_norms).abs().max()) < 1e-10
     with pytest.raises(ValueError):
         davidson_mixed(h.apply, x0, gpu.t, gpu.mask, sweeps=1, buffer=-1)
+    # nconv: only the first 16 bands gate convergence; they match the full solve
+    rc = davidson_batched(h.apply, x0, gpu.t, gpu.mask, tol=TOL, max_iter=200, nconv=16)
+    assert float(rc.residual_norms[:, :16].max()) < TOL and rc.n_iter <= r64.n_iter
+    assert float((rc.eigenvalues[:, :16] - r64.eigenvalues[:, :16]).abs().max()) < 1e-10 * HARTREE_EV
     with pytest.raises(ValueError):
         davidson_mixed(h.apply, x0, gpu.t, gpu.mask, draft_tol=1e-3, max_iter=200, sweeps=-1)
 
