@@ -56,6 +56,9 @@ SIGNATURES = {
     "gw_residual": (_I, [_I, _I, _I, _I, _P, _P, _L, _L, _P, _P, _P, _P]),
     "gw_teter_expand": (_I, [_I, _I, _I, _I, _I, _P, _P, _L, _L, _P, _L, _P, _P, _L, _L, _P]),
     "gw_row_norms": (_I, [_I, _I, _I, _I, _P, _L, _L, _P, _P]),
+    "gw_heev_workspace_bytes": (_Z, [_I, _I]),
+    "gw_heev_max_n": (_I, []),
+    "gw_heev_batched": (_I, [_I, _P, _P]),
     "gw_cg_update": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _L, _P, _L, _P, _P, _P, _P, _P]),
     "gw_cg_direction": (_I, [_I, _I, _I, _P, _P, _P, _P, _L, _P, _L, _P]),
     "gw_row_dots": (_I, [_I, _I, _I, _P, _P, _P, _L, _P, _L, _P, _P, _P]),
@@ -85,6 +88,14 @@ SIGNATURES = {
     "gw_gamma_pack": (_I, [_I, _I, _I, _I, _P, _L, _P, _P, _P, _L, _P]),
     "gw_gamma_unpack": (_I, [_I, _I, _I, _P, _L, _P, _P, _P, _L, _P]),
 }
+
+
+
+class HeevJob(C.Structure):
+    """gw_heev_job of include/gradwave_hip.h."""
+    _fields_ = [("a", _P), ("w", _P), ("z", _P), ("ws", _P),
+                ("n", _I), ("lda", _I), ("nev", _I), ("ldz", _I)]
+
 
 _lib = None
 

@@ -34,6 +34,7 @@ import numpy as np
 import torch
 
 from gradwave_amd import _lib as L
+from gradwave_amd import linalg as LA
 from gradwave_amd import workloads as W
 from gradwave_amd.davidson import BatchedDavidsonResult, _refine_steps, davidson_steps
 from gradwave_amd.hamiltonian import BatchedHamiltonian
@@ -233,6 +234,28 @@ def _eigh_batched(mats: list[torch.Tensor]) -> list[tuple[torch.Tensor, torch.Te
     return out
 
 
+def _eigh_lowest_batched(reqs: list, mode: str = "kernel") -> list:
+    """Lowest nev eigenpairs for each (matrix, nev) request. ``mode="kernel"``:
+    every matrix up to ``gw_heev_max_n`` in ONE launch of the batched
+    subspace eigensolver (one workgroup per matrix); larger ones, and
+    ``mode="library"``, take ``_eigh_batched`` (host LAPACK / rocSOLVER) and keep
+    the first nev pairs."""
+    out: list = [None] * len(reqs)
+    if mode not in ("kernel", "library"):
+        raise ValueError(f"subspace eigensolve mode must be 'kernel' or 'library', got {mode!r}")
+    small = [j for j, (s, _) in enumerate(reqs) if mode == "kernel" and int(s.shape[-1]) <= LA.heev_max_n()]
+    rest = [j for j in range(len(reqs)) if j not in set(small)]
+    if small:
+        for j, wu in zip(small, LA.eigh_lowest([reqs[j][0].to(torch.complex128) for j in small],
+                                               [reqs[j][1] for j in small])):
+            out[j] = wu
+    if rest:
+        for j, (w, u) in zip(rest, _eigh_batched([reqs[j][0] for j in rest])):
+            nev = reqs[j][1]
+            out[j] = (w[..., :nev], u[..., :nev])
+    return out
+
+
 @dataclass
 class RaggedSolveStats:
     rounds: int = 0
@@ -255,18 +278,20 @@ def _check_finite(kind: str, i: int, req, resp) -> None:
 def davidson_ragged(batch: RaggedBatch, x0s: list[torch.Tensor], ts: list[torch.Tensor],
                     masks: list[torch.Tensor], tol: float = 1e-9, max_iter: int = 40,
                     max_dim_factor: int = 4, stats: RaggedSolveStats | None = None,
-                    check_finite: bool = False) -> list[BatchedDavidsonResult]:
+                    check_finite: bool = False, subspace_eigh: str = "kernel") -> list[BatchedDavidsonResult]:
     """One ``davidson_steps`` solve per instance, advanced in lockstep: each round
     serves all pending apply requests with one ragged apply (one per precision)
-    and the pending subspace eigensolves with one batched eigensolve per size.
+    and the pending subspace eigensolves with one launch of the batched
+    subspace eigensolver (``subspace_eigh="kernel"``; "library" serves them
+    with host LAPACK / rocSOLVER, batched per size, as ``drive`` would).
     ``check_finite`` raises on the first non-finite result (debugging aid)."""
     gens = [davidson_steps(x0, t, m, tol=tol, max_iter=max_iter, max_dim_factor=max_dim_factor)
             for x0, t, m in zip(x0s, ts, masks)]
-    return _lockstep(batch, gens, stats, check_finite)
+    return _lockstep(batch, gens, stats, check_finite, subspace_eigh)
 
 
 def _lockstep(batch: RaggedBatch, gens: list, stats: RaggedSolveStats | None,
-              check_finite: bool) -> list:
+              check_finite: bool, subspace_eigh: str = "kernel") -> list:
     """Advance step generators (``davidson_steps``, ``_refine_steps``) together:
     each round serves every pending apply with one ragged apply per precision,
     every pending eigensolve with one batched call and every host read with one
@@ -315,6 +340,12 @@ def _lockstep(batch: RaggedBatch, gens: list, stats: RaggedSolveStats | None,
             for i, wu in zip(keys, _eigh_batched([eighs[i] for i in keys])):
                 resp[i] = wu
             st.batched_eighs += 1
+        lows = {i: a for i, (k, a) in pending.items() if k == "eigh_low"}
+        if lows:
+            keys = list(lows)
+            for i, wu in zip(keys, _eigh_lowest_batched([lows[i] for i in keys], subspace_eigh)):
+                resp[i] = wu
+            st.batched_eighs += 1
         t0 = mark("eigh", t0)
         hosts = {i: t for i, (k, t) in pending.items() if k == "host"}
         if hosts:  # every pending host read in one device-to-host copy
@@ -338,7 +369,8 @@ def _lockstep(batch: RaggedBatch, gens: list, stats: RaggedSolveStats | None,
 
 def davidson_ragged_mixed(batch: RaggedBatch, x0s, ts, masks, tol: float = 1e-9, draft_tol: float = 1e-4,
                           max_iter: int = 100, max_dim_factor: int = 4, sweeps: int | None = None,
-                          stats: RaggedSolveStats | None = None) -> list[BatchedDavidsonResult]:
+                          stats: RaggedSolveStats | None = None,
+                          subspace_eigh: str = "kernel") -> list[BatchedDavidsonResult]:
     """Precision form of the batch: all instances draft in complex64 (FP32 FFTs,
     V multiply and MFMA GEMMs) to ``draft_tol`` in lockstep, then polish in
     complex128 to ``tol`` from the drafted Ritz vectors, FP64 subspace algebra
@@ -347,15 +379,17 @@ def davidson_ragged_mixed(batch: RaggedBatch, x0s, ts, masks, tol: float = 1e-9,
     (no gate, ``tol`` unused), as in ``davidson_mixed``."""
     hi = [x.dtype for x in x0s]
     d = davidson_ragged(batch, [x.to(torch.complex64) for x in x0s], [t.to(torch.float32) for t in ts], masks,
-                        tol=draft_tol, max_iter=max_iter, max_dim_factor=max_dim_factor, stats=stats)
+                        tol=draft_tol, max_iter=max_iter, max_dim_factor=max_dim_factor, stats=stats,
+                        subspace_eigh=subspace_eigh)
     if sweeps is not None:
         if sweeps < 0:
             raise ValueError(f"sweeps must be >= 0, got {sweeps}")
         gens = [_refine_steps(r.eigenvectors.to(h), t, m, int(sweeps)) for r, h, t, m in zip(d, hi, ts, masks)]
-        p = _lockstep(batch, gens, stats, False)
+        p = _lockstep(batch, gens, stats, False, subspace_eigh)
     else:
         p = davidson_ragged(batch, [r.eigenvectors.to(h) for r, h in zip(d, hi)], ts, masks, tol=tol,
-                            max_iter=max_iter, max_dim_factor=max_dim_factor, stats=stats)
+                            max_iter=max_iter, max_dim_factor=max_dim_factor, stats=stats,
+                            subspace_eigh=subspace_eigh)
     for pi, di in zip(p, d):
         pi.n_apply_low += di.n_apply_full
         pi.n_draft_iter = di.n_iter

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgradwave_hip.so")
 SOURCES = ["hpsi_local.hip", "gemm.hip", "davidson_kernels.hip", "adjoint.hip", "gamma.hip", "toeplitz.hip",
-           "response.hip"]
+           "response.hip", "heev.hip"]
 ARCH = os.environ.get("GW_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

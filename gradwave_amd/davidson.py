@@ -223,8 +223,10 @@ def _apply_steps(z, low, counts):
 
 def drive(steps, h_apply):
     """Run a step generator to completion, serving its ("apply", block) requests
-    with ``h_apply``, its ("eigh", matrix) requests with ``_eigh`` and its
-    ("host", tensor) requests with a copy to the host."""
+    with ``h_apply``, its ("eigh", matrix) requests with ``_eigh``, its
+    ("eigh_low", (matrix, nev)) requests (only the lowest nev pairs are used)
+    with the first nev pairs of ``_eigh`` and its ("host", tensor) requests
+    with a copy to the host."""
     try:
         req = next(steps)
         while True:
@@ -233,6 +235,9 @@ def drive(steps, h_apply):
                 resp = h_apply(arg)
             elif kind == "eigh":
                 resp = _eigh(arg)
+            elif kind == "eigh_low":
+                w, u = _eigh(arg[0])
+                resp = (w[..., : arg[1]], u[..., : arg[1]])
             else:
                 resp = arg.cpu()
             req = steps.send(resp)
@@ -263,7 +268,7 @@ def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: f
                    max_iter: int = 40, max_dim_factor: int = 4, history_out: list | None = None,
                    profile_out: dict | None = None, nconv: int | None = None):
     """The solver of ``davidson_batched`` as a step generator: it yields
-    ("apply", block) and ("eigh", subspace matrix) requests and returns the
+    ("apply", block) and ("eigh_low", (subspace matrix, nb)) requests and returns the
     BatchedDavidsonResult, so a driver can serve many independent solves in
     lockstep with one batched apply and one batched eigensolve per round."""
     nk, nb, m = x0.shape
@@ -293,7 +298,7 @@ def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: f
     for it in range(1, max_iter + 1):
         v, hv = basis.views()
         tm("eigh")
-        w, u = yield ("eigh", basis.subspace())
+        w, u = yield ("eigh_low", (basis.subspace(), nb))
         u = u[:, :, :nb].to(v.dtype).contiguous()
         eig = w[:, :nb].real.to(rdtype)
         tm("ritz_gemm")
@@ -386,7 +391,7 @@ def _refine_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, sweeps:
     sel = torch.arange(nb, device=x0.device).expand(nk, nb).contiguous()
     for sweep in range(sweeps + 1):
         v, hv = basis.views()
-        w, u = yield ("eigh", basis.subspace())
+        w, u = yield ("eigh_low", (basis.subspace(), nb))
         u = u[:, :, :nb].to(v.dtype).contiguous()
         eig = w[:, :nb].real.to(x0.real.dtype)
         x, hx = LA.combine(u, v), LA.combine(u, hv)

@@ -159,3 +159,59 @@ def rcombine(u: torch.Tensor, y: torch.Tensor, out: torch.Tensor | None = None, 
                                      yb.stride(1), yb.stride(0), ob.data_ptr(), ob.stride(1), ob.stride(0),
                                      float(alpha), float(beta), L.stream_ptr(y.device)), "gw_rgemm_combine")
     return out
+
+
+_HEEV_WS: dict[tuple, torch.Tensor] = {}
+
+
+def heev_max_n() -> int:
+    return int(L.lib().gw_heev_max_n())
+
+
+def eigh_lowest(mats: list, nevs: list) -> list:
+    """Lowest ``nev`` eigenpairs of Hermitian complex128 matrices, all in one
+    batched kernel launch (``gw_heev_batched``): the subspace eigensolve of
+    solvers/davidson.py:640-652, of which davidson.py:651-652 keeps the first nb
+    Ritz pairs. ``mats[i]`` is (nk, m_i, m_i) on the GPU with a contiguous last
+    axis (views such as ``s[:, :m, :m]`` are read in place); returns
+    [(w (nk, nev_i) float64 ascending, u (nk, m_i, nev_i) complex128)]. Sizes
+    above ``heev_max_n()`` raise ``GWError``."""
+    if not mats:
+        return []
+    lib = L.lib()
+    dev = mats[0].device
+    out, jobs, sizes = [], [], []
+    for s, nev in zip(mats, nevs):
+        L.require_gpu(s, "s")
+        if s.dtype != torch.complex128 or s.dim() != 3 or s.shape[1] != s.shape[2]:
+            raise TypeError(f"expected (nk, m, m) complex128, got {tuple(s.shape)} {s.dtype}")
+        if s.stride(-1) != 1:
+            s = s.contiguous()
+        nk, m = int(s.shape[0]), int(s.shape[1])
+        nev = int(nev)
+        need = int(lib.gw_heev_workspace_bytes(m, nev))
+        if need == 0:
+            raise L.GWError(f"eigh_lowest: unsupported size m={m}, nev={nev} (max m {heev_max_n()})")
+        w = torch.empty(nk, nev, dtype=torch.float64, device=dev)
+        u = torch.empty(nk, m, nev, dtype=torch.complex128, device=dev)
+        out.append((w, u))
+        for k in range(nk):
+            jobs.append((s, k, w, u, m, nev))
+            sizes.append(need)
+    total = sum(sizes)
+    key = (dev.type, dev.index)
+    ws = _HEEV_WS.get(key)
+    if ws is None or ws.numel() < total:
+        ws = torch.empty(max(total, 1 << 20), dtype=torch.uint8, device=dev)
+        _HEEV_WS[key] = ws
+    arr = (L.HeevJob * len(jobs))()
+    off = 0
+    for j, ((s, k, w, u, m, nev), need) in enumerate(zip(jobs, sizes)):
+        arr[j].a = s.data_ptr() + k * s.stride(0) * 16
+        arr[j].w = w.data_ptr() + k * nev * 8
+        arr[j].z = u.data_ptr() + k * m * nev * 16
+        arr[j].ws = ws.data_ptr() + off
+        arr[j].n, arr[j].lda, arr[j].nev, arr[j].ldz = m, int(s.stride(1)), nev, nev
+        off += need
+    L.check(lib.gw_heev_batched(len(jobs), arr, L.stream_ptr(dev)), "gw_heev_batched")
+    return out

@@ -169,6 +169,35 @@ int gw_teter_expand(int precision, int nk, int nb, int nsel, int npw, const void
 int gw_row_norms(int precision, int nk, int m, int npw, const void* v, long ld, long s,
                  double* out, void* stream);
 
+/* ---- batched subspace eigensolve ----
+ * Replaces the subspace eigensolve of davidson_batched (davidson.py:640-652,
+ * torch.linalg.eigh of the Hermitised (nk, m, m) matrix, of which only the
+ * first nb Ritz pairs are used, davidson.py:651-652) for many independent
+ * matrices of different sizes: the lowest nev eigenpairs of each, one
+ * workgroup per matrix, all matrices in one launch (32 per launch).
+ * a: n x n complex128, row-major with leading dimension lda, Hermitised as
+ *    (A + A^H) / 2 on load (read only);
+ * w: nev float64, ascending; z: n x nev complex128 row-major (ldz), column j
+ *    the unit eigenvector of w[j];
+ * ws: gw_heev_workspace_bytes(n, nev) bytes of device memory per job
+ *    (16-byte aligned, not shared between jobs of one call).
+ * 1 <= n <= gw_heev_max_n() (GW_ERR_UNSUPPORTED above), 1 <= nev <= n.
+ * Householder tridiagonalisation, multisection on Sturm counts, inverse
+ * iteration with reorthogonalisation inside clusters closer than 1e-3 |T|,
+ * back-transformation; eigenvalues to a few ulp of |T|, eigenvectors
+ * orthonormal to ~1e-13. The call copies the job array into the launch
+ * arguments, so the host array may be reused as soon as it returns. */
+typedef struct gw_heev_job {
+  const void* a;
+  void* w;
+  void* z;
+  void* ws;
+  int n, lda, nev, ldz;
+} gw_heev_job;
+size_t gw_heev_workspace_bytes(int n, int nev);
+int gw_heev_max_n(void);
+int gw_heev_batched(int count, const gw_heev_job* jobs, void* stream);
+
 /* ---- Sternheimer conjugate-gradient column kernels ----
  * Replace the per-column updates of cg_sternheimer / _cg_percolumn
  * (postscf/_response.py:502-690). The CG vectors x, p, r, z, ap live in the

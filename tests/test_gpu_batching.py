@@ -113,20 +113,21 @@ def test_ragged_graph_replay_matches_eager():
         assert len(other) == len(wls)
 
 
-def test_lockstep_davidson_matches_independent_solves():
+@pytest.mark.parametrize("subspace_eigh", ["kernel", "library"])
+def test_lockstep_davidson_matches_independent_solves(subspace_eigh):
     from gradwave_amd import batching as B
     from gradwave_amd.davidson import davidson_batched
     from gradwave_amd.workloads import start_vectors
 
     wls = _instances()
-    say("lockstep davidson vs independent")
+    say(f"lockstep davidson vs independent ({subspace_eigh} subspace eigensolves)")
     hams = _hams(wls)
     batch = B.RaggedBatch(hams)
     tol = 1e-10 * HARTREE_EV
     x0s = [start_vectors(w, seed=i) for i, w in enumerate(wls)]
     stats = B.RaggedSolveStats()
     res = B.davidson_ragged(batch, x0s, [w.bk.t for w in wls], [w.bk.mask for w in wls], tol=tol, max_iter=200,
-                            stats=stats, check_finite=True)
+                            stats=stats, check_finite=True, subspace_eigh=subspace_eigh)
     say(f"rounds {stats.rounds} applies {stats.batched_applies} eighs {stats.batched_eighs}")
     for i, (w, h, x0, r) in enumerate(zip(wls, hams, x0s, res)):
         ref = davidson_batched(h.apply, x0, w.bk.t, w.bk.mask, tol=tol, max_iter=200)
@@ -134,13 +135,13 @@ def test_lockstep_davidson_matches_independent_solves():
         assert float((r.eigenvalues - ref.eigenvalues).abs().max()) / HARTREE_EV < 1e-10, (i, w.name)
         assert abs(r.n_iter - ref.n_iter) <= 2, (i, w.name, r.n_iter, ref.n_iter)
     mixed = B.davidson_ragged_mixed(batch, x0s, [w.bk.t for w in wls], [w.bk.mask for w in wls], tol=tol,
-                                    draft_tol=1e-3, max_iter=200)
+                                    draft_tol=1e-3, max_iter=200, subspace_eigh=subspace_eigh)
     for r, m in zip(res, mixed):
         assert float(m.residual_norms.max()) < tol
         assert float((m.eigenvalues - r.eigenvalues).abs().max()) / HARTREE_EV < 1e-10
     # sweep form in lockstep: each instance as its own draft + one FP64 sweep would give
     swept = B.davidson_ragged_mixed(batch, x0s, [w.bk.t for w in wls], [w.bk.mask for w in wls],
-                                    draft_tol=1e-3, max_iter=200, sweeps=1)
+                                    draft_tol=1e-3, max_iter=200, sweeps=1, subspace_eigh=subspace_eigh)
     for i, (w, h, x0, r, sw) in enumerate(zip(wls, hams, x0s, res, swept)):
         assert sw.n_iter == 1
         assert float((sw.eigenvalues - r.eigenvalues).min()) > -1e-12 * HARTREE_EV, (i, w.name)

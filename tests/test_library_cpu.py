@@ -69,6 +69,17 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert err(lib.gw_gamma_pack(0, 0, 5, 5, None, 5, None, None, None, 5, None))
     assert err(lib.gw_vloc_g(1, 8, None, None, None, None, 0.0, None, None))
     assert err(lib.gw_proj_phase(1, 1, 1, None, None, None, None, None, None, None))
+    # batched subspace eigensolve: sizes and pointers are checked before the launch
+    big = lib.gw_heev_max_n() + 1
+    assert f"outside [1, {big - 1}]" in err(lib.gw_heev_batched(1, C.byref(L.HeevJob(None, None, None, None,
+                                                                                     big, big, 1, 1)), None))
+    assert "nev = 5" in err(lib.gw_heev_batched(1, C.byref(L.HeevJob(None, None, None, None, 4, 4, 5, 5)), None))
+    assert "lda 3 < n" in err(lib.gw_heev_batched(1, C.byref(L.HeevJob(None, None, None, None, 4, 3, 2, 2)), None))
+    assert "null pointer" in err(lib.gw_heev_batched(1, C.byref(L.HeevJob(None, None, None, None, 4, 4, 2, 2)),
+                                                     None))
+    assert lib.gw_heev_workspace_bytes(big, 1) == 0 and lib.gw_heev_workspace_bytes(4, 5) == 0
+    assert lib.gw_heev_workspace_bytes(4, 2) % 256 == 0 and lib.gw_heev_workspace_bytes(4, 2) > 0
+    assert lib.gw_heev_batched(0, None, None) == 0
     # a box length that no two codelets <= 16 can split (grids.py sizes are 7-smooth, 11 is not)
     flat = (C.c_int64 * 1)(0)
     npw = (C.c_int64 * 1)(1)
