@@ -19,7 +19,7 @@
 //   4. computes T's eigenvectors by inverse iteration (LU of T - lambda I with
 //      partial pivoting, tiny pivots raised to eps |T|, three steps from a
 //      hashed start vector), orthogonalising each vector against the earlier
-//      members of its cluster (eigenvalues closer than 1e-3 |T|) in every step;
+//      members of its cluster (eigenvalues closer than 1e-5 |T|) in every step;
 //      clusters are processed member by member, all clusters at once;
 //   5. back-transforms u = H_0 H_1 ... H_{n-2} z into the caller's output.
 // Every loop has a fixed bound (n, nev, 64 multisection steps, 3 inverse
@@ -61,10 +61,29 @@ __device__ __forceinline__ double2 hc_cmul(double2 a, double2 b) {  // conj(a) b
   return double2{a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
 }
 
+// A double moved between lanes by a DPP pattern (both 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double read_lane_d(double v, int lane) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// Sum over the 64 lanes (all active), the same in every lane: butterflies inside
+// each row of 16 lanes (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror,
+// row_mirror), then the four row sums read from lanes 0, 16, 32, 48.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  return (read_lane_d(v, 0) + read_lane_d(v, 16)) + (read_lane_d(v, 32) + read_lane_d(v, 48));
 }
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
@@ -131,51 +150,40 @@ __device__ __forceinline__ void rank2_sub(double2& a, double2 vi, double2 wi, do
   a.y -= vi.y * wk.x - vi.x * wk.y + wi.y * vk.x - wi.x * vk.y;
 }
 
-// Reflector for x = src[0..L-1] (LDS): H^H x = beta e_1 with H = I - tau v v^H,
-// v[0] = 1, beta real. v goes to vdst[0..L-1] (LDS) and to gdst[1..L-1]
-// (the workspace column kept for the back-transformation); the results are
-// the same in every thread. Ends with a barrier.
-__device__ void house(const double2* src, double2* gdst, int L, double2* vdst, double* red, double2& tau,
-                      double& beta, bool& skip) {
-  double p = 0;
-  for (int i = 1 + (int)threadIdx.x; i < L; i += kHeevThreads) { const double2 x = src[i]; p += x.x * x.x + x.y * x.y; }
-  const double xn2 = block_sum(p, red);
-  const double2 al = src[0];
-  skip = xn2 == 0.0 && al.y == 0.0;
+// Reflector scalars for x with x[0] = al and |x[1:]|^2 = xn2: H^H x = beta e_1
+// with H = I - tau v v^H, v = (1, x[1:] sc), beta real, sc = 1 / (al - beta).
+// x = 0 (or x = al e_1 with al real) gives tau = 0, H = I.
+__device__ __forceinline__ void house_scalars(double2 al, double xn2, double2& tau, double& beta, double2& sc) {
   beta = al.x;
   tau = double2{0, 0};
-  double2 sc{1, 0};
-  if (!skip) {
-    beta = -copysign(sqrt(al.x * al.x + al.y * al.y + xn2), al.x);
-    tau = double2{(beta - al.x) / beta, -al.y / beta};
-    const double2 den{al.x - beta, al.y};
-    const double dd = den.x * den.x + den.y * den.y;
-    sc = double2{den.x / dd, -den.y / dd};
-  }
-  for (int i = threadIdx.x; i < L; i += kHeevThreads) {
-    double2 v{1, 0};
-    if (i > 0) { v = hc_mul(src[i], sc); gdst[i] = v; }
-    vdst[i] = v;
-  }
-  __syncthreads();
+  sc = double2{1, 0};
+  if (xn2 == 0.0 && al.y == 0.0) return;
+  beta = -copysign(sqrt(al.x * al.x + al.y * al.y + xn2), al.x);
+  const double rb = 1.0 / beta;
+  tau = double2{(beta - al.x) * rb, -al.y * rb};
+  const double dx = al.x - beta, dy = al.y;
+  const double rd = 1.0 / (dx * dx + dy * dy);
+  sc = double2{dx * rd, -dy * rd};
 }
 
 constexpr int kHeevU = 4;  // loads in flight per thread in the passes over the trailing block
 
 // One pass over rows and columns [first, L) of the trailing block a (element
 // (i, k) at a[i + k n]): with UPD, a -= v w^H + w v^H (v = vc, w = sw, indexed
-// like the rows); then y[i - first] = sum_k a(i, k) vn[k - first]. Rows are
-// spread over the threads and, when there are fewer rows than threads, the
-// columns over NT / rows groups whose partial sums meet in sred. Ends with a
-// barrier. Column `first` after the pass also goes to scol[i - first]: it is
-// the first column of the next trailing block.
+// like the rows); then sy[i - first] = tau1 sum_k a(i, k) vn[k - first] and
+// the partial sums of sy^H vn per wave go to red2 (re, then im). Column
+// `first` after the update goes to scol[i - first]: it is the first column of
+// the next trailing block. Rows are spread over the threads and, when there
+// are fewer rows than threads, the columns over NT / rows groups whose partial
+// sums meet in sred. Ends with a barrier.
 template <bool UPD>
 __device__ void trail_pass(double2* a22, int n, int first, int L, const double2* vc, const double2* sw,
-                           const double2* vn, double2* sy, double2* sred, double2* scol) {
+                           const double2* vn, double2 tau1, double2* sy, double2* sred, double2* scol,
+                           double* red2) {
   constexpr int NT = kHeevThreads, U = kHeevU;
   const int R = L - first, tid = threadIdx.x;
-  if (R <= 0) return;
-  const int ng = R >= NT ? 1 : NT / R;
+  const int ng = R >= NT ? 1 : min(8, NT / R);  // column groups (a short final sum)
+  double2 dp{0, 0};
   if (tid < (R >= NT ? NT : ng * R)) {
     const int s = R >= NT ? 0 : tid / R;
     for (int ii = R >= NT ? tid : tid % R; ii < R; ii += NT) {
@@ -203,19 +211,33 @@ __device__ void trail_pass(double2* a22, int n, int first, int L, const double2*
           }
         }
       }
-      if (ng == 1) sy[ii] = acc;
-      else sred[s * R + ii] = acc;
+      if (ng == 1) {
+        const double2 yv = hc_mul(tau1, acc);
+        sy[ii] = yv;
+        const double2 t = hc_cmul(yv, vn[ii]);
+        dp.x += t.x;
+        dp.y += t.y;
+      } else {
+        sred[s * R + ii] = acc;
+      }
     }
   }
-  __syncthreads();
   if (ng > 1) {
+    __syncthreads();
     for (int ii = tid; ii < R; ii += NT) {
       double2 sum{0, 0};
       for (int g = 0; g < ng; ++g) { sum.x += sred[g * R + ii].x; sum.y += sred[g * R + ii].y; }
-      sy[ii] = sum;
+      const double2 yv = hc_mul(tau1, sum);
+      sy[ii] = yv;
+      const double2 t = hc_cmul(yv, vn[ii]);
+      dp.x += t.x;
+      dp.y += t.y;
     }
-    __syncthreads();
   }
+  dp.x = wave_sum(dp.x);
+  dp.y = wave_sum(dp.y);
+  if ((tid & 63) == 0) { red2[tid >> 6] = dp.x; red2[kHeevWaves + (tid >> 6)] = dp.y; }
+  __syncthreads();
 }
 
 // Number of eigenvalues of T (diagonal d, off-diagonal e) below x: signs of
@@ -264,7 +286,7 @@ __global__ void __launch_bounds__(kHeevThreads) k_heev(HeevArgs args) {
   __shared__ double sd[kHeevMaxN], se[kHeevMaxN], slam[kHeevMaxN];
   __shared__ int scs[kHeevMaxN];
   __shared__ double2 sred[NT];
-  __shared__ double red[2 * kHeevWaves];
+  __shared__ double red[2 * kHeevWaves], red2[2 * kHeevWaves];
 
   // 1. W (column-major) = (A + A^H) / 2
   for (long idx = tid; idx < (long)n * n; idx += NT) {
@@ -277,74 +299,77 @@ __global__ void __launch_bounds__(kHeevThreads) k_heev(HeevArgs args) {
   }
   __syncthreads();
 
-  // 2. Householder tridiagonalisation. The rank-2 update of step j and the
-  // product A v of step j+1 share one pass over the trailing block: its first
-  // column is updated first, which gives the reflector of step j+1.
+  // 2. Householder tridiagonalisation. Per step j: (1) w = tau y - (tau/2)
+  // (tau y)^H v v from the pass's tau y and partial dot products, the update
+  // of the trailing block's first column (kept in LDS) and the norm of the
+  // next x; (2) the reflector of step j + 1; (3) one pass over the rest of
+  // the block: rank-2 update of step j and tau y of step j + 1. Three or four
+  // barriers per step.
   constexpr int U = kHeevU;
   double2 *vc = sv, *vn = svn;
-  bool skip = true;
   double2 tau{0, 0};
   if (n > 1) {
+    double p = 0;
+    for (int i = 1 + tid; i < n - 1; i += NT) { const double2 x = scol[i]; p += x.x * x.x + x.y * x.y; }
+    const double xn2 = block_sum(p, red);
     double beta;
-    house(scol, W + 1, n - 1, vc, red, tau, beta, skip);
+    double2 sc;
+    house_scalars(scol[0], xn2, tau, beta, sc);
+    for (int i = tid; i < n - 1; i += NT) {
+      double2 v{1, 0};
+      if (i > 0) { v = hc_mul(scol[i], sc); W[1 + i] = v; }
+      vc[i] = v;
+    }
     if (tid == 0) { se[0] = beta; stau[0] = tau; }
-    trail_pass<false>(W + n + 1, n, 0, n - 1, vc, sw, vc, sy, sred, scol);
+    __syncthreads();
+    trail_pass<false>(W + n + 1, n, 0, n - 1, vc, sw, vc, tau, sy, sred, scol, red2);
   } else if (tid == 0) {
     se[0] = 0.0;
   }
   for (int j = 0; j < n - 1; ++j) {
     const int L = n - j - 1;
     double2* a22 = W + (long)(j + 1) * n + (j + 1);
-    if (!skip) {  // w = tau y - (tau/2) (tau y)^H v v
-      double2 pd{0, 0};
-      for (int i = tid; i < L; i += NT) {
-        const double2 ty = hc_mul(tau, sy[i]);
-        sw[i] = ty;
-        const double2 t = hc_cmul(ty, vc[i]);
-        pd.x += t.x;
-        pd.y += t.y;
-      }
-      const double2 dot = block_sum2(pd, red);
-      const double2 td = hc_mul(tau, dot);
-      const double2 a2{-0.5 * td.x, -0.5 * td.y};
-      for (int i = tid; i < L; i += NT) {
-        const double2 t = hc_mul(a2, vc[i]);
-        sw[i] = double2{sw[i].x + t.x, sw[i].y + t.y};
-      }
-      __syncthreads();
+    // (1)
+    double2 dot{0, 0};
+#pragma unroll
+    for (int w = 0; w < kHeevWaves; ++w) { dot.x += red2[w]; dot.y += red2[kHeevWaves + w]; }
+    const double2 td = hc_mul(tau, dot);
+    const double2 a2{-0.5 * td.x, -0.5 * td.y};
+    const double2 v0 = vc[0], y0 = sy[0];
+    const double2 w0{y0.x + a2.x * v0.x - a2.y * v0.y, y0.y + a2.x * v0.y + a2.y * v0.x};
+    double p = 0;
+    for (int i = tid; i < L; i += NT) {
+      const double2 vi = vc[i], yi = sy[i];
+      const double2 wi{yi.x + a2.x * vi.x - a2.y * vi.y, yi.y + a2.x * vi.y + a2.y * vi.x};
+      sw[i] = wi;
+      double2 a = scol[i];
+      rank2_sub(a, vi, wi, v0, w0);
+      scol[i] = a;
+      if (i >= 2) p += a.x * a.x + a.y * a.y;  // |x[1:]|^2 of the next reflector's x = scol[1..]
     }
     if (L == 1) {  // last diagonal element
-      if (tid == 0) {
-        double2 a = scol[0];
-        if (!skip) a.x -= 2.0 * (vc[0].x * sw[0].x + vc[0].y * sw[0].y);
-        sd[n - 1] = a.x;
-        se[n - 1] = 0.0;
-      }
+      if (tid == 0) { sd[n - 1] = scol[0].x; se[n - 1] = 0.0; }
       __syncthreads();
       break;
     }
-    // (a) first column of the trailing block (kept in LDS), diagonal included
-    if (!skip)
-      for (int i = tid; i < L; i += NT) {
-        double2 a = scol[i];
-        rank2_sub(a, vc[i], sw[i], vc[0], sw[0]);
-        scol[i] = a;
-      }
-    __syncthreads();
+    const double xn2 = block_sum(p, red);
+    // (2)
     if (tid == 0) sd[j + 1] = scol[0].x;
-    // (b) reflector of step j + 1 from rows 1..L-1 of that column
-    double2 tau1;
+    double2 tau1, sc;
     double beta1;
-    bool skip1;
-    house(scol + 1, a22 + 1, L - 1, vn, red, tau1, beta1, skip1);
+    house_scalars(scol[1], xn2, tau1, beta1, sc);
+    for (int i = tid; i < L - 1; i += NT) {
+      double2 v{1, 0};
+      if (i > 0) { v = hc_mul(scol[1 + i], sc); a22[1 + i] = v; }
+      vn[i] = v;
+    }
     if (tid == 0) { se[j + 1] = beta1; stau[j + 1] = tau1; }
-    // (c) columns 1..L-1, rows 1..L-1: update of step j, then y = A v of step j + 1
-    if (skip) trail_pass<false>(a22, n, 1, L, vc, sw, vn, sy, sred, scol);
-    else trail_pass<true>(a22, n, 1, L, vc, sw, vn, sy, sred, scol);
+    __syncthreads();
+    // (3)
+    trail_pass<true>(a22, n, 1, L, vc, sw, vn, tau1, sy, sred, scol, red2);
     double2* t = vc;
     vc = vn;
     vn = t;
-    skip = skip1;
     tau = tau1;
   }
   __syncthreads();
@@ -404,7 +429,13 @@ __global__ void __launch_bounds__(kHeevThreads) k_heev(HeevArgs args) {
   __syncthreads();
 
   // 4. eigenvectors of T: inverse iteration, clusters member by member
-  const double ortol = 1e-3 * tn, ptol = eps * tn;
+  // Clusters: eigenvalues closer than 1e-5 |T| (inverse-iteration vectors of
+  // eigenvalues further apart are orthogonal to ~eps |T| / gap <= 1e-11). The
+  // 1e-3 |T| of LAPACK's dstein would put all wanted eigenvalues of a
+  // Davidson subspace matrix (|T| set by high-kinetic correction directions,
+  // 1e2-1e4 eV, against a few eV between bands) into one cluster, processed
+  // member by member.
+  const double ortol = 1e-5 * tn, ptol = eps * tn;
   double pmax = 0;
   for (int t = tid; t < nev; t += NT) {
     int cs = t;
@@ -495,7 +526,7 @@ __global__ void __launch_bounds__(kHeevThreads) k_heev(HeevArgs args) {
   // The update with H_i and the projection v_{i-1}^H z of the next reflector
   // share one pass over rows i..n-1 (H_i leaves row i alone).
   const int nc = nev < NT ? nev : NT;
-  const int G = NT / nc;
+  const int G = min(16, NT / nc);  // row groups per column (a short final sum)
   for (int cb = 0; cb < nev; cb += nc) {
     const int cl = tid % nc, g = tid / nc, c = cb + cl;
     const bool act = g < G && c < nev;

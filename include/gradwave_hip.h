@@ -183,9 +183,10 @@ int gw_row_norms(int precision, int nk, int m, int npw, const void* v, long ld, 
  *    (16-byte aligned, not shared between jobs of one call).
  * 1 <= n <= gw_heev_max_n() (GW_ERR_UNSUPPORTED above), 1 <= nev <= n.
  * Householder tridiagonalisation, multisection on Sturm counts, inverse
- * iteration with reorthogonalisation inside clusters closer than 1e-3 |T|,
+ * iteration with reorthogonalisation inside clusters closer than 1e-5 |T|,
  * back-transformation; eigenvalues to a few ulp of |T|, eigenvectors
- * orthonormal to ~1e-13. The call copies the job array into the launch
+ * orthonormal to ~eps |T| / (gap to the nearest eigenvalue outside the
+ * cluster), at most ~1e-11. The call copies the job array into the launch
  * arguments, so the host array may be reused as soon as it returns. */
 typedef struct gw_heev_job {
   const void* a;
