@@ -59,6 +59,8 @@ SIGNATURES = {
     "gw_heev_workspace_bytes": (_Z, [_I, _I]),
     "gw_heev_max_n": (_I, []),
     "gw_heev_batched": (_I, [_I, _P, _P]),
+    "gw_chol_max_n": (_I, []),
+    "gw_chol_inv_batched": (_I, [_I, _P, _P]),
     "gw_cg_update": (_I, [_I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _L, _P, _L, _P, _P, _P, _P, _P]),
     "gw_cg_direction": (_I, [_I, _I, _I, _P, _P, _P, _P, _L, _P, _L, _P]),
     "gw_row_dots": (_I, [_I, _I, _I, _P, _P, _P, _L, _P, _L, _P, _P, _P]),
@@ -95,6 +97,11 @@ class HeevJob(C.Structure):
     """gw_heev_job of include/gradwave_hip.h."""
     _fields_ = [("a", _P), ("w", _P), ("z", _P), ("ws", _P),
                 ("n", _I), ("lda", _I), ("nev", _I), ("ldz", _I)]
+
+
+class CholJob(C.Structure):
+    """gw_chol_job of include/gradwave_hip.h."""
+    _fields_ = [("a", _P), ("linv", _P), ("info", _P), ("n", _I), ("lda", _I), ("ldl", _I)]
 
 
 _lib = None

@@ -36,7 +36,7 @@ import torch
 from gradwave_amd import _lib as L
 from gradwave_amd import linalg as LA
 from gradwave_amd import workloads as W
-from gradwave_amd.davidson import BatchedDavidsonResult, _refine_steps, davidson_steps
+from gradwave_amd.davidson import BatchedDavidsonResult, _cholinv_torch, _refine_steps, davidson_steps
 from gradwave_amd.hamiltonian import BatchedHamiltonian
 
 
@@ -256,6 +256,22 @@ def _eigh_lowest_batched(reqs: list, mode: str = "kernel") -> list:
     return out
 
 
+def _cholinv_batched(gs: list, mode: str = "kernel") -> list:
+    """(L^-1, info) for each Gram matrix G = L L^H. ``mode="kernel"``: every
+    matrix up to ``gw_chol_max_n`` in ONE launch of the batched Cholesky
+    inverse; larger ones, and ``mode="library"``, take torch (rocSOLVER potrf,
+    rocBLAS trsm) one by one as ``drive`` does."""
+    out: list = [None] * len(gs)
+    small = [j for j, g in enumerate(gs) if mode == "kernel" and int(g.shape[-1]) <= LA.chol_max_n()]
+    if small:
+        for j, r in zip(small, LA.cholesky_inverse([gs[j] for j in small])):
+            out[j] = r
+    for j in range(len(gs)):
+        if out[j] is None:
+            out[j] = _cholinv_torch(gs[j])
+    return out
+
+
 @dataclass
 class RaggedSolveStats:
     rounds: int = 0
@@ -281,9 +297,10 @@ def davidson_ragged(batch: RaggedBatch, x0s: list[torch.Tensor], ts: list[torch.
                     check_finite: bool = False, subspace_eigh: str = "kernel") -> list[BatchedDavidsonResult]:
     """One ``davidson_steps`` solve per instance, advanced in lockstep: each round
     serves all pending apply requests with one ragged apply (one per precision)
-    and the pending subspace eigensolves with one launch of the batched
-    subspace eigensolver (``subspace_eigh="kernel"``; "library" serves them
-    with host LAPACK / rocSOLVER, batched per size, as ``drive`` would).
+    and the pending subspace eigensolves and Cholesky inverses (CholQR2) with
+    one launch each of the batched kernels (``subspace_eigh="kernel"``;
+    "library" serves them with host LAPACK / rocSOLVER / rocBLAS, as ``drive``
+    would).
     ``check_finite`` raises on the first non-finite result (debugging aid)."""
     gens = [davidson_steps(x0, t, m, tol=tol, max_iter=max_iter, max_dim_factor=max_dim_factor)
             for x0, t, m in zip(x0s, ts, masks)]
@@ -346,6 +363,11 @@ def _lockstep(batch: RaggedBatch, gens: list, stats: RaggedSolveStats | None,
             for i, wu in zip(keys, _eigh_lowest_batched([lows[i] for i in keys], subspace_eigh)):
                 resp[i] = wu
             st.batched_eighs += 1
+        chols = {i: g for i, (k, g) in pending.items() if k == "cholinv"}
+        if chols:
+            keys = list(chols)
+            for i, r in zip(keys, _cholinv_batched([chols[i] for i in keys], subspace_eigh)):
+                resp[i] = r
         t0 = mark("eigh", t0)
         hosts = {i: t for i, (k, t) in pending.items() if k == "host"}
         if hosts:  # every pending host read in one device-to-host copy

@@ -584,6 +584,86 @@ __global__ void __launch_bounds__(kHeevThreads) k_heev(HeevArgs args) {
   }
 }
 
+// ---- batched Cholesky factor inverse (the CholQR2 step of the orthonormalisation)
+//
+// For each Hermitian positive definite G (n <= kCholMaxN, lower triangle read):
+// G = L L^H, and X = L^-1 (lower triangular) is written; info = 0, or the
+// 1-based index of the first non-positive pivot (the factor is then not
+// usable, as for torch.linalg.cholesky_ex). One workgroup per matrix, both
+// triangles in LDS: right-looking factorisation (three barriers per column),
+// then each thread forms one column of L^-1 by forward substitution.
+constexpr int kCholMaxN = 64;
+constexpr int kCholThreads = 256;
+constexpr int kCholPerLaunch = 32;
+
+struct CholArgs {
+  gw_chol_job j[kCholPerLaunch];
+  int count;
+};
+
+__global__ void __launch_bounds__(kCholThreads) k_chol_inv(CholArgs args) {
+  const gw_chol_job J = args.j[blockIdx.x];
+  const int n = J.n, tid = threadIdx.x;
+  constexpr int NT = kCholThreads;
+  __shared__ double2 La[kCholMaxN * kCholMaxN];
+  __shared__ double2 Xs[kCholMaxN * kCholMaxN];
+  __shared__ double rdiag;
+  __shared__ int sinfo;
+  const double2* A = reinterpret_cast<const double2*>(J.a);
+  double2* Out = reinterpret_cast<double2*>(J.linv);
+  for (int idx = tid; idx < n * n; idx += NT) {
+    const int i = idx / n, k = idx - i * n;
+    La[idx] = k <= i ? A[(long)i * J.lda + k] : double2{0, 0};
+  }
+  if (tid == 0) sinfo = 0;
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    if (tid == 0) {
+      double d = La[j * n + j].x;
+      if (!(d > 0.0)) {
+        if (sinfo == 0) sinfo = j + 1;
+        d = 1.0;
+      }
+      const double l = sqrt(d);
+      La[j * n + j] = double2{l, 0.0};
+      rdiag = 1.0 / l;
+    }
+    __syncthreads();
+    const double r = rdiag;
+    for (int i = j + 1 + tid; i < n; i += NT) { La[i * n + j].x *= r; La[i * n + j].y *= r; }
+    __syncthreads();
+    const int m = n - j - 1;
+    for (int q = tid; q < m * m; q += NT) {
+      const int ii = q / m, kk = q - ii * m;
+      if (kk > ii) continue;
+      const int i = j + 1 + ii, k = j + 1 + kk;
+      const double2 a = La[i * n + j], b = La[k * n + j];
+      La[i * n + k].x -= a.x * b.x + a.y * b.y;
+      La[i * n + k].y -= a.y * b.x - a.x * b.y;
+    }
+    __syncthreads();
+  }
+  for (int c = tid; c < n; c += NT) {
+    Xs[c * n + c] = double2{1.0 / La[c * n + c].x, 0.0};
+    for (int i = c + 1; i < n; ++i) {
+      double2 acc{0, 0};
+      for (int k = c; k < i; ++k) {
+        const double2 l = La[i * n + k], x = Xs[k * n + c];
+        acc.x += l.x * x.x - l.y * x.y;
+        acc.y += l.x * x.y + l.y * x.x;
+      }
+      const double inv = 1.0 / La[i * n + i].x;
+      Xs[i * n + c] = double2{-acc.x * inv, -acc.y * inv};
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < n * n; idx += NT) {
+    const int i = idx / n, k = idx - i * n;
+    Out[(long)i * J.ldl + k] = k <= i ? Xs[idx] : double2{0, 0};
+  }
+  if (tid == 0) *reinterpret_cast<int*>(J.info) = sinfo;
+}
+
 }  // namespace gw
 
 using namespace gw;
@@ -615,6 +695,32 @@ extern "C" int gw_heev_batched(int count, const gw_heev_job* jobs, void* stream)
     args.count = std::min(kHeevPerLaunch, count - b);
     for (int i = 0; i < args.count; ++i) args.j[i] = jobs[b + i];
     hipLaunchKernelGGL(k_heev, dim3(args.count), dim3(kHeevThreads), 0, st, args);
+    GW_CHECK_HIP(hipGetLastError());
+  }
+  return GW_OK;
+}
+
+extern "C" int gw_chol_max_n(void) { return kCholMaxN; }
+
+extern "C" int gw_chol_inv_batched(int count, const gw_chol_job* jobs, void* stream) {
+  GW_REQUIRE(count >= 0, GW_ERR_ARG, "negative job count");
+  if (count == 0) return GW_OK;
+  GW_REQUIRE(jobs, GW_ERR_ARG, "null job array");
+  for (int i = 0; i < count; ++i) {
+    const gw_chol_job& j = jobs[i];
+    GW_REQUIRE(j.n >= 1 && j.n <= kCholMaxN, GW_ERR_UNSUPPORTED, "job %d: n = %d outside [1, %d]", i, j.n,
+               kCholMaxN);
+    GW_REQUIRE(j.lda >= j.n && j.ldl >= j.n, GW_ERR_ARG, "job %d: lda %d or ldl %d < n", i, j.lda, j.ldl);
+    GW_REQUIRE(j.a && j.linv && j.info, GW_ERR_ARG, "job %d: null pointer", i);
+    GW_REQUIRE(((uintptr_t)j.a | (uintptr_t)j.linv) % 16 == 0 && (uintptr_t)j.info % 4 == 0, GW_ERR_ARG,
+               "job %d: misaligned pointer", i);
+  }
+  hipStream_t st = (hipStream_t)stream;
+  for (int b = 0; b < count; b += kCholPerLaunch) {
+    CholArgs args{};
+    args.count = std::min(kCholPerLaunch, count - b);
+    for (int i = 0; i < args.count; ++i) args.j[i] = jobs[b + i];
+    hipLaunchKernelGGL(k_chol_inv, dim3(args.count), dim3(kCholThreads), 0, st, args);
     GW_CHECK_HIP(hipGetLastError());
   }
   return GW_OK;

@@ -103,19 +103,28 @@ def _gram(v: torch.Tensor) -> torch.Tensor:
     return 0.5 * (g + g.mH)
 
 
-def _cholqr2_dev(v: torch.Tensor):
+def _cholinv_torch(g: torch.Tensor):
+    """(L^-1, info) of G = L L^H with torch (rocSOLVER potrf, rocBLAS trsm)."""
+    lo, info = torch.linalg.cholesky_ex(g)
+    eye = torch.eye(g.shape[-1], dtype=torch.complex128, device=g.device)
+    return torch.linalg.solve_triangular(lo, eye.expand_as(lo), upper=False), info
+
+
+def _cholqr2_steps(v: torch.Tensor):
     """Orthonormal rows spanning v (nk, n, npw) by two rounds of fp64 Gram ->
     Cholesky -> triangular inverse -> combination (davidson.py:195-222 in row form:
-    G = v v^H = L L^H, q = L^-1 v). Returns (q, breakdown flags on the device)."""
-    n = v.shape[1]
-    eye = torch.eye(n, dtype=torch.complex128, device=v.device)
+    G = v v^H = L L^H, q = L^-1 v). A step generator: each round's L^-1 is a
+    ("cholinv", G) request. Returns (q, breakdown flags on the device)."""
     infos = []
     for _ in range(2):
-        lo, info = torch.linalg.cholesky_ex(_gram(v))
+        linv, info = yield ("cholinv", _gram(v))
         infos.append(info.max())
-        linv = torch.linalg.solve_triangular(lo, eye.expand_as(lo), upper=False)
         v = LA.combine(linv.transpose(-1, -2).to(v.dtype).contiguous(), v)
     return v, torch.stack(infos)
+
+
+def _cholqr2_dev(v: torch.Tensor):
+    return drive(_cholqr2_steps(v), None)
 
 
 def _project(v: torch.Tensor, against: torch.Tensor | None) -> torch.Tensor:
@@ -137,14 +146,14 @@ def _orthonormalize_steps(v: torch.Tensor, mask: torch.Tensor, against: torch.Te
     rn = LA.row_norms(v)
     degenerate = rn < 1e-8
     # CholQR2 runs before the degenerate-row test is read, so both tests cost one host read
-    q, infos = _cholqr2_dev(v)
+    q, infos = yield from _cholqr2_steps(v)
     flags = yield ("host", torch.cat([degenerate.any().reshape(1).to(infos.dtype), infos]))
     if bool(flags[0]):
         gen = torch.Generator(device="cpu").manual_seed(v.shape[1] + 7919)
         noise = torch.randn(*v.shape, 2, generator=gen, dtype=torch.float64)
         jit = torch.view_as_complex(noise).to(v.device).to(v.dtype)
         v = _project(((v + degenerate[..., None] * jit) * mask[:, None, :]).contiguous(), against)
-        q, infos = _cholqr2_dev(v)
+        q, infos = yield from _cholqr2_steps(v)
         flags = yield ("host", torch.cat([flags[:1].to(infos.device, infos.dtype), infos]))
     if bool((flags[1:] != 0).any()):
         qq, _ = torch.linalg.qr(v.transpose(-1, -2), mode="reduced")
@@ -225,8 +234,9 @@ def drive(steps, h_apply):
     """Run a step generator to completion, serving its ("apply", block) requests
     with ``h_apply``, its ("eigh", matrix) requests with ``_eigh``, its
     ("eigh_low", (matrix, nev)) requests (only the lowest nev pairs are used)
-    with the first nev pairs of ``_eigh`` and its ("host", tensor) requests
-    with a copy to the host."""
+    with the first nev pairs of ``_eigh``, its ("cholinv", G) requests with
+    ``_cholinv_torch`` and its ("host", tensor) requests with a copy to the
+    host."""
     try:
         req = next(steps)
         while True:
@@ -238,6 +248,8 @@ def drive(steps, h_apply):
             elif kind == "eigh_low":
                 w, u = _eigh(arg[0])
                 resp = (w[..., : arg[1]], u[..., : arg[1]])
+            elif kind == "cholinv":
+                resp = _cholinv_torch(arg)
             else:
                 resp = arg.cpu()
             req = steps.send(resp)
@@ -327,10 +339,9 @@ def davidson_steps(x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor, tol: f
         tm("restart")
         if basis.n + n_add > max_dim:
             # restart on the Ritz block with its drift repaired: x = L q, hx -> L^-1 hx
-            lo, info = torch.linalg.cholesky_ex(_gram(x))
-            if int(info.max()) == 0:
-                eye = torch.eye(nb, dtype=torch.complex128, device=device).expand_as(lo)
-                linv = torch.linalg.solve_triangular(lo, eye, upper=False)
+            linv, info = yield ("cholinv", _gram(x))
+            bad = yield ("host", info.max().reshape(1))
+            if int(bad[0]) == 0:
                 ut = linv.transpose(-1, -2).to(x.dtype).contiguous()
                 x_orth, hx_orth = LA.combine(ut, x), LA.combine(ut, hx)
             else:

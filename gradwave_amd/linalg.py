@@ -215,3 +215,43 @@ def eigh_lowest(mats: list, nevs: list) -> list:
         off += need
     L.check(lib.gw_heev_batched(len(jobs), arr, L.stream_ptr(dev)), "gw_heev_batched")
     return out
+
+
+def chol_max_n() -> int:
+    return int(L.lib().gw_chol_max_n())
+
+
+def cholesky_inverse(mats: list) -> list:
+    """L^-1 of the Cholesky factor G = L L^H of Hermitian positive definite
+    complex128 matrices, all in one launch (``gw_chol_inv_batched``): the CholQR
+    step of the orthonormalisation (solvers/davidson.py:195-222). ``mats[i]`` is
+    (nk, n_i, n_i) on the GPU (lower triangle read); returns [(linv (nk, n_i,
+    n_i), info (nk,) int32)], info > 0 where the factorisation broke down (as
+    torch.linalg.cholesky_ex reports). Sizes above ``chol_max_n()`` raise."""
+    if not mats:
+        return []
+    lib = L.lib()
+    dev = mats[0].device
+    out, jobs = [], []
+    for g in mats:
+        L.require_gpu(g, "g")
+        if g.dtype != torch.complex128 or g.dim() != 3 or g.shape[1] != g.shape[2]:
+            raise TypeError(f"expected (nk, n, n) complex128, got {tuple(g.shape)} {g.dtype}")
+        if g.stride(-1) != 1:
+            g = g.contiguous()
+        nk, n = int(g.shape[0]), int(g.shape[1])
+        if n > chol_max_n():
+            raise L.GWError(f"cholesky_inverse: n={n} above {chol_max_n()}")
+        linv = torch.empty(nk, n, n, dtype=torch.complex128, device=dev)
+        info = torch.empty(nk, dtype=torch.int32, device=dev)
+        out.append((linv, info))
+        for k in range(nk):
+            jobs.append((g, k, linv, info, n))
+    arr = (L.CholJob * len(jobs))()
+    for j, (g, k, linv, info, n) in enumerate(jobs):
+        arr[j].a = g.data_ptr() + k * g.stride(0) * 16
+        arr[j].linv = linv.data_ptr() + k * n * n * 16
+        arr[j].info = info.data_ptr() + k * 4
+        arr[j].n, arr[j].lda, arr[j].ldl = n, int(g.stride(1)), n
+    L.check(lib.gw_chol_inv_batched(len(jobs), arr, L.stream_ptr(dev)), "gw_chol_inv_batched")
+    return out

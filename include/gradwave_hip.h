@@ -198,6 +198,22 @@ typedef struct gw_heev_job {
 size_t gw_heev_workspace_bytes(int n, int nev);
 int gw_heev_max_n(void);
 int gw_heev_batched(int count, const gw_heev_job* jobs, void* stream);
+/* Batched Cholesky factor inverse, the CholQR step of the orthonormalisation
+ * (davidson.py:195-222: G = V V^H = L L^H, Q = L^-1 V): for each Hermitian
+ * positive definite G (n x n complex128 row-major, leading dimension lda, lower
+ * triangle read), linv = L^-1 (n x n row-major, ldl, zeros above the
+ * diagonal) and info = 0, or the 1-based index of the first non-positive
+ * pivot (linv is then not usable), as torch.linalg.cholesky_ex reports it.
+ * 1 <= n <= gw_chol_max_n(); one workgroup per matrix, 32 per launch; the
+ * job array is copied into the launch arguments. */
+typedef struct gw_chol_job {
+  const void* a;
+  void* linv;
+  void* info; /* int32 */
+  int n, lda, ldl;
+} gw_chol_job;
+int gw_chol_max_n(void);
+int gw_chol_inv_batched(int count, const gw_chol_job* jobs, void* stream);
 
 /* ---- Sternheimer conjugate-gradient column kernels ----
  * Replace the per-column updates of cg_sternheimer / _cg_percolumn

@@ -118,3 +118,45 @@ def test_size_limits_raise():
                                     device=dev())], [1])
     with pytest.raises(Exception):
         LA.eigh_lowest([torch.zeros(1, 4, 4, dtype=torch.complex128, device=dev())], [5])
+
+
+def _hpd(n, seed, cond_scale=1.0):
+    g = torch.Generator().manual_seed(seed)
+    v = torch.randn(n, 3 * n + 5, dtype=torch.complex128, generator=g)
+    a = cond_scale * (v @ v.mH)
+    return 0.5 * (a + a.mH)
+
+
+def test_cholesky_inverse_ragged_batch_matches_torch():
+    """gw_chol_inv_batched against torch.linalg.cholesky_ex + a triangular
+    solve on the host, ragged sizes and a batched k axis in one launch."""
+    sizes = [(1, 1), (1, 2), (2, 8), (1, 17), (3, 32), (1, 64)]
+    mats = [torch.stack([_hpd(n, seed=10 * n + k, cond_scale=10.0 ** (k - 1)) for k in range(nk)])
+            for nk, n in sizes]
+    say(f"cholesky inverse {sizes}")
+    out = LA.cholesky_inverse([m.to(dev()) for m in mats])
+    for (nk, n), m, (linv, info) in zip(sizes, mats, out):
+        assert int(info.abs().max()) == 0
+        lo = torch.linalg.cholesky(m)
+        want = torch.linalg.solve_triangular(lo, torch.eye(n, dtype=torch.complex128).expand_as(lo), upper=False)
+        got = linv.cpu()
+        assert float((got - want).abs().max()) <= 1e-11 * float(want.abs().max())
+        assert float(torch.triu(got, diagonal=1).abs().max()) == 0.0
+        # L^-1 G L^-H = I
+        eye = got @ m @ got.mH
+        assert float((eye - torch.eye(n, dtype=torch.complex128)).abs().max()) < 1e-10
+
+
+def test_cholesky_inverse_reports_breakdown():
+    n = 12
+    ok = _hpd(n, seed=3)
+    bad = ok.clone()
+    bad[5, 5] = -1.0  # the leading 6 x 6 minor is no longer positive definite
+    say("cholesky inverse breakdown")
+    (linv, info), = LA.cholesky_inverse([torch.stack([ok, bad]).to(dev())])
+    info = info.cpu()
+    _, want = torch.linalg.cholesky_ex(torch.stack([ok, bad]))
+    assert int(info[0]) == 0 and int(info[1]) > 0
+    assert int(info[1]) == int(want[1])
+    with pytest.raises(Exception):
+        LA.cholesky_inverse([torch.eye(LA.chol_max_n() + 1, dtype=torch.complex128, device=dev())[None]])

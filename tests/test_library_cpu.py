@@ -80,6 +80,11 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert lib.gw_heev_workspace_bytes(big, 1) == 0 and lib.gw_heev_workspace_bytes(4, 5) == 0
     assert lib.gw_heev_workspace_bytes(4, 2) % 256 == 0 and lib.gw_heev_workspace_bytes(4, 2) > 0
     assert lib.gw_heev_batched(0, None, None) == 0
+    assert "n = 65 outside [1, 64]" in err(lib.gw_chol_inv_batched(1, C.byref(L.CholJob(None, None, None, 65, 65, 65)),
+                                                                   None))
+    assert "null pointer" in err(lib.gw_chol_inv_batched(1, C.byref(L.CholJob(None, None, None, 4, 4, 4)), None))
+    assert "ldl 3 < n" in err(lib.gw_chol_inv_batched(1, C.byref(L.CholJob(None, None, None, 4, 4, 3)), None))
+    assert lib.gw_chol_inv_batched(0, None, None) == 0 and lib.gw_chol_max_n() == 64
     # a box length that no two codelets <= 16 can split (grids.py sizes are 7-smooth, 11 is not)
     flat = (C.c_int64 * 1)(0)
     npw = (C.c_int64 * 1)(1)
