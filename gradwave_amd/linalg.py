@@ -14,7 +14,9 @@ _WS: dict[tuple, torch.Tensor] = {}
 
 
 def _scratch(device: torch.device, nbytes: int) -> torch.Tensor:
-    key = (device.type, device.index)
+    """Split-K workspace of the contractions, one per (device, stream): work on
+    different streams never shares it."""
+    key = (device.type, device.index, torch.cuda.current_stream(device).cuda_stream)
     ws = _WS.get(key)
     if ws is None or ws.numel() < nbytes:
         ws = torch.empty(max(nbytes, 1 << 20), dtype=torch.uint8, device=device)
