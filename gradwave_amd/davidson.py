@@ -36,6 +36,23 @@ _FP32_CERT_STALL = 0.7  # davidson.py:259
 # offloads m <= 256 on CUDA, davidson.py:58-75); measured on MI355X, rocSOLVER
 # wins from m ~ 160 up, host LAPACK below.
 _SUBSPACE_CPU_MAX = int(os.environ.get("GW_EIGH_CPU_MAX", "64"))
+# Requests for the lowest nb pairs of GPU matrices up to this size go to the
+# batched subspace eigensolver (gw_heev_batched: every k of the batch in one
+# launch); GW_SUBSPACE_EIGH=library keeps the policy above for them too.
+_HEEV_MAX = int(os.environ.get("GW_HEEV_MAX", "256"))
+
+
+def _eigh_low(s: torch.Tensor, nev: int):
+    """Lowest nev eigenpairs of the (nk, m, m) subspace matrix: the batched
+    kernel for GPU matrices with m <= GW_HEEV_MAX, else the first nev pairs of
+    ``_eigh``."""
+    mode = os.environ.get("GW_SUBSPACE_EIGH", "kernel").strip().lower()
+    if mode not in ("kernel", "library"):
+        raise ValueError(f"GW_SUBSPACE_EIGH must be kernel|library, got {mode!r}")
+    if mode == "kernel" and s.is_cuda and s.shape[-1] <= min(_HEEV_MAX, LA.heev_max_n()):
+        return LA.eigh_lowest([s.to(torch.complex128)], [nev])[0]
+    w, u = _eigh(s)
+    return w[..., :nev], u[..., :nev]
 
 
 class PhaseTimer:
@@ -234,7 +251,7 @@ def drive(steps, h_apply):
     """Run a step generator to completion, serving its ("apply", block) requests
     with ``h_apply``, its ("eigh", matrix) requests with ``_eigh``, its
     ("eigh_low", (matrix, nev)) requests (only the lowest nev pairs are used)
-    with the first nev pairs of ``_eigh``, its ("cholinv", G) requests with
+    with ``_eigh_low``, its ("cholinv", G) requests with
     ``_cholinv_torch`` and its ("host", tensor) requests with a copy to the
     host."""
     try:
@@ -246,8 +263,7 @@ def drive(steps, h_apply):
             elif kind == "eigh":
                 resp = _eigh(arg)
             elif kind == "eigh_low":
-                w, u = _eigh(arg[0])
-                resp = (w[..., : arg[1]], u[..., : arg[1]])
+                resp = _eigh_low(arg[0], arg[1])
             elif kind == "cholinv":
                 resp = _cholinv_torch(arg)
             else:
