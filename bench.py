@@ -54,9 +54,11 @@ def parse():
     ap.add_argument("--no-forces", action="store_true")
     ap.add_argument("--no-gamma", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile and split-K width")
-    ap.add_argument("--only", choices=["apply", "gamma", "forces", "copy"], default=None,
+    ap.add_argument("--only", choices=["apply", "gamma", "forces", "copy", "davidson"], default=None,
                     help="profiling runs: time just the block apply (FP64 then FP32), the Gamma-layout "
-                         "apply, the adjoint forces, or a 1 GiB device copy (traffic-counter calibration)")
+                         "apply, the adjoint forces, or a 1 GiB device copy (traffic-counter calibration); "
+                         "with --workload ragged32, 'davidson' runs only the lockstep FP64 solve and two "
+                         "graph-captured draft + sweep solves")
     ap.add_argument("--profile-davidson", action="store_true")
     ap.add_argument("--tol-ha", type=float, default=1e-10)
     ap.add_argument("--draft-tol", type=float, default=1e-3,
@@ -378,6 +380,9 @@ def bench_ragged(args) -> None:
           "workgroups",
           file=sys.stderr, flush=True)
     x64 = [start_vectors(w, seed=i) for i, w in enumerate(wls)]
+    if args.only == "davidson":
+        ragged_davidson_only(args, batch, wls, x64)
+        return
     x32 = [x.to(torch.complex64) for x in x64]
 
     def timed(fn, steps, warmup):
@@ -515,6 +520,35 @@ def bench_ragged(args) -> None:
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(result), flush=True)
+
+
+def ragged_davidson_only(args, batch, wls, x64) -> None:
+    """Profiling run of the batching form's solves: the lockstep FP64 solve,
+    then the graph-captured draft + FP64 sweep twice on one solver (the second
+    only replays)."""
+    import torch
+
+    from gradwave_amd import batching as B
+    from gradwave_amd import graph_davidson as GD
+    from gradwave_amd.workloads import HARTREE_EV
+
+    ts_, ms_ = [w.bk.t for w in wls], [w.bk.mask for w in wls]
+    out = {"workload": "ragged32", "only": "davidson"}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = B.RaggedSolveStats()
+    B.davidson_ragged(batch, x64, ts_, ms_, tol=args.tol_ha * HARTREE_EV, max_iter=args.davidson_max_iter, stats=st)
+    torch.cuda.synchronize()
+    out["lockstep_fp64_s"], out["lockstep_rounds"] = time.perf_counter() - t0, st.rounds
+    solver = GD.RaggedGraphDavidson(batch, ts_, ms_, [int(x.shape[1]) for x in x64])
+    for key in ("graph_first_s", "graph_replayed_s"):
+        t0 = time.perf_counter()
+        GD.davidson_ragged_graph_mixed(batch, x64, ts_, ms_, draft_tol=args.draft_tol, sweeps=args.sweeps,
+                                       max_iter=args.davidson_max_iter, solver=solver)
+        torch.cuda.synchronize()
+        out[key] = time.perf_counter() - t0
+    print(f"[bench] ragged davidson only: {out}", file=sys.stderr, flush=True)
+    print(json.dumps(out), flush=True)
 
 
 def copy_bandwidth(dev, nbytes: int = 2 << 30, reps: int = 10) -> float:
