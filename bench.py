@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--davidson-max-iter", type=int, default=300)
     ap.add_argument("--no-forces", action="store_true")
     ap.add_argument("--no-gamma", action="store_true")
+    ap.add_argument("--no-ragged", action="store_true", help="skip the ragged32 batching-form section")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default GEMM tile and split-K width")
     ap.add_argument("--only", choices=["apply", "gamma", "forces", "copy", "davidson"], default=None,
                     help="profiling runs: time just the block apply (FP64 then FP32), the Gamma-layout "
@@ -279,8 +280,71 @@ def main() -> None:
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args, wl, result.get("davidson", {}).get("fp64_iters"))
+    if not args.no_ragged and not args.no_davidson:
+        result["batching_form"] = ragged_section(args)
 
     print(json.dumps(result), flush=True)
+
+
+def ragged_section(args) -> dict:
+    """The batching form next to the headline line: 32 independent cells
+    (Al1/Si2/Si8/Si16, 25-40 Ry, 8-64 bands) solved in lockstep FP64 (every
+    round's eigensolves and Cholesky inverses in one launch each), one cell at
+    a time (davidson_batched each), and as graph-captured draft rounds + one
+    FP64 sweep (captured once, then replayed). ``bench.py --workload ragged32``
+    has the full set (apply rates, library eigensolves, mixed forms)."""
+    import torch
+
+    from gradwave_amd import autotune
+    from gradwave_amd import batching as B
+    from gradwave_amd import graph_davidson as GD
+    from gradwave_amd.davidson import davidson_batched
+    from gradwave_amd.hamiltonian import BatchedHamiltonian
+    from gradwave_amd.workloads import HARTREE_EV, start_vectors
+
+    dev = torch.device("cuda", 0)
+    t0 = time.perf_counter()
+    wls = B.ragged_instances(device=dev, n=32, seed=0)
+    autotune.tune_radix(wls)
+    hams = [BatchedHamiltonian(w.bk, w.shape, w.v_r, w.p) for w in wls]
+    batch = B.RaggedBatch(hams)
+    x64 = [start_vectors(w, seed=i) for i, w in enumerate(wls)]
+    ts_, ms_ = [w.bk.t for w in wls], [w.bk.mask for w in wls]
+    tol = args.tol_ha * HARTREE_EV
+    t_setup = time.perf_counter() - t0
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - a
+
+    lock, t_lock = timed(lambda: B.davidson_ragged(batch, x64, ts_, ms_, tol=tol, max_iter=args.davidson_max_iter))
+    seq, t_seq = timed(lambda: [davidson_batched(h.apply, x, w.bk.t, w.bk.mask, tol=tol,
+                                                 max_iter=args.davidson_max_iter)
+                                for h, x, w in zip(hams, x64, wls)])
+    solver = GD.RaggedGraphDavidson(batch, ts_, ms_, [int(x.shape[1]) for x in x64])
+
+    def graph_solve():
+        return GD.davidson_ragged_graph_mixed(batch, x64, ts_, ms_, draft_tol=args.draft_tol, sweeps=args.sweeps,
+                                              max_iter=args.davidson_max_iter, solver=solver)
+
+    _, t_graph_first = timed(graph_solve)
+    gr, t_graph = timed(graph_solve)
+    de_lock = max(float((a.eigenvalues - b.eigenvalues).abs().max()) for a, b in zip(lock, seq)) / HARTREE_EV
+    de_graph = max(float((a.eigenvalues - b.eigenvalues).abs().max()) for a, b in zip(gr, seq)) / HARTREE_EV
+    out = {"workload": "ragged32: Al1/Si2/Si8/Si16 x 8, 25-40 Ry, seeded k, 8-64 bands", "setup_s": t_setup,
+           "lockstep_fp64_s": t_lock, "one_by_one_fp64_s": t_seq, "lockstep_speedup_vs_one_by_one": t_seq / t_lock,
+           "max_eig_diff_lockstep_vs_one_by_one_Ha": de_lock,
+           "graph_draft_plus_fp64_sweep_s": t_graph, "graph_first_solve_with_capture_s": t_graph_first,
+           "graph_speedup_vs_one_by_one_fp64": t_seq / t_graph,
+           "max_eig_diff_graph_vs_one_by_one_Ha": de_graph, "graph_draft_tol_eV": args.draft_tol,
+           "tol_Ha": args.tol_ha}
+    print(f"[bench] batching form: lockstep FP64 {t_lock:.2f}s, one-by-one {t_seq:.2f}s "
+          f"({t_seq / t_lock:.1f}x), graph draft + sweep {t_graph:.2f}s replayed ({t_graph_first:.2f}s first), "
+          f"max|de| {de_lock:.1e} / {de_graph:.1e} Ha", file=sys.stderr, flush=True)
+    return out
 
 
 def profile_only(args, wl, x64) -> None:
