@@ -15,7 +15,8 @@
 //      tau = (beta - a) / beta, v = x / (a - beta)); the trailing block is
 //      updated as A -= v w^H + w v^H with w = y - (tau/2)(y^H v) v, y = tau A v;
 //   3. finds the nev lowest eigenvalues of T by multisection on Sturm counts
-//      (three interior points per step, one per thread and eigenvalue);
+//      (P = 2..16 lanes of one wave per eigenvalue, one interior point each,
+//      the interval shrinking by P + 1 per step);
 //   4. computes T's eigenvectors by inverse iteration (LU of T - lambda I with
 //      partial pivoting, tiny pivots raised to eps |T|, three steps from a
 //      hashed start vector), orthogonalising each vector against the earlier
@@ -24,6 +25,8 @@
 //   5. back-transforms u = H_0 H_1 ... H_{n-2} z into the caller's output.
 // Every loop has a fixed bound (n, nev, 64 multisection steps, 3 inverse
 // iteration steps), so the kernel ends for any input, NaN included.
+// The file also holds the batched Cholesky factor inverse of the CholQR step
+// (k_chol_inv, below).
 #include <cfloat>
 
 #include "gw_internal.h"
