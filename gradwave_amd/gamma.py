@@ -247,16 +247,17 @@ def _teter(r: torch.Tensor, t_g: torch.Tensor, t_band: torch.Tensor) -> torch.Te
     return r * (num / (num + 16.0 * x2 * x2)).to(r.dtype)
 
 
-def _project_out(v: torch.Tensor, against: torch.Tensor) -> torch.Tensor:
-    """v - (v a^T) a, twice (solvers/davidson.py:321-323), on the MFMA kernels."""
+def _project_out(v: torch.Tensor, against: torch.Tensor, passes: int = 2) -> torch.Tensor:
+    """v - (v a^T) a, ``passes`` times (twice in the reference,
+    solvers/davidson.py:321-323), on the MFMA kernels."""
     v = v.clone()
-    for _ in range(2):
+    for _ in range(passes):
         c = LA.rinner(against, v)  # (m, k): c[j][a] = against_j . v_a
         LA.rcombine(c, against, out=v, alpha=-1.0, beta=1.0)
     return v
 
 
-def _orthonormalize(v: torch.Tensor, against: torch.Tensor | None = None) -> torch.Tensor:
+def _orthonormalize(v: torch.Tensor, against: torch.Tensor | None = None, passes: int = 2) -> torch.Tensor:
     """solvers/davidson.py:310-327 in real arithmetic: project out ``against``,
     then orthonormalize the rows, dropping rows whose QR diagonal |R_ii| is at
     most 1e-10.
@@ -267,7 +268,7 @@ def _orthonormalize(v: torch.Tensor, against: torch.Tensor | None = None) -> tor
     or the Gram is too ill-conditioned for CholQR, Householder QR runs instead
     and applies the reference's test directly."""
     if against is not None and against.shape[0]:
-        v = _project_out(v, against)
+        v = _project_out(v, against, passes)
     k = v.shape[0]
     if k == 0:
         return v
@@ -313,14 +314,17 @@ class _Timer:
 
 @torch.no_grad()
 def davidson_real(h_apply, x0: torch.Tensor, t_g: torch.Tensor, tol: float = 1e-9, max_iter: int = 40,
-                  max_dim_factor: int = 4, profile_out: dict | None = None, nconv: int | None = None):
+                  max_dim_factor: int = 4, profile_out: dict | None = None, nconv: int | None = None,
+                  proj_passes: int = 2):
     """The reference's single-k block Davidson (solvers/davidson.py:329-395) for a
     real symmetric operator on the GPU. The basis lives in preallocated buffers
     of x0's precision, every contraction over the long axis is a real MFMA
     kernel, the subspace matrix (FP64) grows by its new rows and columns
     instead of being rebuilt, and its eigensolve is FP64. ``nconv`` (default:
     every band) stops once the first ``nconv`` bands are converged; the others
-    are buffer bands, still expanded while unconverged."""
+    are buffer bands, still expanded while unconverged. ``proj_passes``
+    projections of the expansion block against the basis (two in the
+    reference; the mixed solver's FP32 draft uses one)."""
     nb, n = x0.shape
     ng = nb if nconv is None else max(1, min(int(nconv), nb))
     dev = x0.device
@@ -373,7 +377,7 @@ def davidson_real(h_apply, x0: torch.Tensor, t_g: torch.Tensor, tol: float = 1e-
         idx = bad.nonzero().squeeze(1)
         t = _teter(r[idx], t_g.to(f64), tb[idx])
         tm("orthonormalize")
-        t = _orthonormalize(t, against=V[:m])
+        t = _orthonormalize(t, against=V[:m], passes=proj_passes)
         k = t.shape[0]
         if k == 0:
             tm(None)
@@ -461,7 +465,8 @@ def davidson_gamma_mixed(ham: GammaHamiltonian, x0: torch.Tensor, tol: float = 1
     x32 = embed_real(gb, x0.to(torch.complex128)).to(torch.float32)
     nconv = nb if buffer else None  # buffer bands are expanded but do not gate
     _, xd, itd, _ = davidson_real(ham.apply_real, x32, t_phi, tol=draft_tol, max_iter=max_iter,
-                                  max_dim_factor=max_dim_factor, profile_out=profile_out, nconv=nconv)
+                                  max_dim_factor=max_dim_factor, profile_out=profile_out, nconv=nconv,
+                                  proj_passes=1)  # the FP64 polish or sweeps re-orthonormalise
     if sweeps is not None:
         if sweeps < 0:
             raise ValueError(f"sweeps must be >= 0, got {sweeps}")

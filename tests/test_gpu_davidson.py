@@ -182,7 +182,9 @@ def test_si8_mixed_refinement_sweeps():
     h = BatchedHamiltonian(gpu, s.shape, s.v_r.to(dev()), p.to(dev()))
     x0 = rand_block(3, 24, cpu.npw_max, cpu.mask, seed=1).to(dev())
     r64 = davidson_batched(h.apply, x0, gpu.t, gpu.mask, tol=TOL, max_iter=200)
-    draft = davidson_batched(h.apply, x0.to(LOW), gpu.t.to(torch.float32), gpu.mask, tol=1e-3, max_iter=200)
+    # the draft as davidson_mixed runs it: one projection per round
+    draft = davidson_batched(h.apply, x0.to(LOW), gpu.t.to(torch.float32), gpu.mask, tol=1e-3, max_iter=200,
+                             proj_passes=1)
     xd = draft.eigenvectors.to(torch.complex128)
     res = [drive(_refine_steps(xd, gpu.t, gpu.mask, n), h.apply) for n in (0, 1, 2)]
     eps = 1e-12 * HARTREE_EV
