@@ -244,7 +244,8 @@ def _eigh_lowest_batched(reqs: list, mode: str = "kernel") -> list:
     if mode not in ("kernel", "library"):
         raise ValueError(f"subspace eigensolve mode must be 'kernel' or 'library', got {mode!r}")
     small = [j for j, (s, _) in enumerate(reqs) if mode == "kernel" and int(s.shape[-1]) <= LA.heev_max_n()]
-    rest = [j for j in range(len(reqs)) if j not in set(small)]
+    taken = set(small)
+    rest = [j for j in range(len(reqs)) if j not in taken]
     if small:
         for j, wu in zip(small, LA.eigh_lowest([reqs[j][0].to(torch.complex128) for j in small],
                                                [reqs[j][1] for j in small])):
@@ -261,6 +262,8 @@ def _cholinv_batched(gs: list, mode: str = "kernel") -> list:
     matrix up to ``gw_chol_max_n`` in ONE launch of the batched Cholesky
     inverse; larger ones, and ``mode="library"``, take torch (rocSOLVER potrf,
     rocBLAS trsm) one by one as ``drive`` does."""
+    if mode not in ("kernel", "library"):
+        raise ValueError(f"factorisation mode must be 'kernel' or 'library', got {mode!r}")
     out: list = [None] * len(gs)
     small = [j for j, g in enumerate(gs) if mode == "kernel" and int(g.shape[-1]) <= LA.chol_max_n()]
     if small:
