@@ -66,12 +66,12 @@ def _mark(st: GraphSolveStats, key: str, t0: float) -> float:
 class _Cell:
     """Static device state of one cell: basis, images, subspace matrix, Ritz block."""
 
-    def __init__(self, x0: torch.Tensor, t: torch.Tensor, mask: torch.Tensor) -> None:
-        nk, nb, m = x0.shape
+    def __init__(self, nb: int, t: torch.Tensor, mask: torch.Tensor) -> None:
+        nk, m = int(mask.shape[0]), int(mask.shape[1])
         if int(mask.sum(dim=1).min()) < 4 * nb:
             raise ValueError(f"graph rounds need >= 4 nb = {4 * nb} plane waves per k, got "
                              f"{int(mask.sum(dim=1).min())}")
-        dev = x0.device
+        dev = mask.device
         self.nk, self.nb, self.m = nk, nb, m
         self.t = t.to(torch.float64).contiguous()
         self.mask = mask
@@ -202,8 +202,7 @@ class RaggedGraphDavidson:
     def __init__(self, batch: RaggedBatch, ts: list, masks: list, nbs: list, lanes: int = 4) -> None:
         self.batch = batch
         dev = masks[0].device
-        self.cells = [_Cell(torch.zeros(int(m.shape[0]), nb, int(m.shape[1]), dtype=LOW, device=dev), t, m)
-                      for t, m, nb in zip(ts, masks, nbs)]
+        self.cells = [_Cell(int(nb), t, m) for t, m, nb in zip(ts, masks, nbs)]
         self.status = torch.zeros(len(self.cells), 2, dtype=torch.float64, device=dev)
         self.binding = batch.bind([c.d for c in self.cells], [c.hd for c in self.cells])
         self.lanes = _Lanes(lanes, dev)
