@@ -71,6 +71,9 @@ struct Plan {
   int* slot_of;     // (n1) active-plane slot of each x index, -1 if inactive
   int xlo, xhi;     // active planes = [0, xlo) U [xhi, n1) when xlo >= 0
   bool have_v;
+  // 0, or the first box length with no codelet split: the plan then serves the
+  // kinetic and nonlocal terms and the sphere<->box maps, not the box FFTs
+  int nosplit_len;
   // nonlocal: projectors (nk, nproj, npw_max) both precisions, D as CSR by column
   int nproj;
   double2* p64;

@@ -138,6 +138,16 @@ extern "C" int gw_box_split(int n, int* f) {
   return GW_OK;
 }
 
+// Entry points that run the box FFTs refuse a plan whose box has a length with
+// no codelet split.
+static int require_box_fft(const Plan& P, const char* what) {
+  if (!P.nosplit_len) return GW_OK;
+  set_error("%s needs the box FFT kernels, and box length %d (box %d x %d x %d) has no split into two "
+            "codelets <= 16; this plan serves the kinetic and nonlocal terms and the sphere/box maps",
+            what, P.nosplit_len, P.n[0], P.n[1], P.n[2]);
+  return GW_ERR_UNSUPPORTED;
+}
+
 constexpr int kMaxLds = 160 * 1024;
 constexpr int kXTilePad = 0;  // extra elements per row of the staged x-pass tiles (1, 2, 4 measured slower)
 constexpr int kPlaneThreads = 512;
@@ -189,7 +199,8 @@ ApplyLayout apply_layout(const Plan* p, int nb, int precision, bool chunked) {
   size_t off = 0;
   L.box_off = off;
   const int chunk = chunked ? g_band_chunk[precision == GW_FP64 ? 0 : 1] : 0;
-  const size_t nbt_box = (size_t)p->nk * (chunk > 0 && chunk < nb ? chunk : nb);
+  // a plan without box FFTs (a length with no codelet split) needs no box
+  const size_t nbt_box = p->nosplit_len ? 0 : (size_t)p->nk * (chunk > 0 && chunk < nb ? chunk : nb);
   L.box_bytes = nbt_box * p->na * p->n[1] * p->n[2] * e;
   off = align256(off + L.box_bytes);
   L.stick_off = off;
@@ -1732,10 +1743,11 @@ extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int
   P.nk = nk; P.npw_max = npw_max;
   for (int ax = 0; ax < 3; ++ax) {
     if (!plan_split(P.n[ax], P.f[ax])) {
-      const int n = P.n[ax];
-      delete h;
-      set_error("box length %d has no split into two codelets <= 16 (three-factor splits are not enabled)", n);
-      return GW_ERR_UNSUPPORTED;
+      // No codelet split: the plan is still built for the kinetic and nonlocal
+      // terms and the sphere<->box maps; every box-FFT entry point refuses it.
+      P.f[ax][0] = P.n[ax];
+      P.f[ax][1] = P.f[ax][2] = 1;
+      if (!P.nosplit_len) P.nosplit_len = P.n[ax];
     }
   }
   const long N = (long)n1 * n2 * n3;
@@ -1940,7 +1952,7 @@ extern "C" int gw_plan_info(const gw_plan* h, int precision, int* info, int n) {
   const Plan& P = h->p;
   const int v[] = {P.na, P.nst_max, wave_path_ok(P, precision) ? 2 : (plane_fits_lds(&P, precision) ? 1 : 0), P.lds_max, P.nproj,
                    P.f[0][0], P.f[0][1], P.f[1][0], P.f[1][1], P.f[2][0], P.f[2][1],
-                   P.f[0][2], P.f[1][2], P.f[2][2]};
+                   P.f[0][2], P.f[1][2], P.f[2][2], P.nosplit_len};
   for (int i = 0; i < n && i < (int)(sizeof(v) / sizeof(v[0])); ++i) info[i] = v[i];
   return GW_OK;
 }
@@ -2133,6 +2145,7 @@ static int apply_t(gw_plan* h, int terms, const cpx<T>* c, long ldc, long sc, cp
   hipEvent_t ev;
   if (terms & GW_TERM_LOCAL) {
     GW_REQUIRE(P.have_v, GW_ERR_ARG, "local term requested but no potential set");
+    if (int rc = require_box_fft(P, "the local term")) return rc;
     const int chunk0 = g_band_chunk[prec == GW_FP64 ? 0 : 1];
     const int chunk = chunk0 > 0 && chunk0 < nb ? chunk0 : nb;
     for (int b0 = 0; b0 < nb; b0 += chunk) {  // band chunks (one pass sequence each)
@@ -2305,6 +2318,7 @@ static int to_real_t(gw_plan* h, const cpx<T>* c, cpx<T>* psi, int nb, unsigned 
 extern "C" int gw_wfc_to_real(gw_plan* h, int precision, const void* c, void* psi, int nb, void* ws,
                               size_t ws_bytes, void* stream) {
   GW_REQUIRE(h && c && psi && nb > 0, GW_ERR_ARG, "bad argument");
+  if (int rc = require_box_fft(h->p, "psi(r)")) return rc;
   GW_REQUIRE(plan_rmax(&h->p) < kRmaxF3, GW_ERR_UNSUPPORTED,
              "psi(r) is not available on boxes with a three-factor length (%d x %d x %d)", h->p.n[0], h->p.n[1],
              h->p.n[2]);
@@ -2382,6 +2396,7 @@ extern "C" int gw_hpsi_adjoint_v(gw_plan* h, int precision, const void* c, long 
                                  long ldg, long sg, int nb, double* grad_v, void* ws, size_t ws_bytes,
                                  void* stream) {
   GW_REQUIRE(h && c && g && grad_v && nb > 0, GW_ERR_ARG, "bad argument");
+  if (int rc = require_box_fft(h->p, "the V(r) adjoint")) return rc;
   const size_t need = gw_hpsi_adjoint_v_workspace_bytes(h, nb, precision);
   GW_REQUIRE(ws && ws_bytes >= need, GW_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, need);
   if (precision == GW_FP64)
@@ -2549,6 +2564,10 @@ extern "C" int gw_batch_create(gw_batch** out, gw_plan* const* plans, int n) {
     if (!plans[i]) {
       delete b;
       GW_REQUIRE(false, GW_ERR_ARG, "plan %d is null", i);
+    }
+    if (int rc = require_box_fft(plans[i]->p, "a ragged batch")) {
+      delete b;
+      return rc;
     }
     b->plans.push_back(plans[i]);
   }
@@ -2908,6 +2927,7 @@ static int density_t(gw_plan* h, const cpx<T>* c, long ldc, long sc, int nb, con
 extern "C" int gw_density(gw_plan* h, int precision, const void* c, long ldc, long sc, int nb, const double* weights,
                           double* rho, void* ws, size_t ws_bytes, void* stream) {
   GW_REQUIRE(h && c && weights && rho && nb > 0, GW_ERR_ARG, "bad argument");
+  if (int rc = require_box_fft(h->p, "the band density")) return rc;
   const size_t need = gw_density_workspace_bytes(h, nb, precision);
   GW_REQUIRE(ws && ws_bytes >= need, GW_ERR_WORKSPACE, "workspace %zu < %zu bytes", ws_bytes, need);
   if (precision == GW_FP64)

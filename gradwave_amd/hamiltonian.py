@@ -31,6 +31,9 @@ _TOEPLITZ_ON_GPU = os.environ.get("GW_TOEPLITZ_ON_GPU", "0") == "1"
 _TOEPLITZ_M_BUDGET_BYTES = int(os.environ.get("GW_TOEPLITZ_BUDGET", str(4 << 30)))
 _TOEP_TRIAL_MARGIN = 0.7
 _TOEP_TRIAL: dict[tuple, bool] = {}
+# Device bytes one band chunk of the library-FFT local term may hold (box,
+# psi(r) and the transformed box).
+_LIB_FFT_BUDGET_BYTES = int(os.environ.get("GW_LIB_FFT_BUDGET", str(2 << 30)))
 
 
 def reset_happly_tally() -> None:
@@ -58,6 +61,10 @@ class Plan:
                                          flat.data_ptr(), npw_h.data_ptr()), "gw_plan_create")
         self.shape = (n1, n2, n3)
         self.nk, self.npw_max = nk, npw_max
+        info = (C.c_int * 15)()
+        L.check(self._lib.gw_plan_info(self.handle, L.GW_FP64, info, 15), "gw_plan_info")
+        # 0, or a box length with no codelet split: no box FFT kernels on this plan
+        self.nosplit_len = int(info[14])
 
     def set_kinetic(self, t: torch.Tensor) -> None:
         t = t.detach().to(torch.float64).contiguous()
@@ -113,6 +120,15 @@ class BatchedHamiltonian:
     adds the DFT+U term (batch.py:272-274, 501-504): the plan carries the KB and
     Hubbard projectors stacked, with D block-diagonal, so both run in the same
     MFMA contractions. A complex hub_dij is refused.
+
+    Boxes with a length that has no split into two codelets <= 16 (125, 162,
+    ..., every length above 256; good_fft_size returns them, grids.py:32-41)
+    have no box FFT kernels here: their local term is the Toeplitz contraction
+    (M built by gw_toeplitz_build, applied by the MFMA inner product) when the
+    matrix fits the memory cap, else scatter -> rocFFT inverse -> x V ->
+    rocFFT forward -> gather (torch.fft on the GPU; the scatter and gather are
+    the plan's kernels). psi(r), the density and the V(r) adjoint refuse such
+    boxes with GWError.
     """
 
     def __init__(self, bk, shape, v_eff_r: torch.Tensor, p: torch.Tensor, hub_q=None,
@@ -148,9 +164,13 @@ class BatchedHamiltonian:
         nk, npw_max = self.plan.nk, self.plan.npw_max
         self._toep_mode = mode
         # the Toeplitz path reads the dense sphere map (ineligible with a dual grid, batch.py:288-296)
-        self._toep_eligible = (mode != "off" and smooth is None
-                               and nk * npw_max * npw_max * 16 <= _TOEPLITZ_M_BUDGET_BYTES)
+        toep_fits = smooth is None and nk * npw_max * npw_max * 16 <= _TOEPLITZ_M_BUDGET_BYTES
+        self._toep_eligible = mode != "off" and toep_fits
         self._toep_M: dict[torch.dtype, torch.Tensor] = {}
+        # local term of a box without FFT kernels: Toeplitz unless switched off or too large
+        self.local_path = "fft" if not self.plan.nosplit_len else ("toeplitz" if self._toep_eligible
+                                                                   else "library_fft")
+        self._v_lib: dict[torch.dtype, torch.Tensor] = {}
 
     def _workspace(self, nb: int, prec: int) -> torch.Tensor:
         need = int(L.lib().gw_hpsi_workspace_bytes(self.plan.handle, nb, prec))
@@ -174,6 +194,14 @@ class BatchedHamiltonian:
             out = torch.empty(c.shape, dtype=c.dtype, device=c.device)
         if nb == 0:
             return out
+        if (terms & L.GW_TERM_LOCAL) and self.plan.nosplit_len:
+            rest = terms & ~L.GW_TERM_LOCAL
+            if rest:
+                self.apply_terms(c, rest, out=out)
+            else:
+                out.zero_()
+            out += self._local_without_box_fft(c)
+            return out
         ws = self._workspace(nb, prec)
         L.check(L.lib().gw_hpsi_apply_strided(
             self.plan.handle, prec, terms, c.data_ptr(), c.stride(1), c.stride(0),
@@ -190,6 +218,43 @@ class BatchedHamiltonian:
             out += self.local_toeplitz(c)
             return out
         return self.apply_terms(c, L.GW_TERM_ALL)
+
+    def _local_without_box_fft(self, c: torch.Tensor) -> torch.Tensor:
+        if self.local_path == "toeplitz":
+            return self.local_toeplitz(c)
+        return self._local_library_fft(c)
+
+    def _local_library_fft(self, c: torch.Tensor) -> torch.Tensor:
+        """V psi on a box the FFT kernels cannot split (batch.py:405-428): the
+        plan's scatter, rocFFT ifftn, x V(r), rocFFT fftn, the plan's gather
+        (padding 0), in band chunks of at most GW_LIB_FFT_BUDGET bytes."""
+        nk, nb, m = (int(x) for x in c.shape)
+        prec = L.precision_of(c.dtype)
+        rdt = torch.float64 if c.dtype == torch.complex128 else torch.float32
+        v = self._v_lib.get(rdt)
+        if v is None:
+            v = self.v_eff_r.to(self.device, rdt).contiguous()
+            self._v_lib[rdt] = v
+        lib, st = L.lib(), L.stream_ptr(c.device)
+        res = torch.empty(nk, nb, m, dtype=c.dtype, device=c.device)
+        step = max(1, min(nb, _LIB_FFT_BUDGET_BYTES // (3 * nk * self.n * c.element_size())))
+        for b0 in range(0, nb, step):
+            nbs = min(step, nb - b0)
+            cb = c[:, b0:b0 + nbs].contiguous()
+            box = torch.empty(nk, nbs, *self.shape, dtype=c.dtype, device=c.device)
+            L.check(lib.gw_sphere_to_box(self.plan.handle, prec, cb.data_ptr(), box.data_ptr(), nbs, st),
+                    "gw_sphere_to_box")
+            psi = torch.fft.ifftn(box, dim=(-3, -2, -1))
+            del box
+            psi *= v
+            hb = torch.fft.fftn(psi, dim=(-3, -2, -1)).contiguous()
+            del psi
+            ob = res[:, b0:b0 + nbs] if nbs == nb else torch.empty(nk, nbs, m, dtype=c.dtype, device=c.device)
+            L.check(lib.gw_box_to_sphere(self.plan.handle, prec, hb.data_ptr(), ob.data_ptr(), nbs, st),
+                    "gw_box_to_sphere")
+            if nbs != nb:
+                res[:, b0:b0 + nbs] = ob
+        return res
 
     # -- Toeplitz local term (core/batch.py:333-464) --
 
@@ -286,11 +351,12 @@ class BatchedHamiltonian:
         return {n: (ms[i], calls[i]) for i, n in enumerate(L.STAGES)}
 
     def info(self, dtype: torch.dtype = torch.complex128) -> dict[str, int]:
-        v = (C.c_int * 14)()
-        L.check(L.lib().gw_plan_info(self.plan.handle, L.precision_of(dtype), v, 14), "gw_plan_info")
+        v = (C.c_int * 15)()
+        L.check(L.lib().gw_plan_info(self.plan.handle, L.precision_of(dtype), v, 15), "gw_plan_info")
         split = [tuple(f for f in (v[5 + 2 * a], v[6 + 2 * a], v[11 + a]) if f != 1) or (1,) for a in range(3)]
         return {"active_planes": v[0], "max_sticks": v[1], "fused_planes": v[2], "lds_bytes": v[3],
-                "nproj": v[4], "split_x": split[0], "split_y": split[1], "split_z": split[2]}
+                "nproj": v[4], "split_x": split[0], "split_y": split[1], "split_z": split[2],
+                "nosplit_len": v[14], "local_path": self.local_path}
 
     def wfc_to_real(self, c: torch.Tensor) -> torch.Tensor:
         """psi(r) = N ifftn(scatter c) on the natural box (nk, nb, n1, n2, n3)

@@ -46,8 +46,13 @@ int gw_version(void);
 
 /* Geometry from the padded BatchedK maps (core/batch.py:126-205): flat_idx
  * (nk, npw_max) int64 host array, 0 in padding; npw (nk) int64 host array.
- * Box lengths must factor as a*b with a, b in {1..10,12,14,15,16} (the
- * three-factor splits a*b*c are built but not enabled). */
+ * Any box is accepted. The box FFT kernels need every length to factor as a*b
+ * with a, b in {1..10,12,14,15,16} (the three-factor splits a*b*c are built but
+ * not enabled); on a box with another length (good_fft_size, grids.py:32-41,
+ * also returns 125, 162, ..., every 7-smooth length) the plan serves the
+ * kinetic and nonlocal terms and gw_sphere_to_box / gw_box_to_sphere, and the
+ * local term, psi(r), the V(r) adjoint, the density and ragged batches return
+ * GW_ERR_UNSUPPORTED naming the length (info[14] of gw_plan_info holds it). */
 int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int npw_max,
                    const int64_t* flat_idx, const int64_t* npw);
 /* The split a plan takes for box length n: f[0..2] with f[0]*f[1]*f[2] = n and
@@ -67,10 +72,11 @@ int gw_plan_set_flags(gw_plan* plan, int flags);
  * 3 projection <p|c>, 4 D multiply, 5 back-projection. This returns the summed
  * milliseconds and call counts per stage (synchronising on pending events). */
 int gw_plan_stage_times(gw_plan* plan, double* ms, long* calls, int n, int reset);
-/* info[0..13] = active x-planes, max sticks per k, fused plane path (0/1) for
+/* info[0..14] = active x-planes, max sticks per k, fused plane path (0/1) for
  * `precision` (2 wave-line, 1 LDS plane tile, 0 split), LDS bytes per
- * workgroup, nproj, the first two factors of n1, n2, n3 (pairs), then the
- * third factor of n1, n2, n3 (1 for a two-factor split). */
+ * workgroup, nproj, the first two factors of n1, n2, n3 (pairs), the third
+ * factor of n1, n2, n3 (1 for a two-factor split), then 0 or the first box
+ * length with no split (no box FFTs on this plan). */
 int gw_plan_info(const gw_plan* plan, int precision, int* info, int n);
 /* kinetic diagonal t = HBAR2_2M |k+G|^2 (nk, npw_max) float64, device (BatchedK.t) */
 int gw_plan_set_kinetic(gw_plan* plan, const double* t, void* stream);

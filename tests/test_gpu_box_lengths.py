@@ -1,0 +1,147 @@
+"""GPU parity on boxes other than the cubic ones of the workloads.
+
+build_fft_grid (grids.py:44-60) gives a non-cubic box for a non-cubic cell or
+an explicit shape, and good_fft_size (grids.py:32-41) returns every 7-smooth
+length. Two-codelet lengths run the generic box FFT kernels on any box shape;
+a box with a length that has no such split (125, 162, ..., every length above
+256) takes its local term through the Toeplitz contraction or, with Toeplitz
+off or too large, the library FFT between the plan's scatter and gather. The
+apply matches the oracle's (core/batch.py:405-505) to 1e-12 (FP64) and 2e-5
+(FP32) relative, and the Davidson eigenvalues the oracle's solve to 1e-10 Ha.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from gpu_cases import dev, rand_block, ref_h, say, select_k
+
+pytestmark = pytest.mark.gpu
+
+from gradwave_amd import _lib as L  # noqa: E402
+from gradwave_amd.hamiltonian import BatchedHamiltonian  # noqa: E402
+from oracle import systems as S  # noqa: E402
+from oracle.units import HARTREE_EV  # noqa: E402
+
+_KPTS = np.array([[0.125, 0.125, 0.125], [0.375, -0.125, 0.25]])
+_SYS: dict = {}
+
+
+def _si2(shape):
+    if shape not in _SYS:
+        cell = S.fcc_primitive(S.A_SI)
+        pos = np.array([[0.0, 0.0, 0.0], [0.25 * S.A_SI] * 3])
+        _SYS[shape] = S.build_system(f"si2_{shape}", cell, pos, [S.SI, S.SI], 30.0, _KPTS, np.full(2, 0.5), 8,
+                                     shape=shape)
+    return _SYS[shape]
+
+
+def _gpu_h(s, toeplitz=None, flags=0):
+    cpu, gpu, p = select_k(s)
+    h = BatchedHamiltonian(gpu, s.shape, s.v_r.to(dev()), p.to(dev()), toeplitz=toeplitz)
+    if flags:
+        h.set_flags(flags)
+    return cpu, gpu, h
+
+
+def _rel(a, b):
+    return float((a - b).abs().max() / b.abs().max())
+
+
+def _check_apply(s, h, cpu, seed):
+    c = rand_block(cpu.nk, 3, cpu.npw_max, cpu.mask, seed=seed)
+    c_pad = c + 5.0 * (~cpu.mask[:, None, :])  # padded input must be ignored
+    want = ref_h(s).apply(c)
+    got = h.apply(c_pad.to(dev())).cpu()
+    assert _rel(got, want) < 1e-12
+    assert float(got[~cpu.mask[:, None, :].expand_as(got)].abs().max()) == 0.0
+    got32 = h.apply(c_pad.to(torch.complex64).to(dev())).cpu().to(torch.complex128)
+    assert _rel(got32, want) < 2e-5
+
+
+@pytest.mark.parametrize("shape,flags", [((24, 30, 36), 0), ((36, 24, 30), 0), ((30, 36, 24), 0),
+                                         ((24, 30, 36), L.GW_PLAN_FORCE_UNFUSED | L.GW_PLAN_LEGACY),
+                                         ((24, 120, 96), 0)])
+def test_noncubic_boxes_match_oracle(shape, flags):
+    """Non-cubic two-codelet boxes on the generic kernels; (24, 120, 96) has
+    FP64 x-planes above the LDS budget (split stick/column kernels), its FP32
+    planes fit (one plane per workgroup)."""
+    say(f"non-cubic box {shape} flags {flags}")
+    s = _si2(shape)
+    cpu, _, h = _gpu_h(s, toeplitz="off", flags=flags)
+    info = h.info()
+    assert info["nosplit_len"] == 0 and h.local_path == "fft"
+    _check_apply(s, h, cpu, seed=11)
+
+
+@pytest.mark.parametrize("shape", [(125, 24, 30), (24, 30, 125), (27, 200, 21), (270, 24, 25)])
+@pytest.mark.parametrize("path", ["toeplitz", "library_fft"])
+def test_boxes_without_codelet_split_match_oracle(shape, path):
+    say(f"box without codelet split {shape} via {path}")
+    s = _si2(shape)
+    cpu, _, h = _gpu_h(s, toeplitz="on" if path == "toeplitz" else "off")
+    assert h.local_path == path and h.info()["nosplit_len"] in shape
+    _check_apply(s, h, cpu, seed=12)
+    # every term separately (the kinetic and nonlocal terms still run on the plan)
+    c = rand_block(cpu.nk, 2, cpu.npw_max, cpu.mask, seed=13)
+    want_all = ref_h(s).apply(c)
+    parts = sum(h.apply_terms(c.to(dev()), t).cpu() for t in (L.GW_TERM_KINETIC, L.GW_TERM_LOCAL,
+                                                               L.GW_TERM_NONLOCAL))
+    assert _rel(parts, want_all) < 1e-12
+
+
+def test_box_without_codelet_split_refuses_box_fft_entry_points():
+    from gradwave_amd import batching as B
+    from gradwave_amd.hamiltonian import density_b
+
+    s = _si2((125, 24, 30))
+    cpu, gpu, h = _gpu_h(s)
+    c = rand_block(cpu.nk, 2, cpu.npw_max, cpu.mask, seed=14).to(dev())
+    with pytest.raises(L.GWError, match="box length 125"):
+        h.wfc_to_real(c)
+    occ = torch.ones(cpu.nk, 2, dtype=torch.float64, device=dev())
+    kw = torch.full((cpu.nk,), 0.5, dtype=torch.float64, device=dev())
+    with pytest.raises(L.GWError, match="box FFT kernels"):
+        density_b(c, occ, kw, gpu, s.shape, s.grid.volume)
+    with pytest.raises(L.GWError, match="ragged batch"):
+        B.RaggedBatch([h])
+
+
+@pytest.mark.parametrize("path", ["toeplitz", "library_fft"])
+def test_davidson_on_box_without_codelet_split_matches_oracle(path):
+    """FP64 Davidson (solvers/davidson.py:527-774) on a 125 x 24 x 30 box, both
+    k-points, 8 bands, against the oracle's solve from the same start."""
+    from gradwave_amd.davidson import davidson_batched
+    from oracle.davidson import davidson_batched as ref_davidson
+
+    say(f"davidson on a 125x24x30 box via {path}")
+    tol = 1e-10 * HARTREE_EV
+    s = _si2((125, 24, 30))
+    cpu, gpu, h = _gpu_h(s, toeplitz="on" if path == "toeplitz" else "off")
+    x0 = rand_block(cpu.nk, 8, cpu.npw_max, cpu.mask, seed=15)
+    r = davidson_batched(h.apply, x0.to(dev()), gpu.t, gpu.mask, tol=tol, max_iter=100)
+    assert float(r.residual_norms.max()) < tol
+    rr = ref_davidson(ref_h(s).apply, x0, cpu.t, cpu.mask, tol=tol, max_iter=100)
+    assert float((r.eigenvalues.cpu() - rr.eigenvalues).abs().max()) < 1e-10 * HARTREE_EV
+
+
+def test_noncubic_cells_in_a_ragged_batch():
+    """The ragged kernels share the generic bodies: non-cubic boxes applied as
+    one batch, against the oracle per cell."""
+    from gradwave_amd import batching as B
+
+    say("non-cubic boxes in one ragged batch")
+    shapes = [(24, 30, 36), (36, 24, 30), (30, 36, 24)]
+    hams, cs, wants = [], [], []
+    for i, sh in enumerate(shapes):
+        s = _si2(sh)
+        cpu, _, h = _gpu_h(s, toeplitz="off")
+        hams.append(h)
+        c = rand_block(cpu.nk, 3, cpu.npw_max, cpu.mask, seed=20 + i)
+        cs.append(c)
+        wants.append(ref_h(s).apply(c))
+    batch = B.RaggedBatch(hams)
+    for dtype, tol in ((torch.complex128, 1e-12), (torch.complex64, 2e-5)):
+        outs = batch.apply([c.to(dtype).to(dev()) for c in cs])
+        for o, w in zip(outs, wants):
+            assert _rel(o.cpu().to(torch.complex128), w) < tol

@@ -86,18 +86,17 @@ def test_argument_errors_are_reported_without_a_gpu():
     assert "ldl 3 < n" in err(lib.gw_chol_inv_batched(1, C.byref(L.CholJob(None, None, None, 4, 4, 3)), None))
     assert lib.gw_chol_inv_batched(0, None, None) == 0 and lib.gw_chol_max_n() == 64
     # a box length that no two codelets <= 16 can split (grids.py sizes are 7-smooth, 11 is not)
-    flat = (C.c_int64 * 1)(0)
-    npw = (C.c_int64 * 1)(1)
-    h = C.c_void_p()
-    assert "box length 11 has no split" in err(lib.gw_plan_create(C.byref(h), 11, 11, 11, 1, 1, flat, npw))
+    f = (C.c_int * 3)()
+    assert "box length 11 has no split" in err(lib.gw_box_split(11, f)) and tuple(f) == (0, 0, 0)
 
 
 @pytest.mark.skipif(not os.path.exists(L.LIB_PATH), reason="library not built")
 def test_box_splits_of_good_fft_sizes():
     """Box lengths the reference's good_fft_size returns (grids.py:32-41): every
     one with a two-codelet split plans with the most balanced such split; the
-    others (125, 162, 200, 216, 243, 250, above 256) are refused, because the
-    three-factor splits are not enabled."""
+    others (125, 162, 200, 216, 243, 250, above 256) have none, because the
+    three-factor splits are not enabled, and plans on such boxes run no box
+    FFT kernels (their local term takes the Toeplitz or library-FFT path)."""
     import ctypes as C
 
     from oracle.grids import good_fft_size
