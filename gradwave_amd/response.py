@@ -30,7 +30,16 @@ device in the padded (nk, nocc, npw_max) layout, one row per column:
   start a fresh CG (at most ``_CERTIFY_ROUNDS`` measurements); ``max_iter``
   bounds the CG steps over all rounds.
 
-The S-metric (ultrasoft/PAW) branch of the reference is not provided.
+S metric (ultrasoft/PAW, postscf/_response.py:691-745): with the overlap
+operator ``s_apply`` and ``s_occ = S c_occ`` the solved system is
+
+    [(1 - P_S^H)(H - eps_n S) + shift * Pi] dpsi = rhs,
+    P_S = sum_m |c_m><c_m| S,   Pi = sum_m S|c_m><c_m| S,
+
+and the result (1 - P_S) dpsi. The same column CG drives it: only the three
+projections change (their bra and ket blocks are c_occ or s_occ), and every
+apply is a full-block one, since S is a block operator. With S the identity
+and s_occ = c_occ every operation is the norm-conserving one.
 """
 
 from __future__ import annotations
@@ -50,8 +59,10 @@ def _i64(t: torch.Tensor) -> torch.Tensor:
 class _ColumnCG:
     """Device state and kernels of one batched Sternheimer solve."""
 
-    def __init__(self, h, t, c_occ, eps_occ, rhs, shift):
+    def __init__(self, h, t, c_occ, eps_occ, rhs, shift, s_apply=None, s_occ=None):
         self.h, self.c_occ, self.rhs, self.shift = h, c_occ.contiguous(), rhs.contiguous(), float(shift)
+        self.s_apply = s_apply
+        self.s_occ = self.c_occ if s_occ is None else s_occ.to(rhs.dtype).contiguous()
         self.nk, self.nocc, self.m = rhs.shape
         self.nc = self.nk * self.nocc
         self.dev = rhs.device
@@ -80,12 +91,27 @@ class _ColumnCG:
                                     self._stream()), "gw_row_dots")
         return out
 
-    def project_occ(self, y: torch.Tensor, sign: float = -1.0, out: torch.Tensor | None = None) -> torch.Tensor:
-        """out + sign * P_occ y on the full layout (out = y, in place, when None)."""
-        ov = LA.inner(self.c_occ, y)  # ov[n][b] = <c_n|y_b>
+    def project(self, y: torch.Tensor, sign: float, bra: torch.Tensor, ket: torch.Tensor,
+                out: torch.Tensor | None = None) -> torch.Tensor:
+        """out + sign * sum_n |ket_n><bra_n|y> on the full layout (out = y, in place,
+        when None): two MFMA contractions."""
+        ov = LA.inner(bra, y)  # ov[n][b] = <bra_n|y_b>
         target = y if out is None else out
-        LA.combine((sign * ov).contiguous(), self.c_occ, out=target, beta=1.0)
+        LA.combine((sign * ov).contiguous(), ket, out=target, beta=1.0)
         return target
+
+    def _hs(self, v: torch.Tensor) -> torch.Tensor:
+        """(H - eps_n S) v on the full block (S = 1 without ``s_apply``)."""
+        hv = self.h.apply(v)
+        hv -= self.eps[..., None] * (v if self.s_apply is None else self.s_apply(v))
+        return hv
+
+    def _finish_a(self, w: torch.Tensor, v: torch.Tensor) -> torch.Tensor:
+        """w <- (1 - P_S^H) w + shift * Pi v, where w = (H - eps S) v."""
+        self.project(w, -1.0, self.c_occ, self.s_occ)
+        if self.shift != 0.0:
+            self.project(v, self.shift, self.s_occ, self.s_occ, out=w)
+        return w
 
     def update(self, x, r, rows, alpha, ap):
         n = int(rows.numel())
@@ -107,39 +133,36 @@ class _ColumnCG:
         return pc
 
     def apply_a(self, rows, pc):
-        """A p = (1 - P_occ)(H - eps) p + shift P_occ p into self.w at the listed rows."""
-        flat_w = self.w.view(self.nc, self.m)
+        """A p = (1 - P_S^H)(H - eps S) p + shift Pi p into self.w at the listed rows
+        (pc: the compacted active directions, norm-conserving case only)."""
         if pc is not None:
             kc = self.kcol_all[rows]
             hp = self.h.apply_cols(pc, kc)
             hp -= self.eps.view(-1)[rows][:, None] * pc
-            flat_w.index_copy_(0, rows, hp)
+            self.w.view(self.nc, self.m).index_copy_(0, rows, hp)
         else:
-            self.w.copy_(self.h.apply(self.p))
-            self.w -= self.eps[..., None] * self.p
-        self.project_occ(self.w, -1.0)
-        if self.shift != 0.0:
-            self.project_occ(self.p, self.shift, out=self.w)
-        return self.w
+            self.w.copy_(self._hs(self.p))
+        return self._finish_a(self.w, self.p)
 
     def true_residual(self, x):
         """rhs - A x for every column (one block apply)."""
-        hx = self.h.apply(x) - self.eps[..., None] * x
-        self.project_occ(hx, -1.0)
-        if self.shift != 0.0:
-            self.project_occ(x, self.shift, out=hx)
-        return self.rhs - hx
+        return self.rhs - self._finish_a(self._hs(x), x)
 
 
 @torch.no_grad()
 def cg_sternheimer(h, bk, c_occ: torch.Tensor, eps_occ: torch.Tensor, rhs: torch.Tensor, x0: torch.Tensor,
-                   shift: float, tol: float = 1e-8, max_iter: int = 400) -> torch.Tensor:
-    """Batched conduction-projected Sternheimer solve (postscf/_response.py:502-595).
+                   shift: float, tol: float = 1e-8, max_iter: int = 400, s_apply=None,
+                   s_occ: torch.Tensor | None = None) -> torch.Tensor:
+    """Batched conduction-projected Sternheimer solve (postscf/_response.py:502-595;
+    S metric 691-745).
 
     h: operator with ``apply`` (and optionally ``apply_cols``); bk: object with
     the kinetic table ``t`` (nk, npw_max) and optionally ``mask``;
     c_occ, rhs, x0: (nk, nocc, npw_max) complex on the GPU; eps_occ (nk, nocc).
-    Returns (1 - P_occ) dpsi."""
+    ``s_apply`` (a block operator, S c for c of that layout) with ``s_occ`` =
+    S c_occ selects the S metric. Returns (1 - P_occ) dpsi, or (1 - P_S) dpsi."""
+    if s_apply is not None and s_occ is None:
+        raise ValueError("the S-metric cg_sternheimer needs s_occ = S c_occ")
     L.require_gpu(rhs, "rhs")
     nk, nocc, m = rhs.shape
     x = x0.to(rhs.dtype).clone().contiguous()
@@ -148,8 +171,8 @@ def cg_sternheimer(h, bk, c_occ: torch.Tensor, eps_occ: torch.Tensor, rhs: torch
         x *= mask[:, None, :].to(x.dtype)
     if nk * nocc == 0:
         return x
-    cg = _ColumnCG(h, bk.t, c_occ, eps_occ, rhs, shift)
-    compact = hasattr(h, "apply_cols")
+    cg = _ColumnCG(h, bk.t, c_occ, eps_occ, rhs, shift, s_apply, s_occ)
+    compact = hasattr(h, "apply_cols") and s_apply is None
     used = 0
     for _ in range(_CERTIFY_ROUNDS):
         r = cg.true_residual(x).contiguous()
@@ -173,4 +196,4 @@ def cg_sternheimer(h, bk, c_occ: torch.Tensor, eps_occ: torch.Tensor, rhs: torch
             beta = rz_new[keep] / rz[keep].clamp(min=1e-300)
             rows, rz = rows[keep].contiguous(), rz_new[keep]
             pc = cg.direction(rows, beta.contiguous(), compact)
-    return cg.project_occ(x, -1.0)
+    return cg.project(x, -1.0, cg.s_occ, cg.c_occ)

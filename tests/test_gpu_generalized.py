@@ -149,3 +149,71 @@ def test_dual_grid_and_hubbard_term_match_oracle():
     eig, _ = davidson_gen_batched(hs, x0.to(d), 8, tol=1e-9, max_iter=200)
     want_e, _ = ref_gen(rhs, x0, 8, tol=1e-9, max_iter=200)
     assert float((eig.cpu() - want_e).abs().max()) / HARTREE_EV < 1e-10
+
+
+def test_sternheimer_s_metric_matches_dense_solve():
+    """S-metric cg_sternheimer (postscf/_response.py:691-745) on Si2, two
+    k-points, the four lowest pairs of H c = eps S c (c^H S c = 1): against a
+    dense solve of [(1 - S P)(H - eps_n S) + s S P S] x = rhs with P = sum |c><c|
+    and rhs in the S-conduction space, returned as (1 - P S) x. With S = 1 and
+    s_occ = c_occ (orthonormal) it runs the norm-conserving solve's operations
+    (_response.py:706-709)."""
+    from types import SimpleNamespace
+
+    from gradwave_amd.generalized import BatchedHS
+    from gradwave_amd.response import cg_sternheimer
+
+    say("sternheimer cg, S metric, si2")
+    s, cpu, gpu, p, q, _, sub, dense_h = _cases()
+    d = dev()
+    hs = BatchedHS(gpu, s.shape, s.v_r.to(d), p.to(d), cpu.dij_full, q)
+    nk, m = cpu.mask.shape
+    nocc, shift = 4, 5.0
+    c_occ = torch.zeros(nk, nocc, m, dtype=torch.complex128)
+    eps = torch.zeros(nk, nocc, dtype=torch.float64)
+    rhs = torch.zeros_like(c_occ)
+    want = torch.zeros_like(c_occ)
+    rng = np.random.default_rng(7)
+    for i in range(nk):
+        n = int(cpu.npw[i])
+        hd, sd = dense_h(sub, i).numpy(), dense_s(p[i], q, n).numpy()
+        w, v = sl.eigh(hd, sd)
+        c = v[:, :nocc]
+        c_occ[i, :, :n] = torch.as_tensor(c.T)
+        eps[i] = torch.as_tensor(w[:nocc])
+        pm = c @ c.conj().T
+        b = rng.standard_normal((n, nocc)) + 1j * rng.standard_normal((n, nocc))
+        b = b - sd @ (pm @ b)
+        rhs[i, :, :n] = torch.as_tensor(b.T)
+        for j in range(nocc):
+            a = (np.eye(n) - sd @ pm) @ (hd - w[j] * sd) + shift * (sd @ pm @ sd)
+            x = np.linalg.solve(a, b[:, j])
+            want[i, j, :n] = torch.as_tensor(x - pm @ (sd @ x))
+    x0 = torch.zeros_like(rhs).to(d)
+    c_d = c_occ.to(d)
+    got = cg_sternheimer(hs.ham, gpu, c_d, eps.to(d), rhs.to(d), x0, shift, tol=1e-10, max_iter=500,
+                         s_apply=hs.s, s_occ=hs.s(c_d)).cpu()
+    err = float((got - want).abs().max() / want.abs().max())
+    assert err < 1e-7, err
+    with pytest.raises(ValueError, match="s_occ"):
+        cg_sternheimer(hs.ham, gpu, c_d, eps.to(d), rhs.to(d), x0, shift, s_apply=hs.s)
+    # the norm-conserving limit: S = 1 and s_occ = c_occ (H's own orthonormal
+    # eigenvectors here) run the plain solve's operations
+    c_nc = torch.zeros_like(c_occ)
+    eps_nc = torch.zeros_like(eps)
+    rhs_nc = torch.zeros_like(rhs)
+    for i in range(nk):
+        n = int(cpu.npw[i])
+        w, u = torch.linalg.eigh(dense_h(sub, i))
+        c = u[:, :nocc]
+        c_nc[i, :, :n] = c.T
+        eps_nc[i] = w[:nocc]
+        b = torch.as_tensor(rng.standard_normal((n, nocc)) + 1j * rng.standard_normal((n, nocc)))
+        rhs_nc[i, :, :n] = (b - c @ (c.conj().T @ b)).T
+    nc = SimpleNamespace(apply=hs.ham.apply)
+    args = (gpu, c_nc.to(d), eps_nc.to(d), rhs_nc.to(d), x0, shift)
+    plain = cg_sternheimer(nc, *args, tol=1e-10, max_iter=500)
+    ident = cg_sternheimer(nc, *args, tol=1e-10, max_iter=500, s_apply=lambda y: y.clone(),
+                           s_occ=c_nc.to(d))
+    assert bool(torch.isfinite(plain).all())
+    assert float((plain - ident).abs().max() / plain.abs().max()) < 1e-13
