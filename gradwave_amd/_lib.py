@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libgradwave_hip.so")
 
 GW_FP64, GW_FP32 = 0, 1
 GW_TERM_KINETIC, GW_TERM_LOCAL, GW_TERM_NONLOCAL, GW_TERM_ALL = 1, 2, 4, 7
-GW_PLAN_FORCE_UNFUSED, GW_PLAN_TIMING, GW_PLAN_LEGACY = 1, 2, 4
+GW_PLAN_FORCE_UNFUSED, GW_PLAN_TIMING, GW_PLAN_LEGACY, GW_PLAN_PENCIL = 1, 2, 4, 8
 GW_TUNE_RADIX, GW_TUNE_GEMM_WGS, GW_TUNE_GEMM_TILE, GW_TUNE_XPASS, GW_TUNE_COMBINE_LIB, GW_TUNE_INNER_LIB = 1, 2, 3, 4, 5, 6
 GW_TUNE_BAND_CHUNK = 7
 GW_TUNE_PLANE_BANDS = 8

@@ -67,6 +67,8 @@ struct Plan {
   double* vp64;
   float* vp32;
   double* vx64;     // V with natural x order, permuted y/z (register x pass)
+  double* vz64;     // V z-major at the permuted positions (pencil slabs)
+  float* vz32;
   float* vx32;
   int* slot_of;     // (n1) active-plane slot of each x index, -1 if inactive
   int xlo, xhi;     // active planes = [0, xlo) U [xhi, n1) when xlo >= 0
@@ -133,6 +135,7 @@ struct ApplyLayout {
   size_t part_off, part_bytes;    // split-K partials of the projection GEMM
   size_t total;
   bool fused;
+  int pencil_yc;  // > 0: the pencil path, y-lines per x chunk of its slab kernel
 };
 // chunked: the layout of gw_hpsi_apply, whose box holds one band chunk (GW_TUNE_BAND_CHUNK)
 ApplyLayout apply_layout(const Plan* p, int nb, int precision, bool chunked = false);

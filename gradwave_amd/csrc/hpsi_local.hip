@@ -183,6 +183,20 @@ bool plane_fits_lds(const Plan* p, int precision) {
   return (size_t)p->n[1] * p->n[2] * csize(precision) <= (size_t)std::min(p->lds_max, kMaxLds);
 }
 
+// y-lines per x chunk of the pencil slab kernel, 0 when the pencil path is off
+// or a slab of the active planes does not fit LDS: two workgroups per CU when
+// that leaves at least 8 lines, else one.
+static int pencil_yc(const Plan* p, int precision) {
+  if (!(p->flags & GW_PLAN_PENCIL) || p->nosplit_len) return 0;
+  const long e = (long)csize(precision);
+  const long cap = std::min(p->lds_max, kMaxLds);
+  const long slab = (long)p->na * p->n[1] * e;
+  const long line = (long)p->n[0] * e;
+  const long two = (cap / 2 - slab) / line, one = (cap - slab) / line;
+  const long yc = two >= 8 ? two : one;
+  return yc >= 1 ? (int)std::min<long>(yc, p->n[1]) : 0;
+}
+
 static int x_lines(const Plan* p) {
   int n23 = p->n[1] * p->n[2];
   int L = (kXVpt * kXThreads) / p->n[0];
@@ -196,15 +210,16 @@ ApplyLayout apply_layout(const Plan* p, int nb, int precision, bool chunked) {
   const size_t e = csize(precision);
   const size_t nbt = (size_t)p->nk * nb;
   L.fused = plane_fits_lds(p, precision);
+  L.pencil_yc = chunked ? pencil_yc(p, precision) : 0;  // the apply's layout only
   size_t off = 0;
   L.box_off = off;
   const int chunk = chunked ? g_band_chunk[precision == GW_FP64 ? 0 : 1] : 0;
   // a plan without box FFTs (a length with no codelet split) needs no box
   const size_t nbt_box = p->nosplit_len ? 0 : (size_t)p->nk * (chunk > 0 && chunk < nb ? chunk : nb);
-  L.box_bytes = nbt_box * p->na * p->n[1] * p->n[2] * e;
+  L.box_bytes = L.pencil_yc ? 0 : nbt_box * p->na * p->n[1] * p->n[2] * e;
   off = align256(off + L.box_bytes);
   L.stick_off = off;
-  L.stick_bytes = L.fused ? 0 : nbt_box * p->nst_max * p->n[2] * e;
+  L.stick_bytes = L.fused && !L.pencil_yc ? 0 : nbt_box * p->nst_max * p->n[2] * e;
   off = align256(off + L.stick_bytes);
   L.becp_off = off;
   L.becp_bytes = (size_t)p->nk * p->nproj * nb * e;
@@ -810,6 +825,170 @@ k_sticks_fwd(Geom g, const cpx<T>* __restrict__ sticks, const cpx<T>* __restrict
       }
       ob[si] = v;
     }
+}
+
+// ---- pencil path: z sticks -> (y, x) slabs -> z sticks ----
+// The z transforms run on chunks of sticks and store them z-major,
+// zs[bi][z][s] (s < nst_max), so that one slab kernel per (z, band) reads and
+// writes its nst contiguous values; the slab kernel does the y inverse on the
+// active planes, the x inverse / V / x forward in chunks of y-lines, and the y
+// forward, all in LDS. Between kernels only sticks move (nst * n3 values per
+// band, against na * n2 * n3 for the plane pipeline).
+constexpr int kPencilChunk = 16;    // sticks per z-pass workgroup
+constexpr int kPencilThreads = 512;
+
+template <typename T, int RMAX = 16>
+__global__ void __launch_bounds__(256)
+k_pencil_z_inv(Geom g, const cpx<T>* __restrict__ c, long ldc, long sc, cpx<T>* __restrict__ zs,
+               const cpx<T>* __restrict__ twz) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);  // [kPencilChunk][n3]
+  const int bi = blockIdx.y;
+  const int k = bi / g.nb, b = bi - k * g.nb;
+  const int nst = g.st_plane_ptr[(long)k * (g.na + 1) + g.na];
+  const int s0 = blockIdx.x * kPencilChunk;
+  if (s0 >= nst) return;
+  const int ns = min(kPencilChunk, nst - s0);
+  for (int e = threadIdx.x; e < kPencilChunk * g.n3; e += blockDim.x) tile[e] = czero<T>();
+  __syncthreads();
+  const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
+  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  const int* src = g.pt_src + (long)k * g.npw_max;
+  const int* yz = g.pt_yz + (long)k * g.npw_max;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  for (int j = wave; j < ns; j += nw)
+    for (int q = pp[s0 + j] + lane; q < pp[s0 + j + 1]; q += 64) tile[j * g.n3 + yz[q] % g.n3] = cb[src[q]];
+  __syncthreads();
+  line_fft_inverse<T, false, RMAX>(tile, twz, g.fz0, g.fz1, ns, g.n3, 1, nullptr, g.fz2);
+  __syncthreads();
+  cpx<T>* dst = zs + (long)bi * g.n3 * g.nst_max + s0;
+  for (int e = threadIdx.x; e < ns * g.n3; e += blockDim.x) {
+    const int z = e / ns, j = e - z * ns;
+    dst[(long)z * g.nst_max + j] = tile[j * g.n3 + z];
+  }
+}
+
+template <typename T, bool KIN, int RMAX = 16>
+__global__ void __launch_bounds__(256)
+k_pencil_z_fwd(Geom g, const cpx<T>* __restrict__ zs, const cpx<T>* __restrict__ c, long ldc, long sc,
+               const T* __restrict__ t, cpx<T>* __restrict__ out, long ldo, long so, T scale,
+               const cpx<T>* __restrict__ twz) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cpx<T>* tile = reinterpret_cast<cpx<T>*>(smem_raw);
+  const int bi = blockIdx.y;
+  const int k = bi / g.nb, b = bi - k * g.nb;
+  cpx<T>* ob = out + (long)k * so + (long)b * ldo;
+  if (blockIdx.x == 0) {
+    for (int q = g.npw[k] + threadIdx.x; q < g.npw_max; q += blockDim.x) ob[q] = czero<T>();
+  }
+  const int nst = g.st_plane_ptr[(long)k * (g.na + 1) + g.na];
+  const int s0 = blockIdx.x * kPencilChunk;
+  if (s0 >= nst) return;
+  const int ns = min(kPencilChunk, nst - s0);
+  const cpx<T>* srcb = zs + (long)bi * g.n3 * g.nst_max + s0;
+  for (int e = threadIdx.x; e < ns * g.n3; e += blockDim.x) {
+    const int z = e / ns, j = e - z * ns;
+    tile[j * g.n3 + z] = srcb[(long)z * g.nst_max + j];
+  }
+  __syncthreads();
+  line_fft_forward<T, false, RMAX>(tile, twz, g.fz0, g.fz1, ns, g.n3, 1, nullptr, g.fz2);
+  __syncthreads();
+  const int* pp = g.st_pt_ptr + (long)k * (g.nst_max + 1);
+  const cpx<T>* cb = c + (long)k * sc + (long)b * ldc;
+  const int* src = g.pt_src + (long)k * g.npw_max;
+  const int* yz = g.pt_yz + (long)k * g.npw_max;
+  const T* tk = t + (long)k * g.npw_max;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, nw = blockDim.x / 64;
+  for (int j = wave; j < ns; j += nw)
+    for (int q = pp[s0 + j] + lane; q < pp[s0 + j + 1]; q += 64) {
+      const int si = src[q];
+      cpx<T> v = tile[j * g.n3 + yz[q] % g.n3];
+      v.x *= scale; v.y *= scale;
+      if constexpr (KIN) {
+        const cpx<T> cv = cb[si];
+        v.x += tk[si] * cv.x; v.y += tk[si] * cv.y;
+      }
+      ob[si] = v;
+    }
+}
+
+// plane slot a of stick s of one k: sp[a] <= s < sp[a + 1] (sp has na + 1 entries)
+__device__ __forceinline__ int stick_plane(const int* sp, int na, int s) {
+  int lo = 0, hi = na;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (sp[mid] <= s) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// One (z, band) slab: y inverse on the na active planes' lines, then per chunk
+// of YC y-lines x inverse, x V(r) (vz: z-major, permuted positions), x forward,
+// then y forward; reads and writes the band's nst stick values at z.
+template <typename T, int RMAX = 16>
+__global__ void __launch_bounds__(kPencilThreads)
+k_pencil_yx(Geom g, int nbt, int YC, cpx<T>* __restrict__ zs, const T* __restrict__ vz,
+            const cpx<T>* __restrict__ twx, const cpx<T>* __restrict__ twy) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int n1 = g.n1, n2 = g.n2, na = g.na;
+  cpx<T>* A = reinterpret_cast<cpx<T>*>(smem_raw);  // [na][n2]
+  cpx<T>* X = A + (long)na * n2;                     // [YC][n1]
+  const long w = xcd_work_id((long)g.n3 * nbt);     // consecutive work ids share z (V reuse in L2)
+  if (w < 0) return;
+  const int z = (int)(w / nbt), bi = (int)(w - (long)z * nbt);
+  const int k = bi / g.nb;
+  const int* sp = g.st_plane_ptr + (long)k * (na + 1);
+  const int* i1s = g.st_i1 + (long)k * g.nst_max;
+  const int nst = sp[na];
+  cpx<T>* row = zs + ((long)bi * g.n3 + z) * g.nst_max;
+  for (int e = threadIdx.x; e < na * n2; e += blockDim.x) A[e] = czero<T>();
+  __syncthreads();
+  for (int s = threadIdx.x; s < nst; s += blockDim.x) A[stick_plane(sp, na, s) * n2 + i1s[s]] = row[s];
+  __syncthreads();
+  line_fft_inverse<T, false, RMAX>(A, twy, g.fy0, g.fy1, na, n2, 1, nullptr, g.fy2);
+  __syncthreads();
+  const T* vzz = vz + (long)z * n2 * n1;
+  for (int y0 = 0; y0 < n2; y0 += YC) {
+    const int yc = min(YC, n2 - y0);
+    for (int e = threadIdx.x; e < yc * n1; e += blockDim.x) X[e] = czero<T>();
+    __syncthreads();
+    for (int e = threadIdx.x; e < yc * na; e += blockDim.x) {
+      const int yy = e / na, a = e - yy * na;
+      X[yy * n1 + g.planes[a]] = A[a * n2 + y0 + yy];
+    }
+    __syncthreads();
+    line_fft_inverse<T, false, RMAX>(X, twx, g.fx0, g.fx1, yc, n1, 1, nullptr, g.fx2);
+    __syncthreads();
+    for (int e = threadIdx.x; e < yc * n1; e += blockDim.x) {
+      const T v = vzz[(long)y0 * n1 + e];
+      X[e].x *= v; X[e].y *= v;
+    }
+    __syncthreads();
+    line_fft_forward<T, false, RMAX>(X, twx, g.fx0, g.fx1, yc, n1, 1, nullptr, g.fx2);
+    __syncthreads();
+    for (int e = threadIdx.x; e < yc * na; e += blockDim.x) {
+      const int yy = e / na, a = e - yy * na;
+      A[a * n2 + y0 + yy] = X[yy * n1 + g.planes[a]];
+    }
+    __syncthreads();
+  }
+  line_fft_forward<T, false, RMAX>(A, twy, g.fy0, g.fy1, na, n2, 1, nullptr, g.fy2);
+  __syncthreads();
+  for (int s = threadIdx.x; s < nst; s += blockDim.x) row[s] = A[stick_plane(sp, na, s) * n2 + i1s[s]];
+}
+
+// V(r) for the pencil slabs: vz[(zp * n2 + yp) * n1 + xp] = V at the permuted
+// positions (xp, yp, zp), i.e. the transpose of vp.
+template <typename T>
+__global__ void k_permute_v_zyx(int n1, int n2, int n3, const int* __restrict__ px, const int* __restrict__ py,
+                                const int* __restrict__ pz, const double* __restrict__ v, T* __restrict__ vz) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n = (long)n1 * n2 * n3;
+  if (e >= n) return;
+  const int xp = (int)(e % n1);
+  const int yp = (int)((e / n1) % n2);
+  const int zp = (int)(e / ((long)n1 * n2));
+  vz[e] = (T)v[((long)px[xp] * n2 + py[yp]) * n3 + pz[zp]];
 }
 
 // kinetic only: out = t*c (0 in padding)
@@ -1875,6 +2054,8 @@ extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int
       hipMalloc(&P.vp32, (size_t)N * sizeof(float)) != hipSuccess ||
       hipMalloc(&P.vx64, (size_t)N * sizeof(double)) != hipSuccess ||
       hipMalloc(&P.vx32, (size_t)N * sizeof(float)) != hipSuccess ||
+      hipMalloc(&P.vz64, (size_t)N * sizeof(double)) != hipSuccess ||
+      hipMalloc(&P.vz32, (size_t)N * sizeof(float)) != hipSuccess ||
       hipMalloc(&h->ident, (size_t)n1 * sizeof(int)) != hipSuccess ||
       hipMalloc(&h->ident_y, (size_t)n2 * sizeof(int)) != hipSuccess ||
       hipMalloc(&h->ident_z, (size_t)n3 * sizeof(int)) != hipSuccess) {
@@ -1886,6 +2067,8 @@ extern "C" int gw_plan_create(gw_plan** out, int n1, int n2, int n3, int nk, int
   hipMemset(P.vp32, 0, (size_t)N * sizeof(float));
   hipMemset(P.vx64, 0, (size_t)N * sizeof(double));
   hipMemset(P.vx32, 0, (size_t)N * sizeof(float));
+  hipMemset(P.vz64, 0, (size_t)N * sizeof(double));
+  hipMemset(P.vz32, 0, (size_t)N * sizeof(float));
   {
     std::vector<int> id(std::max(n1, std::max(n2, n3)));
     std::iota(id.begin(), id.end(), 0);
@@ -1962,7 +2145,7 @@ extern "C" int gw_plan_destroy(gw_plan* h) {
   Plan& P = h->p;
   void* ptrs[] = {P.planes, P.st_plane_ptr, P.st_i1, P.st_pt_ptr, P.pt_src, P.pt_yz, P.npw,
                   P.tw64[0], P.tw64[1], P.tw64[2], P.tw32[0], P.tw32[1], P.tw32[2],
-                  P.t64, P.t32, P.vp64, P.vp32, P.vx64, P.vx32, P.slot_of, P.p64, P.p32, P.d_colptr,
+                  P.t64, P.t32, P.vp64, P.vp32, P.vx64, P.vx32, P.vz64, P.vz32, P.slot_of, P.p64, P.p32, P.d_colptr,
                   P.d_row, P.d_val, h->perm[0], h->perm[1], h->perm[2], h->ident, h->ident_y,
                   h->ident_z};
   hipDeviceSynchronize();
@@ -1998,6 +2181,10 @@ extern "C" int gw_plan_set_potential(gw_plan* h, const double* v, void* stream) 
                      P.n[2], h->ident, h->ident_y, h->ident_z, v, P.vx64);
   hipLaunchKernelGGL(k_permute_v<float>, dim3(grid_blocks(n, 256)), dim3(256), 0, st, P.n[0], P.n[1],
                      P.n[2], h->ident, h->ident_y, h->ident_z, v, P.vx32);
+  hipLaunchKernelGGL(k_permute_v_zyx<double>, dim3(grid_blocks(n, 256)), dim3(256), 0, st, P.n[0], P.n[1],
+                     P.n[2], h->perm[0], h->perm[1], h->perm[2], v, P.vz64);
+  hipLaunchKernelGGL(k_permute_v_zyx<float>, dim3(grid_blocks(n, 256)), dim3(256), 0, st, P.n[0], P.n[1],
+                     P.n[2], h->perm[0], h->perm[1], h->perm[2], v, P.vz32);
   GW_CHECK_HIP(hipGetLastError());
   P.have_v = true;
   return GW_OK;
@@ -2061,6 +2248,31 @@ static int generic_local(const Plan& P, const ApplyLayout& L, const Geom& g, int
   const dim3 gcol(P.na, nbt, (P.n[2] + CC - 1) / CC);
   const size_t col_lds = (size_t)P.n[1] * CC * sizeof(cpx<T>);
   const size_t stick_lds = (size_t)kStickChunk * P.n[2] * sizeof(cpx<T>);
+  if (L.pencil_yc > 0) {
+    cpx<T>* zs = (cpx<T>*)(ws + L.stick_off);
+    const T* vz = sizeof(T) == 8 ? (const T*)P.vz64 : (const T*)P.vz32;
+    const size_t zlds = (size_t)kPencilChunk * P.n[2] * sizeof(cpx<T>);
+    const dim3 zgrid((P.nst_max + kPencilChunk - 1) / kPencilChunk, nbt);
+    hipEvent_t ev = tm.begin();
+    hipLaunchKernelGGL((k_pencil_z_inv<T, RMAX>), zgrid, dim3(256), zlds, st, g, c, ldc, sc, zs, twz);
+    tm.end(ST_PLANE_INV, ev);
+    const long total = (long)P.n[2] * nbt;
+    const size_t slab_lds = ((size_t)P.na * P.n[1] + (size_t)L.pencil_yc * P.n[0]) * sizeof(cpx<T>);
+    ev = tm.begin();
+    hipLaunchKernelGGL((k_pencil_yx<T, RMAX>), dim3((unsigned)(8 * ((total + 7) / 8))), dim3(kPencilThreads),
+                       slab_lds, st, g, nbt, L.pencil_yc, zs, vz, twx, twy);
+    tm.end(ST_XPASS, ev);
+    ev = tm.begin();
+    if (kin)
+      hipLaunchKernelGGL((k_pencil_z_fwd<T, true, RMAX>), zgrid, dim3(256), zlds, st, g, zs, c, ldc, sc, t, out,
+                         ldo, so, scale, twz);
+    else
+      hipLaunchKernelGGL((k_pencil_z_fwd<T, false, RMAX>), zgrid, dim3(256), zlds, st, g, zs, c, ldc, sc, t, out,
+                         ldo, so, scale, twz);
+    tm.end(ST_PLANE_FWD, ev);
+    GW_CHECK_HIP(hipGetLastError());
+    return GW_OK;
+  }
   hipEvent_t ev = tm.begin();
   if (L.fused) {
     hipLaunchKernelGGL((k_plane_inv<T, RMAX>), dim3(P.na, nbt), dim3(kPlaneThreads), plane_lds, st, g, c, ldc, sc,
@@ -2117,6 +2329,13 @@ static void set_generic_lds_attrs_r(int want) {
                             want);
   (void)hipFuncSetAttribute((const void*)k_plane_inv_y<T, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
   (void)hipFuncSetAttribute((const void*)k_plane_fwd_y<T, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_pencil_yx<T, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize, want);
+  (void)hipFuncSetAttribute((const void*)k_pencil_z_inv<T, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_pencil_z_fwd<T, true, RMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            want);
+  (void)hipFuncSetAttribute((const void*)k_pencil_z_fwd<T, false, RMAX>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, want);
 }
 template <typename T>
 static void set_generic_lds_attrs(int want) {
@@ -2155,7 +2374,7 @@ static int apply_t(gw_plan* h, int terms, const cpx<T>* c, long ldc, long sc, cp
       cpx<T>* oc = out + (long)b0 * ldo;
       tm.count_calls = b0 == 0;
       int rc;
-      if (wave_path_ok(P, prec)) {
+      if (wave_path_ok(P, prec) && !L.pencil_yc) {
         const T* vn = prec == GW_FP64 ? (const T*)P.vx64 : (const T*)P.vx32;
         rc = launch_wave_path<T>(P, gc, P.nk * nbc, cc, ldc, sc, box, vn, t, oc, ldo, so, scale, kin, tm, st);
       } else {

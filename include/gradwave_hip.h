@@ -63,8 +63,11 @@ int gw_plan_destroy(gw_plan* plan);
 /* plan options (no reference counterpart): GW_PLAN_LEGACY selects the generic
  * LDS kernels (any 7-smooth box) instead of the wave-line kernels of
  * cubic 24/36/90/120 boxes; GW_PLAN_FORCE_UNFUSED then splits each x-plane pass
- * into stick and column-chunk kernels (both for testing). */
-enum { GW_PLAN_FORCE_UNFUSED = 1, GW_PLAN_TIMING = 2, GW_PLAN_LEGACY = 4 };
+ * into stick and column-chunk kernels (both for testing). GW_PLAN_PENCIL runs
+ * the local term as z-stick transforms around one (y, x) slab kernel per
+ * (z, band) wherever a slab of the active planes fits LDS (moving sticks, not
+ * planes, between kernels). */
+enum { GW_PLAN_FORCE_UNFUSED = 1, GW_PLAN_TIMING = 2, GW_PLAN_LEGACY = 4, GW_PLAN_PENCIL = 8 };
 int gw_plan_set_flags(gw_plan* plan, int flags);
 /* With GW_PLAN_TIMING set, gw_hpsi_apply records HIP events around each of its
  * stages on the launch stream: 0 x-plane inverse (pack, z, y), 1 x pass

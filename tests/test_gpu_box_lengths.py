@@ -145,3 +145,39 @@ def test_noncubic_cells_in_a_ragged_batch():
         outs = batch.apply([c.to(dtype).to(dev()) for c in cs])
         for o, w in zip(outs, wants):
             assert _rel(o.cpu().to(torch.complex128), w) < tol
+
+
+@pytest.mark.parametrize("shape", [(24, 24, 24), (24, 30, 36), (24, 120, 96), (120, 120, 120)])
+def test_pencil_path_matches_oracle(shape):
+    """GW_PLAN_PENCIL: the local term as z-stick transforms around one (y, x)
+    slab kernel per (z, band), against the oracle's apply (FP64 1e-12, FP32
+    2e-5) and against the plan's default path."""
+    say(f"pencil path {shape}")
+    s = _si2(shape)
+    cpu, _, h = _gpu_h(s, toeplitz="off")
+    c = rand_block(cpu.nk, 3, cpu.npw_max, cpu.mask, seed=31)
+    base = h.apply(c.to(dev())).cpu()
+    h.set_flags(L.GW_PLAN_PENCIL)
+    _check_apply(s, h, cpu, seed=31)
+    assert _rel(h.apply(c.to(dev())).cpu(), base) < 1e-12
+
+
+@pytest.mark.parametrize("name,nb", [("si8", 40), ("si64", 24)])
+def test_pencil_path_matches_default_path(name, nb):
+    """Many bands (band chunks, every (z, band) work item) on the 36^3 and
+    90^3 workloads: the pencil path equals the wave-line path to 1e-12 (FP64)
+    and 2e-6 (FP32), and the oracle on three bands."""
+    from gpu_cases import system
+
+    say(f"pencil path {name} nb={nb}")
+    s = system(name)
+    sel = [0, 2] if name == "si8" else None
+    cpu, gpu, p = select_k(s, sel)
+    h = BatchedHamiltonian(gpu, s.shape, s.v_r.to(dev()), p.to(dev()), toeplitz="off")
+    c = rand_block(cpu.npw.shape[0], nb, cpu.mask.shape[1], cpu.mask, seed=32).to(dev())
+    base64, base32 = h.apply(c).cpu(), h.apply(c.to(torch.complex64)).cpu()
+    h.set_flags(L.GW_PLAN_PENCIL)
+    assert _rel(h.apply(c).cpu(), base64) < 1e-12
+    assert _rel(h.apply(c.to(torch.complex64)).cpu(), base32) < 2e-6
+    want = ref_h(s, sel).apply(c[:, :3].cpu())
+    assert _rel(h.apply(c[:, :3].contiguous()).cpu(), want) < 1e-12
