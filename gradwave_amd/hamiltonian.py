@@ -127,8 +127,9 @@ class BatchedHamiltonian:
     (M built by gw_toeplitz_build, applied by the MFMA inner product) when the
     matrix fits the memory cap, else scatter -> rocFFT inverse -> x V ->
     rocFFT forward -> gather (torch.fft on the GPU; the scatter and gather are
-    the plan's kernels). psi(r), the density and the V(r) adjoint refuse such
-    boxes with GWError.
+    the plan's kernels). psi(r) and the density take the scatter and rocFFT
+    there too; the V(r) adjoint (forces) and ragged batches refuse such boxes
+    with GWError.
     """
 
     def __init__(self, bk, shape, v_eff_r: torch.Tensor, p: torch.Tensor, hub_q=None,
@@ -362,6 +363,8 @@ class BatchedHamiltonian:
         """psi(r) = N ifftn(scatter c) on the natural box (nk, nb, n1, n2, n3)
         -- core/batch.py:208-216 (g_to_r_b)."""
         L.require_gpu(c, "c")
+        if self.plan.nosplit_len:
+            return _to_real_library(self.plan, c)
         prec = L.precision_of(c.dtype)
         c = c.contiguous()
         nk, nb, _ = c.shape
@@ -372,6 +375,18 @@ class BatchedHamiltonian:
                                        ws.data_ptr(), ws.numel(), L.stream_ptr(c.device)),
                 "gw_wfc_to_real")
         return psi
+
+
+def _to_real_library(plan: Plan, c: torch.Tensor) -> torch.Tensor:
+    """psi(r) = N ifftn(scatter c) for a plan without box FFT kernels: the plan's
+    scatter, then rocFFT (torch.fft on the GPU)."""
+    c = c.contiguous()
+    nk, nb, _ = (int(x) for x in c.shape)
+    box = torch.empty(nk, nb, *plan.shape, dtype=c.dtype, device=c.device)
+    if nb:
+        L.check(L.lib().gw_sphere_to_box(plan.handle, L.precision_of(c.dtype), c.data_ptr(), box.data_ptr(), nb,
+                                         L.stream_ptr(c.device)), "gw_sphere_to_box")
+    return torch.fft.ifftn(box, dim=(-3, -2, -1), norm="forward")
 
 
 _DENSITY_PLANS: dict[tuple, tuple] = {}
@@ -394,7 +409,8 @@ def density_b(coeffs: torch.Tensor, occ: torch.Tensor, kweights: torch.Tensor, b
 
     Cubic 24/36/90/120 boxes run the fused wave-line density kernel (one inverse
     transform per band, FP64 accumulation); other boxes transform with
-    ``gw_wfc_to_real`` and reduce on the GPU."""
+    ``gw_wfc_to_real`` and reduce on the GPU; boxes with a length that has no
+    codelet split transform with the plan's scatter and rocFFT, in band chunks."""
     L.require_gpu(coeffs, "coeffs")
     plan = _density_plan(bk, shape)
     prec = L.precision_of(coeffs.dtype)
@@ -402,6 +418,13 @@ def density_b(coeffs: torch.Tensor, occ: torch.Tensor, kweights: torch.Tensor, b
     nk, nb, _ = c.shape
     w = (kweights.to(c.device, torch.float64)[:, None] * occ.to(c.device, torch.float64)).contiguous()
     rho = torch.zeros(tuple(int(s) for s in shape), dtype=torch.float64, device=c.device)
+    if plan.nosplit_len:
+        step = max(1, min(nb, _LIB_FFT_BUDGET_BYTES // (2 * nk * rho.numel() * c.element_size())))
+        for b0 in range(0, nb, step):
+            psi = _to_real_library(plan, c[:, b0:b0 + step])
+            rho += torch.einsum("kb,kbxyz->xyz", w[:, b0:b0 + step],
+                                psi.real.double() ** 2 + psi.imag.double() ** 2)
+        return rho / volume
     info = (C.c_int * 11)()
     L.check(L.lib().gw_plan_info(plan.handle, prec, info, 11), "gw_plan_info")
     if info[2] == 2:

@@ -90,21 +90,43 @@ def test_boxes_without_codelet_split_match_oracle(shape, path):
     assert _rel(parts, want_all) < 1e-12
 
 
-def test_box_without_codelet_split_refuses_box_fft_entry_points():
+def test_box_without_codelet_split_real_space_and_refusals():
+    """psi(r) (core/batch.py:208-216) and density_b (batch.py:562-587) on a
+    125 x 24 x 30 box through the plan's scatter and rocFFT, against the
+    oracle's; the V(r) adjoint and ragged batches refuse the box."""
+    import ctypes as C
+
     from gradwave_amd import batching as B
     from gradwave_amd.hamiltonian import density_b
+    from oracle.hamiltonian import density_b as ref_density
+    from oracle.hamiltonian import g_to_r
 
+    say("psi(r), density and refusals on a 125x24x30 box")
     s = _si2((125, 24, 30))
     cpu, gpu, h = _gpu_h(s)
-    c = rand_block(cpu.nk, 2, cpu.npw_max, cpu.mask, seed=14).to(dev())
-    with pytest.raises(L.GWError, match="box length 125"):
-        h.wfc_to_real(c)
-    occ = torch.ones(cpu.nk, 2, dtype=torch.float64, device=dev())
-    kw = torch.full((cpu.nk,), 0.5, dtype=torch.float64, device=dev())
-    with pytest.raises(L.GWError, match="box FFT kernels"):
-        density_b(c, occ, kw, gpu, s.shape, s.grid.volume)
+    c = rand_block(cpu.nk, 3, cpu.npw_max, cpu.mask, seed=14)
+    psi = h.wfc_to_real(c.to(dev())).cpu()
+    for k in range(cpu.nk):
+        n = int(cpu.npw[k])
+        assert _rel(psi[k], g_to_r(c[k, :, :n], cpu.flat_idx[k, :n], s.shape)) < 1e-12
+    g = torch.Generator().manual_seed(2)
+    occ = torch.rand(cpu.nk, 3, dtype=torch.float64, generator=g)
+    kw = torch.rand(cpu.nk, dtype=torch.float64, generator=g)
+    want = ref_density(c, occ, kw, cpu, s.shape, s.grid.volume)
+    got = density_b(c.to(dev()), occ.to(dev()), kw.to(dev()), gpu, s.shape, s.grid.volume).cpu()
+    assert _rel(got, want) < 1e-12
+    got32 = density_b(c.to(torch.complex64).to(dev()), occ.to(dev()), kw.to(dev()), gpu, s.shape,
+                      s.grid.volume).cpu()
+    assert _rel(got32, want) < 2e-5
     with pytest.raises(L.GWError, match="ragged batch"):
         B.RaggedBatch([h])
+    cd = c.to(dev())
+    gv = torch.zeros(s.shape, dtype=torch.float64, device=dev())
+    ws = torch.empty(256, dtype=torch.uint8, device=dev())
+    rc = L.lib().gw_hpsi_adjoint_v(h.plan.handle, L.GW_FP64, cd.data_ptr(), cd.stride(1), cd.stride(0),
+                                   cd.data_ptr(), cd.stride(1), cd.stride(0), 3, gv.data_ptr(), ws.data_ptr(),
+                                   C.c_size_t(ws.numel()), L.stream_ptr(dev()))
+    assert rc != 0 and "box FFT kernels" in L.lib().gw_last_error().decode()
 
 
 @pytest.mark.parametrize("path", ["toeplitz", "library_fft"])
