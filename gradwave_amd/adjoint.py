@@ -28,6 +28,8 @@ per-band path.
 
 from __future__ import annotations
 
+import ctypes as C
+
 from types import SimpleNamespace
 
 import torch
@@ -76,6 +78,23 @@ class HOperator:
         self.shape = self.fwd.shape
 
 
+def _adjoint_v_real_space(op: HOperator, c: torch.Tensor, g: torch.Tensor, gv: torch.Tensor) -> torch.Tensor:
+    """The V(r) adjoint on boxes without the wave-line adjoint kernel: psi(r) of
+    both blocks from the operator's own transform (the generic box FFT kernels,
+    or the plan's scatter and rocFFT when a length has no codelet split), in
+    band chunks, the sum in float64."""
+    from gradwave_amd.hamiltonian import _LIB_FFT_BUDGET_BYTES
+
+    nk, nb = int(c.shape[0]), int(c.shape[1])
+    n = gv.numel()
+    step = max(1, min(nb, _LIB_FFT_BUDGET_BYTES // (4 * nk * n * c.element_size())))
+    for b0 in range(0, nb, step):
+        pc = op.fwd.wfc_to_real(c[:, b0:b0 + step].contiguous()).to(torch.complex128)
+        pg = op.fwd.wfc_to_real(g[:, b0:b0 + step].contiguous()).to(torch.complex128)
+        gv += (pg.conj() * pc).real.sum(dim=(0, 1))
+    return gv.div_(n)
+
+
 def adjoint_v(op: HOperator, c: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
     """(1/N) sum_{k,b} Re[conj(N ifftn g) (N ifftn c)] on the natural box, float64."""
     plan = op.fwd.plan
@@ -87,6 +106,10 @@ def adjoint_v(op: HOperator, c: torch.Tensor, g: torch.Tensor) -> torch.Tensor:
     gv = torch.zeros(op.shape, dtype=torch.float64, device=c.device)
     if nb == 0:
         return gv
+    info = (C.c_int * 3)()
+    L.check(L.lib().gw_plan_info(plan.handle, prec, info, 3), "gw_plan_info")
+    if info[2] != 2:  # no wave-line path on this box: transform to r and sum
+        return _adjoint_v_real_space(op, c, g, gv)
     need = int(L.lib().gw_hpsi_adjoint_v_workspace_bytes(plan.handle, nb, prec))
     ws = _scratch(c.device, need, "adjv")
     L.check(L.lib().gw_hpsi_adjoint_v(plan.handle, prec, c.data_ptr(), c.stride(1), c.stride(0),

@@ -127,9 +127,9 @@ class BatchedHamiltonian:
     (M built by gw_toeplitz_build, applied by the MFMA inner product) when the
     matrix fits the memory cap, else scatter -> rocFFT inverse -> x V ->
     rocFFT forward -> gather (torch.fft on the GPU; the scatter and gather are
-    the plan's kernels). psi(r) and the density take the scatter and rocFFT
-    there too; the V(r) adjoint (forces) and ragged batches refuse such boxes
-    with GWError.
+    the plan's kernels). psi(r), the density and the V(r) adjoint of the
+    forces take the scatter and rocFFT there too; ragged batches refuse such
+    boxes with GWError.
     """
 
     def __init__(self, bk, shape, v_eff_r: torch.Tensor, p: torch.Tensor, hub_q=None,
