@@ -36,9 +36,12 @@ template <typename T, int N, int M, int S> struct WaveLine {
   static_assert(M * S == N, "N = M*S");
   static_assert(M <= S, "pass 2 must have no more tasks than lanes");
   static constexpr int LW = 64 / S;
-  // padded k stride of the exchange area (one more for FP64: measured fewer LDS
-  // bank conflicts in the plane kernels, no change for FP32)
-  static constexpr int EK = M + (sizeof(T) == 8 ? 2 : 1);
+  // padded k stride of the exchange area. A 16-byte FP64 element fills 4 of the
+  // 64 four-byte banks, so 16 element slots repeat: with an even stride the
+  // line stride S*EK is a multiple of 4 slots (of 16 when S*EK = 144) and the
+  // LW lines of a wave collide; the smallest odd EK > M spreads them (at most
+  // 2-way). FP32 elements (8 bytes, 32 slots) keep M + 1.
+  static constexpr int EK = sizeof(T) == 8 ? (M % 2 == 0 ? M + 1 : M + 2) : M + 1;
   static constexpr int E_PER_WAVE = LW * S * EK;
   // gcd(M, S) = 1: Good-Thomas split, no twiddles between the passes
   static constexpr bool PFA = M > 1 && gcd_c(M, S) == 1;
