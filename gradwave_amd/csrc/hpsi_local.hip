@@ -1090,6 +1090,12 @@ __device__ __forceinline__ void gather_points(const cpx<T>* __restrict__ cb, con
 // plane's geometry (stick map, point indices, twiddles) is set up once, and
 // while one band's transforms run the next band's points are already loaded
 // and the previous band's stores drain.
+// LDS row stride of the wave-line plane kernels' stick rows: N + 1 points (odd,
+// N is even), so rows of different sticks start in different bank slots for
+// 16-byte FP64 and 8-byte FP32 elements alike (a stride of N = 120 repeats
+// every 2 rows in FP64 slots). The padding point is never read.
+template <int N> constexpr int kStickRowStride = N + 1;
+
 struct PlaneItem {
   int a, k, b0, b1;
 };
@@ -1110,11 +1116,12 @@ k_plane_inv_w(Geom g, long total, int bpw, const cpx<T>* __restrict__ c, long ld
               cpx<T>* __restrict__ box, const cpx<T>* __restrict__ twg) {
   using W = WaveLine<T, N, M, S>;
   constexpr int LW = W::LW;
+  constexpr int RS = kStickRowStride<N>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* E = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* tw = E + NW * W::E_PER_WAVE;
   cpx<T>* rows = tw + N;
-  int* map = reinterpret_cast<int*>(rows + (long)g.st_max_plane * N);
+  int* map = reinterpret_cast<int*>(rows + (long)g.st_max_plane * RS);
   const long wid = xcd_work_id(total);
   if (wid < 0) return;
   const PlaneItem it = plane_item(wid, g.na, g.nb, bpw);
@@ -1159,12 +1166,12 @@ k_plane_inv_w(Geom g, long total, int bpw, const cpx<T>* __restrict__ c, long ld
   const int b_end = LOOP ? it.b1 : it.b0 + 1;  // one band per item unless LOOP
   for (int b = it.b0; b < b_end; ++b) {
     const cpx<T>* cb = ck + (long)b * ldc;
-    for (int e = threadIdx.x; e < ns * N; e += blockDim.x) rows[e] = czero<T>();
+    for (int e = threadIdx.x; e < ns * RS; e += blockDim.x) rows[e] = czero<T>();
     __syncthreads();
     // point (y, z) of the plane -> stick row map[y], column z
 #pragma unroll
     for (int i = 0; i < KB; ++i)
-      if (q0 + i * NT < p1) rows[map[pv[i] / N] * N + pv[i] % N] = cpx<T>{pr[i], pim[i]};
+      if (q0 + i * NT < p1) rows[map[pv[i] / N] * RS + pv[i] % N] = cpx<T>{pr[i], pim[i]};
     for (int qq = q0 + KB * NT; qq < p1; qq += KB * NT) {  // planes of more than KB * NT points
       int v2[KB], s2[KB];
 #pragma unroll
@@ -1177,7 +1184,7 @@ k_plane_inv_w(Geom g, long total, int bpw, const cpx<T>* __restrict__ c, long ld
       gather_points<T, KB>(cb, s2, r2, i2);
 #pragma unroll
       for (int i = 0; i < KB; ++i)
-        if (qq + i * NT < p1) rows[map[v2[i] / N] * N + v2[i] % N] = cpx<T>{r2[i], i2[i]};
+        if (qq + i * NT < p1) rows[map[v2[i] / N] * RS + v2[i] % N] = cpx<T>{r2[i], i2[i]};
     }
     __syncthreads();
     if (LOOP && b + 1 < b_end) gather_points<T, KB>(cb + ldc, si, pr, pim);  // the next band's first batch
@@ -1186,11 +1193,11 @@ k_plane_inv_w(Geom g, long total, int bpw, const cpx<T>* __restrict__ c, long ld
       const int row = r0 + l;
       const bool ok = part && row < ns;
 #pragma unroll
-      for (int q = 0; q < M; ++q) x[q] = ok ? rows[row * N + W::gpos(t, q)] : czero<T>();
+      for (int q = 0; q < M; ++q) x[q] = ok ? rows[row * RS + W::gpos(t, q)] : czero<T>();
       W::inverse(x, y, t, l, part, Ew, tw);
       if (t < M && row < ns) {
 #pragma unroll
-        for (int u = 0; u < S; ++u) rows[row * N + W::rpos(t, u)] = y[u];
+        for (int u = 0; u < S; ++u) rows[row * RS + W::rpos(t, u)] = y[u];
       }
     }
     __syncthreads();
@@ -1200,7 +1207,7 @@ k_plane_inv_w(Geom g, long total, int bpw, const cpx<T>* __restrict__ c, long ld
       const int col = c0 + l;
       const bool ok = col < N;
 #pragma unroll
-      for (int q = 0; q < M; ++q) x[q] = (ok && st[q] >= 0) ? rows[st[q] * N + col] : czero<T>();
+      for (int q = 0; q < M; ++q) x[q] = (ok && st[q] >= 0) ? rows[st[q] * RS + col] : czero<T>();
       W::inverse(x, y, t, l, part, Ew, tw);
       if (t < M && col < N) {
 #pragma unroll
@@ -1219,11 +1226,12 @@ k_plane_fwd_w(Geom g, long total, int bpw, const cpx<T>* __restrict__ box, const
               const cpx<T>* __restrict__ twg) {
   using W = WaveLine<T, N, M, S>;
   constexpr int LW = W::LW;
+  constexpr int RS = kStickRowStride<N>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cpx<T>* E = reinterpret_cast<cpx<T>*>(smem_raw);
   cpx<T>* tw = E + NW * W::E_PER_WAVE;
   cpx<T>* rows = tw + N;
-  int* map = reinterpret_cast<int*>(rows + (long)g.st_max_plane * N);
+  int* map = reinterpret_cast<int*>(rows + (long)g.st_max_plane * RS);
   const long wid = xcd_work_id(total);
   if (wid < 0) return;
   const PlaneItem it = plane_item(wid, g.na, g.nb, bpw);
@@ -1303,7 +1311,7 @@ k_plane_fwd_w(Geom g, long total, int bpw, const cpx<T>* __restrict__ box, const
 #pragma unroll
         for (int q = 0; q < M; ++q) {
           const int sq = HOIST ? st[HOIST ? q : 0] : map[W::gpos(t, q)];
-          if (sq >= 0) rows[sq * N + col] = x[q];
+          if (sq >= 0) rows[sq * RS + col] = x[q];
         }
       }
     }
@@ -1312,17 +1320,17 @@ k_plane_fwd_w(Geom g, long total, int bpw, const cpx<T>* __restrict__ box, const
       const int row = r0 + l;
       const bool ok = row < ns;
 #pragma unroll
-      for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? rows[row * N + W::rpos(t, u)] : czero<T>();
+      for (int u = 0; u < S; ++u) y[u] = (ok && t < M) ? rows[row * RS + W::rpos(t, u)] : czero<T>();
       W::forward(y, x, t, l, part, Ew, tw);
       if (ok && part) {
 #pragma unroll
-        for (int q = 0; q < M; ++q) rows[row * N + W::gpos(t, q)] = x[q];
+        for (int q = 0; q < M; ++q) rows[row * RS + W::gpos(t, q)] = x[q];
       }
     }
     __syncthreads();
     // gather: point (y, z) of the plane sits in stick row map[y], column z
     auto put = [&](int si, int v, cpx<T> kin) {
-      cpx<T> r = rows[map[v / N] * N + v % N];
+      cpx<T> r = rows[map[v / N] * RS + v % N];
       r.x = r.x * scale + kin.x;
       r.y = r.y * scale + kin.y;
       ob[si] = r;
@@ -1364,7 +1372,8 @@ template <typename T, int N, int NW>
 static size_t plane_w_lds(const Plan& P) {
   using C = WaveCfg<N>;
   using W = WaveLine<T, N, C::M, C::S>;
-  return (size_t)(NW * W::E_PER_WAVE + N + (size_t)std::max(1, P.st_max_plane) * N) * sizeof(cpx<T>) +
+  return (size_t)(NW * W::E_PER_WAVE + N + (size_t)std::max(1, P.st_max_plane) * kStickRowStride<N>) *
+             sizeof(cpx<T>) +
          N * sizeof(int);
 }
 
